@@ -1,0 +1,143 @@
+"""FP64 DEM step throughput on one MI355X (particle-steps/s).
+
+Workload (default): pure DEM, 1,000,000 spheres, radius lognormal (median
+1 mm, sigma 0.2, clipped to [0.5, 2] mm), density 2500 kg/m^3, kN = E * median
+radius with E = 1e7 Pa, mu 0.5, restitution 0.5, seeded random sequential
+insertion into a 200 x 200 mm hopper (floor + 4 walls inclined 10 degrees),
+gravity -9.81 z, dt = 5e-7 s, cell = 2.5 x max radius (the reference's grid).
+A step is the reference's pure-DEM loop body: advanceParticles,
+computeForces, finishParticleStep, all state resident in HBM.
+
+Prints one JSON line with the throughput, a roofline entry for the dominant
+kernel (the contact-law kernel, timed with HIP events on the solver's stream)
+and a CPU baseline (the CPU oracle, single thread, on a bounded sample of the
+same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def law_bytes_per_particle(cbar: float, walls: int) -> float:
+    """Algorithmic HBM bytes of the contact-law kernel per particle: own state
+    read (24 f64 + 7 i32 = 220 B), contact list count (4 B), history masks
+    (PP/PW occupancy + seen words read and written, 48 B), outputs (27 f64 =
+    216 B), wall parameters are cached; per stored contact 4 B list entry,
+    4 B key and 72 B history values read + 72 B written (152 B)."""
+    return 220.0 + 4.0 + 48.0 + 216.0 + 152.0 * cbar
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mesh-scale", type=float, default=0.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the DEM step runs on a single device)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import dem_mbd_cuda_amd as dm
+    from dem_mbd_cuda_amd import scene
+
+    t0 = time.time()
+    sc = scene.hopper_scene(n=args.n, seed=2, mesh_scale=args.mesh_scale)
+    print(f"# scene built in {time.time() - t0:.1f}s, {sc.n} spheres", file=sys.stderr, flush=True)
+    dt = sc.dt
+    s = dm.CudaDemSolver()
+    s.initialize(sc)
+    s.compute_forces(collect=True)
+    s.init_half_step(0.5 * dt)
+    for _ in range(args.warmup):
+        s.advance(dt)
+        s.compute_forces(collect=False)
+        s.finish(dt)
+    s.synchronize()
+    print(f"# warmup done", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        s.advance(dt)
+        s.compute_forces(collect=False)
+        s.finish(dt)
+    s.synchronize()
+    t2 = time.perf_counter()
+    elapsed = t2 - t1
+    value = sc.n * args.steps / elapsed
+    # kernel timing (HIP events on the solver's stream) over a few more steps
+    law, scan, grid = [], [], []
+    for _ in range(5):
+        s.advance(dt)
+        s.compute_forces(collect=True)
+        s.finish(dt)
+        tm = s.last_timings()
+        grid.append(tm[0]); scan.append(tm[1]); law.append(tm[2])
+    st = s.stats()
+    cbar = st["particle_contacts"] / sc.n
+    law_ms = float(np.mean(law))
+    bytes_law = law_bytes_per_particle(cbar, len(sc.wall_origin)) * sc.n
+    achieved = bytes_law / (law_ms * 1e-3) / 1e9
+    print(f"# ms/step {1e3 * elapsed / args.steps:.3f}; grid {np.mean(grid):.3f} ms, scan "
+          f"{np.mean(scan):.3f} ms, law {law_ms:.3f} ms; contacts/particle {cbar:.3f}",
+          file=sys.stderr, flush=True)
+    s.close()
+
+    cpu = None
+    if not args.no_cpu:
+        from dem_mbd_cuda_amd import _abi, host
+
+        lib = _abi.Library(os.path.join(ROOT, "oracle", "liboracle.so"), "ora_")
+        o = lib.solver()
+        o.initialize(sc)
+        c0 = time.perf_counter()
+        host.run_pure(o, sc, steps=args.cpu_steps)
+        c1 = time.perf_counter()
+        o.close()
+        # run_pure does one extra initial force evaluation
+        cpu = {"value": sc.n * args.cpu_steps / (c1 - c0), "unit": "particle-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"CPU oracle (C restatement of the reference law, 1 thread) on the same "
+                         f"{sc.n}-sphere scene, initial force evaluation + {args.cpu_steps} steps"}
+
+    out = {
+        "metric": "FP64 particle-steps/s @1M",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded RSA packing)",
+        "config": {"workload": "pure DEM hopper, 1M spheres lognormal r (median 1 mm, sigma 0.2), "
+                               "floor + 4 inclined walls, gravity, dt 5e-7, reference 2.5 r_max grid",
+                   "particles": sc.n, "dt": dt, "contacts_per_particle": cbar},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_law", "kernel_ms": law_ms,
+                     "bytes_per_particle": law_bytes_per_particle(cbar, len(sc.wall_origin))},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

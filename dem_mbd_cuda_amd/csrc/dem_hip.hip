@@ -1,0 +1,2032 @@
+// dem_hip.hip -- libdemhip.so: the FP64 DEM step on gfx950 behind include/dem_hip.h.
+//
+// Step pipeline (one HIP stream per solver, no host synchronisation in a
+// pure-DEM step):
+//   k_cells      component centre, integer cell, 21-bit range check, cell
+//                bounding box (wave shuffles + one atomic per block)
+//   k_keys       bbox-relative linear cell index (same order as the
+//                reference's packed 64-bit key) -> radix sort payload
+//   radix sort   stable LSD sort, 8-bit digits, passes beyond the key's
+//                significant bits are skipped on the device
+//   k_fixup      within a cell, order by input row (the reference's stable
+//                sort of an index sequence orders ties by component index)
+//   k_permute    every reorder_interval steps the whole particle SoA is
+//                gathered into cell order so every later access is coalesced
+//   k_cellstart  dense cell -> first sorted index table (when it fits)
+//   k_posr       packed (x, y, z, r) in cell order for the candidate scan
+//   k_scan       27-cell candidate scan, nine contiguous column ranges per
+//                particle, writes each particle's contact list
+//   k_law        walls, particle pairs, triangles, history, accelerations
+//   k_reduce     deterministic row sums for the body loads and diagnostics
+//
+// History is indexed by input row, so it never moves when particles are
+// reordered; occupancy and "seen this step / last step" live in bit masks
+// instead of the reference's per-slot lastSeen words.
+#include "dem_device.hpp"
+
+#include "../../include/dem_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace dem;
+
+namespace {
+
+thread_local char g_error[512];
+
+int set_error(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr, what)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return set_error(DEM_E_DEVICE, "%s: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int PP_SLOTS = DEM_PARTICLE_HISTORY_SLOTS;
+constexpr int PW_SLOTS = DEM_WALL_HISTORY_SLOTS;
+constexpr int PT_SLOTS = DEM_BODY_HISTORY_SLOTS;
+constexpr int MAX_LIST = PP_SLOTS;  // contact-list capacity per component
+constexpr int BLOCK = 256;
+
+// Double fields of the particle SoA (one array of n doubles each).
+enum DField {
+    F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_WX, F_WY, F_WZ, F_QW, F_QX, F_QY, F_QZ,
+    F_VHX, F_VHY, F_VHZ, F_WHX, F_WHY, F_WHZ, F_HQW, F_HQX, F_HQY, F_HQZ,
+    F_AX, F_AY, F_AZ, F_AWX, F_AWY, F_AWZ,
+    F_MASS, F_IX, F_IY, F_IZ, F_CR, F_BX, F_BY, F_BZ,
+    F_CPX, F_CPY, F_CPZ, F_OFX, F_OFY, F_OFZ,
+    NDF
+};
+// Int fields.
+enum IField { I_PID, I_STATUS, I_CID, I_ORIG, I_CX, I_CY, I_CZ, I_VALID, NIF };
+
+struct Soa {
+    double* d;  // [NDF][n]
+    int* i;     // [NIF][n]
+    long n;
+    __device__ __forceinline__ double& f(int field, long p) const { return d[field * n + p]; }
+    __device__ __forceinline__ int& k(int field, long p) const { return i[field * n + p]; }
+};
+
+// Per-step device scalars.
+struct Scalars {
+    int lo[3], hi[3];
+    unsigned long long dims[3];
+    unsigned long long ncells;
+    int nbits;
+    int dense;
+    int grid_overflow;
+    int list_overflow;
+    int pp_overflow, pw_overflow, pt_overflow;  // pw+pt feed the "body" counter
+    int pp_high, body_high;
+    int particle_contacts, body_contacts;
+    unsigned long long body_max_depth_bits;
+    int tri_overflow;
+    int pad;
+};
+
+struct History {
+    unsigned long long* ppOcc;  // [n]
+    unsigned long long* ppSeen; // [2][n]
+    uint32_t* ppKey;            // [n][64]
+    double* ppVal;              // [n][64][9]
+    unsigned* pwOcc;            // [n]
+    unsigned* pwSeen;           // [2][n]
+    uint32_t* pwKey;            // [n][16]
+    double* pwVal;              // [n][16][9]
+    unsigned long long* ptOcc;  // [n] (only with bodies)
+    unsigned long long* ptSeen; // [2][n]
+    uint32_t* ptKey;            // [n][64]
+    double* ptVal;              // [n][64][9]
+};
+
+struct Walls {
+    int n;
+    const double* o;  // [w][3]
+    const double* nrm;
+};
+
+struct TriSet {
+    int nt, nb;
+    const int* body;        // [nt]
+    const double* local;    // [nt][15]
+    double* world;          // [nt][15]
+    double* aabb;           // [nt][6]
+    const double* pose;     // [nb][13]
+    int entries;
+    const unsigned long long* tkey;  // sorted packCell keys
+    const int* ttri;
+};
+
+struct Outputs {
+    double* contact;  // [6][n] by physical index
+    double* wall;     // [6][n]
+    double* body;     // [6][n]
+    double* field;    // [3][n]
+    double* bodyPart; // [nb][6][n] per-particle body load (sorted index)
+    double* fzPart;   // [n]
+    double* bodyParticle;  // [nb][n_orig][6] (optional)
+    double* bodyDetail;    // [nb][n_orig][19] (optional)
+    int* regime;           // [nb][n] by physical index
+    double* micro;         // [nb][22][n] by physical index
+};
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ unsigned long long pack_cell(int x, int y, int z)
+{
+    const int bias = 1 << 20;
+    const unsigned long long mask = (1ull << 21) - 1ull;
+    return (((unsigned long long)(long long)(x + bias) & mask) << 42) |
+           (((unsigned long long)(long long)(y + bias) & mask) << 21) |
+           ((unsigned long long)(long long)(z + bias) & mask);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_min(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block-wide int reductions finished with one atomic per wave.
+__device__ __forceinline__ void atomic_add_wave(int* dst, int v)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+__device__ __forceinline__ void atomic_max_wave(int* dst, int v)
+{
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && v) atomicMax(dst, v);
+}
+
+// Component view of physical particle p (updateComponentsKernel semantics,
+// particle:116-147, folded into the consumer).
+__device__ __forceinline__ Body load_body(const Soa& S, long p, bool offs)
+{
+    Body B;
+    const double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+    B.w = rot(qw, qx, qy, qz, mk(S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p)));
+    B.wh = rot(qw, qx, qy, qz, mk(S.f(F_WHX, p), S.f(F_WHY, p), S.f(F_WHZ, p)));
+    B.mass = S.f(F_MASS, p);
+    B.r = S.f(F_CR, p);
+    B.status = S.k(I_STATUS, p);
+    B.cid = (uint32_t)S.k(I_CID, p);
+    if (offs) {
+        const V3 o = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
+        B.off = o;
+        B.pos = mk(S.f(F_CPX, p), S.f(F_CPY, p), S.f(F_CPZ, p));
+        B.vel = mk(S.f(F_VX, p) + B.w.y * o.z - B.w.z * o.y, S.f(F_VY, p) + B.w.z * o.x - B.w.x * o.z,
+                   S.f(F_VZ, p) + B.w.x * o.y - B.w.y * o.x);
+        B.vh = mk(S.f(F_VHX, p) + B.wh.y * o.z - B.wh.z * o.y,
+                  S.f(F_VHY, p) + B.wh.z * o.x - B.wh.x * o.z,
+                  S.f(F_VHZ, p) + B.wh.x * o.y - B.wh.y * o.x);
+    } else {
+        B.off = zero3();
+        B.pos = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p));
+        B.vel = mk(S.f(F_VX, p), S.f(F_VY, p), S.f(F_VZ, p));
+        B.vh = mk(S.f(F_VHX, p), S.f(F_VHY, p), S.f(F_VHZ, p));
+    }
+    return B;
+}
+
+// ------------------------------------------------------------ step kernels
+__global__ void k_step_init(Scalars* sc)
+{
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) { sc->lo[a] = INT_MAX; sc->hi[a] = INT_MIN; }
+        sc->grid_overflow = 0;
+        sc->list_overflow = 0;
+        sc->pp_high = 0;
+        sc->body_high = 0;
+        sc->particle_contacts = 0;
+        sc->body_contacts = 0;
+        sc->body_max_depth_bits = 0ull;
+        sc->tri_overflow = 0;
+    }
+}
+
+// component centre + cell + range check + bbox (grid:51-71)
+__global__ void __launch_bounds__(BLOCK) k_cells(Soa S, double inv, bool offs, Scalars* sc)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    if (p < S.n) {
+        double cx = S.f(F_X, p), cy = S.f(F_Y, p), cz = S.f(F_Z, p);
+        if (offs) {
+            const V3 o = rot(S.f(F_QW, p), S.f(F_QX, p), S.f(F_QY, p), S.f(F_QZ, p),
+                             mk(S.f(F_BX, p), S.f(F_BY, p), S.f(F_BZ, p)));
+            S.f(F_OFX, p) = o.x; S.f(F_OFY, p) = o.y; S.f(F_OFZ, p) = o.z;
+            cx = cx + o.x; cy = cy + o.y; cz = cz + o.z;
+            S.f(F_CPX, p) = cx; S.f(F_CPY, p) = cy; S.f(F_CPZ, p) = cz;
+        }
+        const double fx = floor(cx * inv), fy = floor(cy * inv), fz = floor(cz * inv);
+        const double minCell = -(double)(1 << 20), maxCell = (double)((1 << 20) - 1);
+        const bool ok = isfinite(fx) && isfinite(fy) && isfinite(fz) && fx >= minCell &&
+                        fx <= maxCell && fy >= minCell && fy <= maxCell && fz >= minCell &&
+                        fz <= maxCell;
+        if (ok) {
+            const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+            S.k(I_CX, p) = ix; S.k(I_CY, p) = iy; S.k(I_CZ, p) = iz;
+            lo[0] = hi[0] = ix; lo[1] = hi[1] = iy; lo[2] = hi[2] = iz;
+        } else {
+            atomicAdd(&sc->grid_overflow, 1);
+        }
+        S.k(I_VALID, p) = ok ? 1 : 0;
+    }
+    for (int a = 0; a < 3; ++a) {
+        const int l = wave_min(lo[a]), h = wave_max(hi[a]);
+        if ((threadIdx.x & 63) == 0 && l <= h) {
+            atomicMin(&sc->lo[a], l);
+            atomicMax(&sc->hi[a], h);
+        }
+    }
+}
+
+__global__ void k_grid_setup(Scalars* sc, unsigned long long cap)
+{
+    if (threadIdx.x != 0) return;
+    unsigned long long nc = 1ull;
+    bool empty = false;
+    for (int a = 0; a < 3; ++a) {
+        if (sc->lo[a] > sc->hi[a]) { empty = true; sc->lo[a] = 0; sc->hi[a] = 0; }
+        sc->dims[a] = (unsigned long long)((long long)sc->hi[a] - sc->lo[a] + 1);
+        nc *= sc->dims[a];
+    }
+    (void)empty;
+    sc->ncells = nc;
+    const unsigned long long top = nc - 1ull;
+    sc->nbits = top == 0ull ? 0 : 64 - __clzll((long long)top);
+    sc->dense = (nc + 1ull <= cap) ? 1 : 0;
+}
+
+__device__ __forceinline__ unsigned long long rel_cell(const Scalars* sc, int cx, int cy, int cz)
+{
+    return ((unsigned long long)(cx - sc->lo[0]) * sc->dims[1] + (unsigned long long)(cy - sc->lo[1])) *
+               sc->dims[2] +
+           (unsigned long long)(cz - sc->lo[2]);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_keys(Soa S, const Scalars* sc, unsigned long long* key,
+                                                int* val)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n) return;
+    key[p] = S.k(I_VALID, p) ? rel_cell(sc, S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p)) : 0ull;
+    val[p] = (int)p;
+}
+
+// ------------------------------------------------------------ radix sort
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = BLOCK * RS_ITEMS;
+
+// A pass over bits [shift, shift+8) runs only if the pair of passes it belongs
+// to is needed: pairs keep the executed pass count even, so the sorted data
+// always ends in the first buffer.
+__device__ __forceinline__ bool pass_active(const int* nbits, int shift)
+{
+    return (shift & ~15) < *nbits;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_rs_hist(const unsigned long long* key, long n, int shift,
+                                                   const int* nbits, unsigned* counts, int nblocks)
+{
+    if (!pass_active(nbits, shift)) return;
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const long base = (long)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const long e = base + (long)r * BLOCK + threadIdx.x;
+        if (e < n) atomicAdd(&h[(key[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(long)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of the [256][nblocks] digit-major counts, one block.
+__global__ void __launch_bounds__(1024) k_rs_scan(unsigned* counts, long total, int shift,
+                                                  const int* nbits)
+{
+    if (!pass_active(nbits, shift)) return;
+    __shared__ unsigned part[1024];
+    const long per = (total + 1023) / 1024;
+    const long b = threadIdx.x * per, e = min(total, b + per);
+    unsigned s = 0;
+    for (long i = b; i < e; ++i) s += counts[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned run = part[threadIdx.x] - s;
+    for (long i = b; i < e; ++i) {
+        const unsigned c = counts[i];
+        counts[i] = run;
+        run += c;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_rs_scatter(const unsigned long long* kin, const int* vin,
+                                                      unsigned long long* kout, int* vout, long n,
+                                                      int shift, const int* nbits,
+                                                      const unsigned* counts, int nblocks)
+{
+    if (!pass_active(nbits, shift)) return;
+    __shared__ unsigned base[256];
+    __shared__ unsigned wc[BLOCK / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    base[tid] = counts[(long)tid * nblocks + blockIdx.x];
+    const long tile = (long)blockIdx.x * RS_TILE;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const long e = tile + (long)r * BLOCK + tid;
+        const bool valid = e < n;
+        const unsigned long long k = valid ? kin[e] : 0ull;
+        const unsigned d = (unsigned)((k >> shift) & 255u);
+        unsigned long long peers = __ballot(valid);
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (d >> bit) & 1u;
+            const unsigned long long b = __ballot(valid && on);
+            peers &= on ? b : ~b;
+        }
+        const int rank = __popcll(peers & below);
+        for (int w = 0; w < BLOCK / 64; ++w) wc[w][tid] = 0;
+        __syncthreads();
+        if (valid && rank == 0) wc[wave][d] = (unsigned)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            unsigned pos = base[d] + (unsigned)rank;
+            for (int w = 0; w < wave; ++w) pos += wc[w][d];
+            kout[pos] = k;
+            vout[pos] = vin[e];
+        }
+        __syncthreads();
+        unsigned add_ = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) add_ += wc[w][tid];
+        base[tid] += add_;
+        __syncthreads();
+    }
+}
+
+// Within each run of equal keys, order the payload by input row.
+__global__ void __launch_bounds__(BLOCK) k_fixup(const unsigned long long* key, int* val, long n,
+                                                 const int* orig)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= n) return;
+    const unsigned long long k = key[t];
+    if (t > 0 && key[t - 1] == k) return;
+    long e = t + 1;
+    while (e < n && key[e] == k) ++e;
+    for (long a = t + 1; a < e; ++a) {
+        const int v = val[a];
+        const int o = orig[v];
+        long b = a - 1;
+        while (b >= t && orig[val[b]] > o) {
+            val[b + 1] = val[b];
+            --b;
+        }
+        val[b + 1] = v;
+    }
+}
+
+// Gather every particle field into sorted order (new physical order).
+__global__ void __launch_bounds__(BLOCK) k_permute(Soa src, Soa dst, int* sorted)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= src.n) return;
+    const long p = sorted[t];
+    for (int f = 0; f < NDF; ++f) dst.f(f, t) = src.f(f, p);
+    for (int f = 0; f < NIF; ++f) dst.k(f, t) = src.k(f, p);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_identity(int* sorted, long n)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t < n) sorted[t] = (int)t;
+}
+
+__device__ __forceinline__ long lower_bound_u64(const unsigned long long* k, long n,
+                                                unsigned long long key)
+{
+    long first = 0, length = n;
+    while (length > 0) {
+        const long half = length >> 1, mid = first + half;
+        if (k[mid] < key) { first = mid + 1; length -= half + 1; }
+        else length = half;
+    }
+    return first;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cellstart(const unsigned long long* key, long n,
+                                                     const Scalars* sc, unsigned* start,
+                                                     unsigned long long cap)
+{
+    if (!sc->dense) return;
+    const unsigned long long nc = sc->ncells;
+    for (unsigned long long c = (unsigned long long)blockIdx.x * BLOCK + threadIdx.x; c <= nc && c < cap;
+         c += (unsigned long long)gridDim.x * BLOCK)
+        start[c] = (unsigned)lower_bound_u64(key, n, c);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool offs, double4* posr)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= S.n) return;
+    const long p = sorted[t];
+    double4 v;
+    if (offs) { v.x = S.f(F_CPX, p); v.y = S.f(F_CPY, p); v.z = S.f(F_CPZ, p); }
+    else { v.x = S.f(F_X, p); v.y = S.f(F_Y, p); v.z = S.f(F_Z, p); }
+    v.w = S.f(F_CR, p);
+    posr[t] = v;
+}
+
+// 27-cell candidate scan -> contact list (particleContact's overlap test,
+// contact:579-591, evaluated in the reference's cell and entry order).
+__global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const double4* posr,
+                                                const unsigned long long* key, const unsigned* start,
+                                                const Scalars* sc, int* nbr, int* ncount)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long n = S.n;
+    if (t >= n) return;
+    const long p = sorted[t];
+    int count = 0;
+    if (S.k(I_VALID, p)) {
+        const double4 me = posr[t];
+        const long X = (long)sc->dims[0], Y = (long)sc->dims[1], Z = (long)sc->dims[2];
+        const long cx = S.k(I_CX, p) - sc->lo[0], cy = S.k(I_CY, p) - sc->lo[1],
+                   cz = S.k(I_CZ, p) - sc->lo[2];
+        const bool dense = sc->dense;
+        const long zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < Z ? cz + 1 : Z - 1;
+        for (long dx = -1; dx <= 1; ++dx) {
+            const long xx = cx + dx;
+            if (xx < 0 || xx >= X) continue;
+            for (long dy = -1; dy <= 1; ++dy) {
+                const long yy = cy + dy;
+                if (yy < 0 || yy >= Y) continue;
+                const unsigned long long col = ((unsigned long long)xx * Y + yy) * Z;
+                long b, e;
+                if (dense) {
+                    b = start[col + zlo];
+                    e = start[col + zhi + 1];
+                } else {
+                    b = lower_bound_u64(key, n, col + zlo);
+                    e = lower_bound_u64(key, n, col + zhi + 1);
+                }
+                for (long k = b; k < e; ++k) {
+                    if (k == t) continue;
+                    const double4 o = posr[k];
+                    const double dx_ = o.x - me.x, dy_ = o.y - me.y, dz_ = o.z - me.z;
+                    const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+                    const double rs = me.w + o.w;
+                    if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
+                    if (count < MAX_LIST) nbr[(long)count * n + t] = sorted[k];
+                    ++count;
+                }
+            }
+        }
+    }
+    ncount[t] = count;
+}
+
+// History slot acquisition (acquireHistorySlot, contact:190-215) on an
+// occupancy mask.  Returns the slot or -1 (overflow).
+template <typename M>
+__device__ __forceinline__ int acquire(M& occ, M& cur, uint32_t* keys, double* vals, int slots,
+                                       uint32_t key, int& high, int& overflow)
+{
+    M m = occ;
+    while (m) {
+        const int b = (int)__ffsll((unsigned long long)m) - 1;
+        if (keys[b] == key) {
+            cur |= (M)1 << b;
+            high = max(high, b + 1);
+            return b;
+        }
+        m &= m - 1;
+    }
+    const M full = slots >= (int)(8 * sizeof(M)) ? ~(M)0 : (((M)1 << slots) - 1);
+    const M freem = ~occ & full;
+    if (!freem) {
+        ++overflow;
+        return -1;
+    }
+    const int b = (int)__ffsll((unsigned long long)freem) - 1;
+    occ |= (M)1 << b;
+    cur |= (M)1 << b;
+    keys[b] = key;
+    double* v = vals + (long)b * 9;
+    for (int q = 0; q < 9; ++q) v[q] = 0.0;
+    high = max(high, b + 1);
+    return b;
+}
+
+struct LawArgs {
+    Soa S;
+    const int* sorted;
+    const int* nbr;
+    const int* ncount;
+    History H;
+    Walls W;
+    TriSet T;
+    Outputs O;
+    Scalars* sc;
+    Material pp, pt, pw;
+    double g[3];
+    int gravity;
+    int universal;
+    const int* byorig;
+    int step;
+    bool offs;
+    int nb;
+};
+
+__device__ __forceinline__ Tri load_tri(const TriSet& T, int t)
+{
+    const double* w = T.world + (long)t * 15;
+    Tri r;
+    r.a = mk(w[3], w[4], w[5]);
+    r.b = mk(w[6], w[7], w[8]);
+    r.c = mk(w[9], w[10], w[11]);
+    r.n = mk(w[12], w[13], w[14]);
+    r.body = T.body[t];
+    return r;
+}
+
+__device__ __forceinline__ Pose load_pose(const TriSet& T, int b)
+{
+    const double* q = T.pose + (long)b * 13;
+    Pose P;
+    P.c = mk(q[0], q[1], q[2]);
+    P.qw = q[3]; P.qx = q[4]; P.qy = q[5]; P.qz = q[6];
+    P.v = mk(q[7], q[8], q[9]);
+    P.w = rot(q[3], q[4], q[5], q[6], mk(q[10], q[11], q[12]));
+    return P;
+}
+
+__device__ __forceinline__ long upper_bound_u64(const unsigned long long* k, long n,
+                                                unsigned long long key)
+{
+    long first = 0, length = n;
+    while (length > 0) {
+        const long half = length >> 1, mid = first + half;
+        if (key < k[mid]) length = half;
+        else { first = mid + 1; length -= half + 1; }
+    }
+    return first;
+}
+
+// computeContactsKernel (contact:1065-1221) for one owner, plus the
+// acceleration (particle:71-101).
+__global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const Soa& S = A.S;
+    const long n = S.n;
+    int pcount = 0, ppHigh = 0, bodyHigh = 0, ppOver = 0, pwOver = 0, ptOver = 0, bcount = 0;
+    if (t < n) {
+        const long p = A.sorted[t];
+        const long o = S.k(I_ORIG, p);
+        const int s = A.step;
+        const Body I = load_body(S, p, A.offs);
+        V3 cF = zero3(), cT = zero3(), fF = zero3(), wF = zero3(), wT = zero3(), bF = zero3(),
+           bT = zero3();
+        if (A.gravity) fF = mk(I.mass * A.g[0], I.mass * A.g[1], I.mass * A.g[2]);
+        if (A.universal) {
+            const V3 pi = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p));
+            for (long oj = 0; oj < n; ++oj) {
+                const long j = A.byorig[oj];
+                if (j == p) continue;
+                const V3 d = sub(pi, mk(S.f(F_X, j), S.f(F_Y, j), S.f(F_Z, j)));
+                const double d2 = dot(d, d);
+                if (d2 > 1.0e-28)
+                    fF = add(fF, scl(d, kGravityConstant * I.mass * S.f(F_MASS, j) / (d2 * sqrt(d2))));
+            }
+        }
+        // expiry (expireHistory, contact:217-229): drop slots not seen last step
+        const History& H = A.H;
+        unsigned long long ppOcc = H.ppOcc[o];
+        ppOcc &= H.ppSeen[(long)((s + 1) & 1) * n + o];
+        unsigned long long ppCur = 0;
+        unsigned pwOcc = H.pwOcc[o] & H.pwSeen[(long)((s + 1) & 1) * n + o];
+        unsigned pwCur = 0;
+        // walls
+        for (int w = 0; w < A.W.n; ++w) {
+            const V3 outward = mk(A.W.nrm[3 * w], A.W.nrm[3 * w + 1], A.W.nrm[3 * w + 2]);
+            const V3 origin = mk(A.W.o[3 * w], A.W.o[3 * w + 1], A.W.o[3 * w + 2]);
+            const double distance = dot(sub(I.pos, origin), outward);
+            const double depth = I.r - distance;
+            if (depth <= 0.0) continue;
+            const int slot = acquire(pwOcc, pwCur, H.pwKey + o * PW_SLOTS, H.pwVal + o * PW_SLOTS * 9,
+                                     PW_SLOTS, (uint32_t)(w + 1), bodyHigh, pwOver);
+            if (slot < 0) continue;
+            pw_law(I, outward, depth, A.pw, H.pwVal + (o * PW_SLOTS + slot) * 9, wF, wT);
+        }
+        // particle pairs, in candidate order
+        const int cnt = min(A.ncount[t], MAX_LIST);
+        for (int c = 0; c < cnt; ++c) {
+            const long q = A.nbr[(long)c * n + t];
+            const Body J = load_body(S, q, A.offs);
+            const V3 delta = sub(J.pos, I.pos);
+            const double d2 = dot(delta, delta);
+            const int slot = acquire(ppOcc, ppCur, H.ppKey + o * PP_SLOTS, H.ppVal + o * PP_SLOTS * 9,
+                                     PP_SLOTS, J.cid + 1u, ppHigh, ppOver);
+            if (slot < 0) continue;
+            pp_law(I, J, delta, d2, A.pp, H.ppVal + (o * PP_SLOTS + slot) * 9, cF, cT);
+            ++pcount;
+        }
+        if (A.ncount[t] > MAX_LIST) ppOver += A.ncount[t] - MAX_LIST;
+        // triangles of the own cell (contact:1148-1208)
+        if (A.nb > 0) {
+            unsigned long long ptOcc = H.ptOcc[o] & H.ptSeen[(long)((s + 1) & 1) * n + o];
+            unsigned long long ptCur = 0;
+            const TriSet& T = A.T;
+            if (T.nt > 0 && T.entries > 0 && S.k(I_VALID, p)) {
+                const unsigned long long key = pack_cell(S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p));
+                const long b0 = lower_bound_u64(T.tkey, T.entries, key);
+                const long e0 = upper_bound_u64(T.tkey, T.entries, key);
+                for (int body = 0; body < A.nb; ++body) {
+                    double total = 0.0, wD = 0.0;
+                    V3 wN = zero3(), wC = zero3();
+                    for (long k = b0; k < e0; ++k) {
+                        const int tr = T.ttri[k];
+                        if (T.body[tr] != body) continue;
+                        const Tri tri = load_tri(T, tr);
+                        const double scv = tri_area_scale(I.pos, I.r, tri);
+                        if (scv <= 0.0) continue;
+                        const double distance = dot(tri.n, sub(I.pos, tri.a));
+                        const V3 cc = sub(I.pos, scl(tri.n, distance));
+                        total += scv;
+                        wN = add(wN, scl(tri.n, -scv));
+                        wC = add(wC, scl(cc, scv));
+                        wD += distance * scv;
+                    }
+                    double bl[6] = {0, 0, 0, 0, 0, 0};
+                    double fz = 0.0;
+                    if (total > 0.0) {
+                        const V3 mN = unit(wN);
+                        const V3 mC = scl(wC, 1.0 / total);
+                        const double mD = wD / total;
+                        const int slot = acquire(ptOcc, ptCur, H.ptKey + o * PT_SLOTS,
+                                                 H.ptVal + o * PT_SLOTS * 9, PT_SLOTS,
+                                                 (uint32_t)(body + 1), bodyHigh, ptOver);
+                        if (slot >= 0) {
+                            double* sv = H.ptVal + (o * PT_SLOTS + slot) * 9;
+                            const Pose P = load_pose(T, body);
+                            bool slide = false, twisty = false;
+                            manifold_law(I, P, A.pt, mN, mC, mD, sv, &slide, &twisty);
+                            A.O.regime[(long)body * n + p] = 1 | (slide ? 2 : 0) | (twisty ? 4 : 0);
+                            Material ro = A.pt;
+                            ro.dt = 0.0;
+                            for (long k = b0; k < e0; ++k) {
+                                const int tr = T.ttri[k];
+                                if (T.body[tr] != body) continue;
+                                const Tri tri = load_tri(T, tr);
+                                const PtOut r = pt_law(I, tri, P, ro, sv);
+                                if (!r.hit) continue;
+                                bF = add(bF, r.pf);
+                                bT = add(bT, r.ptq);
+                                const V3 btq = add(add(r.bnt, r.bft), add(r.btw, r.brl));
+                                bl[0] += r.bnf.x + r.bfr.x;
+                                bl[1] += r.bnf.y + r.bfr.y;
+                                bl[2] += r.bnf.z + r.bfr.z;
+                                bl[3] += btq.x;
+                                bl[4] += btq.y;
+                                bl[5] += btq.z;
+                                fz += r.bnf.z + r.bfr.z;
+                                ++bcount;
+                                atomicMax(&A.sc->body_max_depth_bits,
+                                          (unsigned long long)__double_as_longlong(r.depth));
+                                if (A.O.bodyParticle) {
+                                    double* d = A.O.bodyParticle + ((long)body * n + o) * 6;
+                                    d[0] += r.pf.x; d[1] += r.pf.y; d[2] += r.pf.z;
+                                    d[3] += r.ptq.x; d[4] += r.ptq.y; d[5] += r.ptq.z;
+                                }
+                                if (A.O.bodyDetail) {
+                                    double* d = A.O.bodyDetail + ((long)body * n + o) * 19;
+                                    const V3 gr[6] = {r.bnf, r.bfr, r.bnt, r.bft, r.btw, r.brl};
+                                    for (int g = 0; g < 6; ++g) {
+                                        d[3 * g] += gr[g].x;
+                                        d[3 * g + 1] += gr[g].y;
+                                        d[3 * g + 2] += gr[g].z;
+                                    }
+                                    if (d[18] < r.depth) d[18] = r.depth;
+                                }
+                            }
+                        }
+                    }
+                    for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * n + t] = bl[k];
+                    A.O.fzPart[(long)body * n + t] = fz;
+                }
+            } else {
+                for (int body = 0; body < A.nb; ++body) {
+                    for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * n + t] = 0.0;
+                    A.O.fzPart[(long)body * n + t] = 0.0;
+                }
+            }
+            H.ptOcc[o] = ptOcc;
+            H.ptSeen[(long)(s & 1) * n + o] = ptCur;
+        }
+        H.ppOcc[o] = ppOcc;
+        H.ppSeen[(long)(s & 1) * n + o] = ppCur;
+        H.pwOcc[o] = pwOcc;
+        H.pwSeen[(long)(s & 1) * n + o] = pwCur;
+        // outputs (contact:1211-1220)
+        double* oc = A.O.contact;
+        oc[0 * n + p] = cF.x; oc[1 * n + p] = cF.y; oc[2 * n + p] = cF.z;
+        oc[3 * n + p] = cT.x; oc[4 * n + p] = cT.y; oc[5 * n + p] = cT.z;
+        double* ow = A.O.wall;
+        ow[0 * n + p] = wF.x; ow[1 * n + p] = wF.y; ow[2 * n + p] = wF.z;
+        ow[3 * n + p] = wT.x; ow[4 * n + p] = wT.y; ow[5 * n + p] = wT.z;
+        double* ob = A.O.body;
+        ob[0 * n + p] = bF.x; ob[1 * n + p] = bF.y; ob[2 * n + p] = bF.z;
+        ob[3 * n + p] = bT.x; ob[4 * n + p] = bT.y; ob[5 * n + p] = bT.z;
+        double* of = A.O.field;
+        of[0 * n + p] = fF.x; of[1 * n + p] = fF.y; of[2 * n + p] = fF.z;
+        const V3 tF = add(add(cF, fF), add(wF, bF));
+        const V3 tT = add(cT, add(wT, bT));
+        // accelerationKernel (particle:71-101)
+        const double m = I.mass;
+        S.f(F_AX, p) = tF.x / m;
+        S.f(F_AY, p) = tF.y / m;
+        S.f(F_AZ, p) = tF.z / m;
+        const double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+        const double tx = (1.0 - 2.0 * (qy * qy + qz * qz)) * tT.x + 2.0 * (qx * qy + qw * qz) * tT.y +
+                          2.0 * (qx * qz - qw * qy) * tT.z;
+        const double ty = 2.0 * (qx * qy - qw * qz) * tT.x + (1.0 - 2.0 * (qx * qx + qz * qz)) * tT.y +
+                          2.0 * (qy * qz + qw * qx) * tT.z;
+        const double tz = 2.0 * (qx * qz + qw * qy) * tT.x + 2.0 * (qy * qz - qw * qx) * tT.y +
+                          (1.0 - 2.0 * (qx * qx + qy * qy)) * tT.z;
+        const double wx = S.f(F_WX, p), wy = S.f(F_WY, p), wz = S.f(F_WZ, p);
+        const double ix = S.f(F_IX, p), iy = S.f(F_IY, p), iz = S.f(F_IZ, p);
+        S.f(F_AWX, p) = (tx - (iz - iy) * wz * wy) / ix;
+        S.f(F_AWY, p) = (ty - (ix - iz) * wx * wz) / iy;
+        S.f(F_AWZ, p) = (tz - (iy - ix) * wy * wx) / iz;
+    }
+    atomic_add_wave(&A.sc->particle_contacts, pcount);
+    atomic_add_wave(&A.sc->body_contacts, bcount);
+    atomic_max_wave(&A.sc->pp_high, ppHigh);
+    atomic_max_wave(&A.sc->body_high, bodyHigh);
+    atomic_add_wave(&A.sc->pp_overflow, ppOver);
+    atomic_add_wave(&A.sc->pw_overflow, pwOver);
+    atomic_add_wave(&A.sc->pt_overflow, ptOver);
+}
+
+// observeBodyMicrostateKernel (contact:1226-1356), one thread per
+// (body, particle); writes [body][22][n] by physical index.
+__global__ void __launch_bounds__(BLOCK) k_observe(Soa S, History H, TriSet T, Outputs O,
+                                                   Material m, int step, bool offs)
+{
+    const long n = S.n;
+    const long idx = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= (long)T.nb * n) return;
+    const int body = (int)(idx / n);
+    const long p = idx - (long)body * n;
+    double* ms = O.micro + (long)body * 22 * n + p;
+    for (int f = 0; f < 22; ++f) ms[(long)f * n] = 0.0;
+    if ((O.regime[idx] & 1) == 0) return;
+    const Body I = load_body(S, p, offs);
+    const unsigned long long key = pack_cell(S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p));
+    const long b0 = lower_bound_u64(T.tkey, T.entries, key);
+    const long e0 = upper_bound_u64(T.tkey, T.entries, key);
+    double total = 0.0, wD = 0.0;
+    V3 wN = zero3(), wC = zero3();
+    for (long k = b0; k < e0; ++k) {
+        const int tr = T.ttri[k];
+        if (T.body[tr] != body) continue;
+        const Tri tri = load_tri(T, tr);
+        const double scv = tri_area_scale(I.pos, I.r, tri);
+        if (scv <= 0.0) continue;
+        const double distance = dot(tri.n, sub(I.pos, tri.a));
+        const V3 cc = sub(I.pos, scl(tri.n, distance));
+        total += scv;
+        wN = add(wN, scl(tri.n, -scv));
+        wC = add(wC, scl(cc, scv));
+        wD += distance * scv;
+    }
+    if (total <= 0.0) return;
+    const long o = S.k(I_ORIG, p);
+    const unsigned long long seen = H.ptSeen[(long)(step & 1) * n + o];
+    int slot = -1;
+    for (int b = 0; b < PT_SLOTS; ++b) {
+        if (((H.ptOcc[o] >> b) & 1ull) && H.ptKey[o * PT_SLOTS + b] == (uint32_t)(body + 1) &&
+            ((seen >> b) & 1ull)) {
+            slot = b;
+            break;
+        }
+    }
+    if (slot < 0) return;
+    const V3 normal = unit(wN);
+    const V3 cc = scl(wC, 1.0 / total);
+    const double distance = wD / total;
+    const Pose P = load_pose(T, body);
+    const V3 arm = sub(cc, P.c);
+    const V3 bv = add(P.v, cross(P.w, arm));
+    const V3 rv = sub(bv, I.vel);
+    const double cN = damping(m.logN, m.kN, I.mass);
+    const double radius = I.r;
+    const double brs = m.beta * radius * m.beta * radius;
+    const double depth = radius - distance;
+    V3 fn = add(scl(normal, -m.kN * depth), scl(normal, cN * dot(rv, normal)));
+    fn = add(fn, scl(normal, m.cohesion * 4.0 * brs));
+    const double fnn = len(fn);
+    const double msc = m.beta * radius * fnn;
+    const double tlim = m.mu * fnn;
+    const double wlim = m.mu * m.muT * msc;
+    const double* sv = H.ptVal + (o * PT_SLOTS + slot) * 9;
+    const V3 th = hget(sv, 1), wh = hget(sv, 2);
+    const V3 axisZ = rot(P.qw, P.qx, P.qy, P.qz, mk(0.0, 0.0, 1.0));
+    const V3 radial = sub(arm, scl(axisZ, dot(arm, axisZ)));
+    const double tu = tlim > 1.0e-30 ? fmin(1.0, len(th) / tlim) : 0.0;
+    const double wu = wlim > 1.0e-30 ? fmin(1.0, len(wh) / wlim) : 0.0;
+    const V3 btw = scl(wh, -total), bnl = scl(fn, -total), bth = scl(th, -total);
+    const V3 btt = cross(arm, bth);
+    const double v[22] = {fnn * total, fnn * radius * total, fnn * len(radial) * total,
+                          btw.x, btw.y, btw.z, len(wh) * total, wlim * total, len(th) * total,
+                          tlim * total, wu * total, tu * total, total, bnl.x, bnl.y, bnl.z,
+                          bth.x, bth.y, bth.z, btt.x, btt.y, btt.z};
+    for (int f = 0; f < 22; ++f) ms[(long)f * n] = v[f];
+}
+
+// regime counters (accumulateBodyRegimeKernel, contact:530-567): integer
+// sums, so atomics give the same result in any order.
+__global__ void __launch_bounds__(BLOCK) k_regime(const int* regime, long n, int nb,
+                                                  unsigned long long* instant,
+                                                  unsigned long long* interval, int accumulate)
+{
+    const long idx = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= (long)nb * n) return;
+    const int st = regime[idx];
+    if ((st & 1) == 0) return;
+    const int body = (int)(idx / n);
+    const bool sl = (st & 2) != 0, tw = (st & 4) != 0;
+    const int primary = tw ? 3 : (sl ? 2 : 1);
+    unsigned long long* ri = instant + body * 7;
+    atomicAdd(&ri[0], 1ull);
+    atomicAdd(&ri[primary], 1ull);
+    if (sl) atomicAdd(&ri[4], 1ull);
+    if (tw) atomicAdd(&ri[5], 1ull);
+    if (sl && tw) atomicAdd(&ri[6], 1ull);
+    if (!accumulate) return;
+    unsigned long long* rv = interval + body * 7;
+    atomicAdd(&rv[0], 1ull);
+    atomicAdd(&rv[primary], 1ull);
+    if (sl) atomicAdd(&rv[4], 1ull);
+    if (tw) atomicAdd(&rv[5], 1ull);
+    if (sl && tw) atomicAdd(&rv[6], 1ull);
+}
+
+// Deterministic row sums: partial[row][block] = tree sum of rows[row][block
+// tile]; fixed order independent of scheduling.
+constexpr int RED_TILE = 4096;
+__global__ void __launch_bounds__(BLOCK) k_reduce_rows(const double* rows, long n, int nrows,
+                                                       double* partial, int nblocks)
+{
+    const int row = blockIdx.y;
+    const double* r = rows + (long)row * n;
+    const long base = (long)blockIdx.x * RED_TILE;
+    double s = 0.0;
+    for (int k = 0; k < RED_TILE / BLOCK; ++k) {
+        const long e = base + (long)k * BLOCK + threadIdx.x;
+        if (e < n) s += r[e];
+    }
+    s = wave_sum(s);
+    __shared__ double ws[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) v += ws[w];
+        partial[(long)row * nblocks + blockIdx.x] = v;
+    }
+}
+
+__global__ void k_reduce_final(const double* partial, int nrows, int nblocks, double* out,
+                               int accumulate)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += partial[(long)row * nblocks + b];
+    if (accumulate) out[row] += s;
+    else out[row] = s;
+}
+
+// ------------------------------------------------------------ leapfrog
+__device__ __forceinline__ void half_quat(const Soa& S, long p, double h)
+{
+    double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+    double dw, dx, dy, dz;
+    quat_dot(qw, qx, qy, qz, S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p), dw, dx, dy, dz);
+    qw += dw * h; qx += dx * h; qy += dy * h; qz += dz * h;
+    quat_unit(qw, qx, qy, qz);
+    double a, b, c, d;
+    quat_dot(qw, qx, qy, qz, S.f(F_WHX, p), S.f(F_WHY, p), S.f(F_WHZ, p), a, b, c, d);
+    S.f(F_HQW, p) = a; S.f(F_HQX, p) = b; S.f(F_HQY, p) = c; S.f(F_HQZ, p) = d;
+}
+
+// initializeLeapfrogKernel (particle:185-211)
+__global__ void __launch_bounds__(BLOCK) k_init_half(Soa S, double h)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n) return;
+    S.f(F_VHX, p) = S.f(F_VX, p) + S.f(F_AX, p) * h;
+    S.f(F_VHY, p) = S.f(F_VY, p) + S.f(F_AY, p) * h;
+    S.f(F_VHZ, p) = S.f(F_VZ, p) + S.f(F_AZ, p) * h;
+    S.f(F_WHX, p) = S.f(F_WX, p) + S.f(F_AWX, p) * h;
+    S.f(F_WHY, p) = S.f(F_WY, p) + S.f(F_AWY, p) * h;
+    S.f(F_WHZ, p) = S.f(F_WZ, p) + S.f(F_AWZ, p) * h;
+    half_quat(S, p, h);
+}
+
+// advancePositionKernel (particle:213-238)
+__global__ void __launch_bounds__(BLOCK) k_advance(Soa S, double dt)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n || S.k(I_STATUS, p) == -1) return;
+    S.f(F_X, p) += S.f(F_VHX, p) * dt;
+    S.f(F_Y, p) += S.f(F_VHY, p) * dt;
+    S.f(F_Z, p) += S.f(F_VHZ, p) * dt;
+    double qw = S.f(F_QW, p) + S.f(F_HQW, p) * dt, qx = S.f(F_QX, p) + S.f(F_HQX, p) * dt,
+           qy = S.f(F_QY, p) + S.f(F_HQY, p) * dt, qz = S.f(F_QZ, p) + S.f(F_HQZ, p) * dt;
+    quat_unit(qw, qx, qy, qz);
+    S.f(F_QW, p) = qw; S.f(F_QX, p) = qx; S.f(F_QY, p) = qy; S.f(F_QZ, p) = qz;
+    S.f(F_VX, p) = S.f(F_VHX, p) + S.f(F_AX, p) * (0.5 * dt);
+    S.f(F_VY, p) = S.f(F_VHY, p) + S.f(F_AY, p) * (0.5 * dt);
+    S.f(F_VZ, p) = S.f(F_VHZ, p) + S.f(F_AZ, p) * (0.5 * dt);
+    S.f(F_WX, p) = S.f(F_WHX, p) + S.f(F_AWX, p) * (0.5 * dt);
+    S.f(F_WY, p) = S.f(F_WHY, p) + S.f(F_AWY, p) * (0.5 * dt);
+    S.f(F_WZ, p) = S.f(F_WHZ, p) + S.f(F_AWZ, p) * (0.5 * dt);
+}
+
+// advanceHalfVelocityKernel (particle:240-265)
+__global__ void __launch_bounds__(BLOCK) k_finish(Soa S, double dt)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n) return;
+    S.f(F_VHX, p) += S.f(F_AX, p) * dt;
+    S.f(F_VHY, p) += S.f(F_AY, p) * dt;
+    S.f(F_VHZ, p) += S.f(F_AZ, p) * dt;
+    S.f(F_WHX, p) += S.f(F_AWX, p) * dt;
+    S.f(F_WHY, p) += S.f(F_AWY, p) * dt;
+    S.f(F_WHZ, p) += S.f(F_AWZ, p) * dt;
+    half_quat(S, p, 0.5 * dt);
+}
+
+// ------------------------------------------------------------ triangles
+// transformTrianglesKernel (body:71-112)
+__global__ void __launch_bounds__(BLOCK) k_transform(TriSet T)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= T.nt) return;
+    const double* b = T.pose + (long)T.body[i] * 13;
+    const double* l = T.local + (long)i * 15;
+    double* w = T.world + (long)i * 15;
+    const double qw = b[3], qx = b[4], qy = b[5], qz = b[6];
+    const double r00 = 1.0 - 2.0 * (qy * qy + qz * qz), r01 = 2.0 * (qx * qy - qw * qz),
+                 r02 = 2.0 * (qx * qz + qw * qy), r10 = 2.0 * (qx * qy + qw * qz),
+                 r11 = 1.0 - 2.0 * (qx * qx + qz * qz), r12 = 2.0 * (qy * qz - qw * qx),
+                 r20 = 2.0 * (qx * qz - qw * qy), r21 = 2.0 * (qy * qz + qw * qx),
+                 r22 = 1.0 - 2.0 * (qx * qx + qy * qy);
+    for (int v = 0; v < 5; ++v) {
+        const double X = l[3 * v], Y = l[3 * v + 1], Z = l[3 * v + 2];
+        double ox = r00 * X + r01 * Y + r02 * Z, oy = r10 * X + r11 * Y + r12 * Z,
+               oz = r20 * X + r21 * Y + r22 * Z;
+        if (v < 4) { ox += b[0]; oy += b[1]; oz += b[2]; }
+        w[3 * v] = ox; w[3 * v + 1] = oy; w[3 * v + 2] = oz;
+    }
+    double* bb = T.aabb + (long)i * 6;
+    double mnx = w[0], mny = w[1], mnz = w[2], mxx = w[0], mxy = w[1], mxz = w[2];
+    for (int v = 1; v < 4; ++v) {
+        mnx = fmin(mnx, w[3 * v]); mny = fmin(mny, w[3 * v + 1]); mnz = fmin(mnz, w[3 * v + 2]);
+        mxx = fmax(mxx, w[3 * v]); mxy = fmax(mxy, w[3 * v + 1]); mxz = fmax(mxz, w[3 * v + 2]);
+    }
+    bb[0] = mnx; bb[1] = mny; bb[2] = mnz; bb[3] = mxx; bb[4] = mxy; bb[5] = mxz;
+}
+
+// countTriangleCellsKernel (contact:409-451)
+__global__ void __launch_bounds__(BLOCK) k_tri_count(TriSet T, double inv, double pad, int* counts,
+                                                     Scalars* sc)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= T.nt) return;
+    const double* bb = T.aabb + (long)i * 6;
+    const double mnx = floor((bb[0] - pad) * inv), mny = floor((bb[1] - pad) * inv),
+                 mnz = floor((bb[2] - pad) * inv), mxx = floor((bb[3] + pad) * inv),
+                 mxy = floor((bb[4] + pad) * inv), mxz = floor((bb[5] + pad) * inv);
+    const double minCell = -(double)(1 << 20), maxCell = (double)((1 << 20) - 1);
+    if (!isfinite(mnx) || !isfinite(mny) || !isfinite(mnz) || !isfinite(mxx) || !isfinite(mxy) ||
+        !isfinite(mxz) || mnx < minCell || mny < minCell || mnz < minCell || mxx > maxCell ||
+        mxy > maxCell || mxz > maxCell) {
+        atomicAdd(&sc->tri_overflow, 1);
+        counts[i] = 0;
+        return;
+    }
+    const long long nx = (long long)(int)mxx - (int)mnx + 1, ny = (long long)(int)mxy - (int)mny + 1,
+                    nz = (long long)(int)mxz - (int)mnz + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > LLONG_MAX / ny || nx * ny > LLONG_MAX / nz ||
+        nx * ny * nz > INT_MAX) {
+        atomicAdd(&sc->tri_overflow, 1);
+        counts[i] = 0;
+        return;
+    }
+    counts[i] = (int)(nx * ny * nz);
+}
+
+// fillTriangleCellsKernel (contact:453-476): x -> y -> z nested order
+__global__ void __launch_bounds__(BLOCK) k_tri_fill(TriSet T, double inv, double pad,
+                                                    const int* counts, const long long* offsets,
+                                                    unsigned long long* key, int* tri)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= T.nt) return;
+    const double* bb = T.aabb + (long)i * 6;
+    const int mnx = (int)floor((bb[0] - pad) * inv), mny = (int)floor((bb[1] - pad) * inv),
+              mnz = (int)floor((bb[2] - pad) * inv), mxx = (int)floor((bb[3] + pad) * inv),
+              mxy = (int)floor((bb[4] + pad) * inv), mxz = (int)floor((bb[5] + pad) * inv);
+    const long long out = offsets[i];
+    int local = 0;
+    const int c = counts[i];
+    for (int x = mnx; x <= mxx && local < c; ++x)
+        for (int y = mny; y <= mxy && local < c; ++y)
+            for (int z = mnz; z <= mxz && local < c; ++z) {
+                key[out + local] = pack_cell(x, y, z);
+                tri[out + local] = i;
+                ++local;
+            }
+}
+
+// ------------------------------------------------------------ downloads
+__global__ void __launch_bounds__(BLOCK) k_gather_state(Soa S, double* out /*[13][n] by row*/)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n) return;
+    const long o = S.k(I_ORIG, p);
+    const int fields[13] = {F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_WX, F_WY, F_WZ, F_QW, F_QX, F_QY, F_QZ};
+    for (int f = 0; f < 13; ++f) out[(long)f * S.n + o] = S.f(fields[f], p);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_gather_rows(Soa S, const double* src, int nrows,
+                                                       double* out)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n) return;
+    const long o = S.k(I_ORIG, p);
+    for (int r = 0; r < nrows; ++r) out[(long)r * S.n + o] = src[(long)r * S.n + p];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_gather_ints(Soa S, int field, int* out)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n) return;
+    out[S.k(I_ORIG, p)] = S.k(field, p);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_byorig(Soa S, int* byorig)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p < S.n) byorig[S.k(I_ORIG, p)] = (int)p;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill(double* a, long count, double v)
+{
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < count; i += (long)gridDim.x * BLOCK)
+        a[i] = v;
+}
+__global__ void __launch_bounds__(BLOCK) k_fill_u64(unsigned long long* a, long count)
+{
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < count; i += (long)gridDim.x * BLOCK)
+        a[i] = 0ull;
+}
+__global__ void __launch_bounds__(BLOCK) k_fill_u32(unsigned* a, long count)
+{
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < count; i += (long)gridDim.x * BLOCK)
+        a[i] = 0u;
+}
+
+// debug: reference-space grid = packed key and input row, in sorted order
+__global__ void __launch_bounds__(BLOCK) k_debug_grid(Soa S, const int* sorted,
+                                                      unsigned long long* keys, int* ids)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= S.n) return;
+    const long p = sorted[t];
+    keys[t] = S.k(I_VALID, p) ? pack_cell(S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p)) : 0ull;
+    ids[t] = S.k(I_ORIG, p);
+}
+
+int blocks_for(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
+
+}  // namespace
+
+// ============================================================ solver object
+struct dem_solver {
+    bool initialized = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    long n = 0;
+    int nb = 0, nt = 0, nw = 0;
+    bool offs = false;
+    double maxRadius = 0.0, meshSize = 0.0, inv = 0.0;
+    Material pp{}, pt{}, pw{};
+    double g[3] = {0, 0, 0};
+    int gravity = 0, universal = 0;
+    int stepIndex = 0;
+    int reorderInterval = 1;
+    uint64_t intervalSteps = 0;
+    size_t freeBefore = 0;
+    bool restored = false;
+
+    // device memory
+    double* dState[2] = {nullptr, nullptr};
+    int* iState[2] = {nullptr, nullptr};
+    int cur = 0;
+    Scalars* sc = nullptr;
+    unsigned long long* keyA = nullptr;
+    unsigned long long* keyB = nullptr;
+    int* valA = nullptr;
+    int* valB = nullptr;
+    unsigned* rsCounts = nullptr;
+    int rsBlocks = 0;
+    unsigned* cellStart = nullptr;
+    unsigned long long cellCap = 0;
+    double4* posr = nullptr;
+    int* nbr = nullptr;
+    int* ncount = nullptr;
+    int* byorig = nullptr;
+    History H{};
+    double* wallBuf = nullptr;
+    Outputs O{};
+    // triangles
+    int* tBody = nullptr;
+    double* tLocal = nullptr;
+    double* tWorld = nullptr;
+    double* tAabb = nullptr;
+    double* pose = nullptr;
+    double* bodyMass = nullptr;
+    int* tCounts = nullptr;
+    long long* tOffsets = nullptr;
+    long long tCap = 0;
+    unsigned long long* tKeyA = nullptr;
+    unsigned long long* tKeyB = nullptr;
+    int* tTriA = nullptr;
+    int* tTriB = nullptr;
+    unsigned* tsCounts = nullptr;
+    int tsBlocks = 0;
+    int tEntries = 0;
+    int* d64 = nullptr;  // device constant 64 (triangle sort bits)
+    unsigned long long* regimeInstant = nullptr;
+    unsigned long long* regimeInterval = nullptr;
+    double* microInstant = nullptr;
+    double* microInterval = nullptr;
+    double* bodyResult = nullptr;  // [nb*6 + nb]  loads, then fz per body
+    double* redPartial = nullptr;
+    int redBlocks = 0;
+    double* dl = nullptr;  // download staging [max rows][n]
+    // pinned
+    double* hPose = nullptr;
+    double* hLoads = nullptr;
+    Scalars* hScalars = nullptr;
+    // cached stats
+    dem_step_stats cached{};
+    std::vector<uint64_t> cRegInst, cRegInt;
+    std::vector<double> cMicInst, cMicInt;
+    bool keepBodyParticle = false, keepBodyDetail = false;
+    // timing
+    hipEvent_t ev[6] = {};
+    bool evReady = false;
+    // metadata for checkpoints
+    std::vector<int> compIds, ownerIds;
+    std::vector<double> compR, compB;
+
+    Soa soa() const { return Soa{dState[cur], iState[cur], n}; }
+
+    ~dem_solver()
+    {
+        if (stream) hipStreamSynchronize(stream);
+        void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
+                        rsCounts, cellStart, posr, nbr, ncount, byorig, H.ppOcc, H.ppSeen, H.ppKey,
+                        H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
+                        H.ptVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart, O.fzPart,
+                        O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, tLocal, tWorld, tAabb,
+                        pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
+                        regimeInstant, regimeInterval, microInstant, microInterval, bodyResult,
+                        redPartial, dl};
+        for (void* p : ptrs)
+            if (p) hipFree(p);
+        if (hPose) hipHostFree(hPose);
+        if (hLoads) hipHostFree(hLoads);
+        if (hScalars) hipHostFree(hScalars);
+        for (auto& e : ev)
+            if (e) hipEventDestroy(e);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T*& p, size_t count, const char* what)
+{
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc((void**)&p, sizeof(T) * count), what);
+    return DEM_OK;
+}
+
+#define TRY(x)                 \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
+
+int launch_check(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(DEM_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    return DEM_OK;
+}
+
+// Stable LSD radix sort of (key, val) pairs held in (ka, va); (kb, vb) is
+// scratch.  Passes over bits >= *nbits (device) are skipped in pairs.
+int radix_sort(hipStream_t st, unsigned long long* ka, int* va, unsigned long long* kb, int* vb,
+               long n, unsigned* counts, int nblocks, const int* nbits_dev)
+{
+    if (n <= 0) return DEM_OK;
+    const long total = 256L * nblocks;
+    for (int shift = 0; shift < 64; shift += 8) {
+        const bool even = ((shift / 8) & 1) == 0;
+        const unsigned long long* kin = even ? ka : kb;
+        const int* vin = even ? va : vb;
+        unsigned long long* kout = even ? kb : ka;
+        int* vout = even ? vb : va;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(BLOCK), 0, st, kin, n, shift, nbits_dev,
+                           counts, nblocks);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, counts, total, shift, nbits_dev);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(BLOCK), 0, st, kin, vin, kout, vout, n,
+                           shift, nbits_dev, counts, nblocks);
+    }
+    return launch_check("radix sort");
+}
+
+bool valid_mech(const dem_mech* m) { return m != nullptr; }
+
+Material make_material(const dem_mech& m, double dt)
+{
+    Material r;
+    r.epsS = m.epsS; r.mu = m.mu; r.epsN = m.epsN; r.cohesion = m.cohesion; r.beta = m.beta;
+    r.muT = m.muT; r.muR = m.muR; r.kN = m.kN; r.dt = dt;
+    r.logN = std::log(m.epsN);
+    r.logS = std::log(m.epsS);
+    return r;
+}
+
+}  // namespace
+
+// ============================================================ C-ABI
+extern "C" {
+
+const char* dem_last_error(void) { return g_error; }
+
+int dem_backend_available(void)
+{
+    int count = 0;
+    return hipGetDeviceCount(&count) == hipSuccess && count > 0 ? 1 : 0;
+}
+
+const char* dem_device_description(void)
+{
+    static char desc[256] = {0};
+    if (desc[0] == '\0') {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess) {
+            snprintf(desc, sizeof desc, "%s (%s), %d CUs, %.1f GiB", prop.name, prop.gcnArchName,
+                     prop.multiProcessorCount,
+                     (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
+        } else {
+            snprintf(desc, sizeof desc, "unavailable");
+        }
+    }
+    return desc;
+}
+
+int dem_create(dem_solver** out)
+{
+    if (!out) return set_error(DEM_E_INVALID, "null output pointer");
+    *out = new (std::nothrow) dem_solver();
+    return *out ? DEM_OK : set_error(DEM_E_INVALID, "out of memory");
+}
+
+void dem_destroy(dem_solver* s) { delete s; }
+
+int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_components_view* C,
+                   const dem_walls_view* W, const dem_bodies_view* B, const dem_mech* mpp,
+                   const dem_mech* mpt, const dem_mech* mpw, const dem_control* ctl)
+{
+    if (!s || !P || !C || !valid_mech(mpp) || !valid_mech(mpt) || !valid_mech(mpw) || !ctl)
+        return set_error(DEM_E_INVALID, "null argument");
+    if (s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver already initialized");
+    if (!dem_backend_available()) return set_error(DEM_E_DEVICE, "no CUDA device is available");
+    if (P->count <= 0) return set_error(DEM_E_INVALID, "CUDA DEM solver requires particles");
+    const long n = P->count;
+    {
+        std::vector<int> ids(P->id, P->id + n);
+        std::sort(ids.begin(), ids.end());
+        for (long i = 1; i < n; ++i)
+            if (ids[i] == ids[i - 1])
+                return set_error(DEM_E_INVALID,
+                                 "duplicate particle ID %d; CUDA contact history requires stable unique particle IDs",
+                                 ids[i]);
+    }
+    if (C->count <= 0)
+        return set_error(DEM_E_INVALID, "CUDA clump topology requires owners and components");
+    // components sorted by (owner, id) (backend:121-190)
+    std::vector<int> order(C->count);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        if (C->owner[a] != C->owner[b]) return C->owner[a] < C->owner[b];
+        return C->id[a] < C->id[b];
+    });
+    std::vector<int> perOwner(n, 0);
+    double maxr = 0.0;
+    for (int k : order) {
+        const int ow = C->owner[k];
+        if (ow < 0 || ow >= n || C->id[k] < 0 || !std::isfinite(C->radius[k]) || C->radius[k] <= 0.0)
+            return set_error(DEM_E_INVALID, "invalid native clump component topology");
+        ++perOwner[ow];
+        maxr = std::max(maxr, C->radius[k]);
+    }
+    for (long i = 0; i < n; ++i)
+        if (perOwner[i] < 1 || perOwner[i] > 8)
+            return set_error(DEM_E_INVALID, "each CUDA owner requires 1 to 8 components");
+    if (C->count != n)
+        return set_error(DEM_E_UNSUPPORTED,
+                         "multi-sphere clumps (more than one component per owner) are not supported by this build");
+    if (n > (long)INT_MAX / PP_SLOTS)
+        return set_error(DEM_E_CAPACITY, "particle count exceeds CUDA contact-history index capacity");
+
+    HIP_TRY(hipSetDevice(0), "hipSetDevice");
+    HIP_TRY(hipFree(nullptr), "initialize HIP context");
+    size_t total = 0;
+    HIP_TRY(hipMemGetInfo(&s->freeBefore, &total), "query HIP memory before allocation");
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+
+    s->n = n;
+    s->maxRadius = maxr;
+    s->meshSize = (ctl->mesh_scale > 0.0 ? ctl->mesh_scale : 2.5) * maxr;
+    s->inv = 1.0 / s->meshSize;
+    s->pp = make_material(*mpp, ctl->dt);
+    s->pt = make_material(*mpt, ctl->dt);
+    s->pw = make_material(*mpw, ctl->dt);
+    s->g[0] = ctl->gravity[0]; s->g[1] = ctl->gravity[1]; s->g[2] = ctl->gravity[2];
+    s->gravity = ctl->gravity_enabled ? 1 : 0;
+    s->universal = ctl->universal_gravity ? 1 : 0;
+    s->keepBodyParticle = ctl->keep_body_particle != 0;
+    s->keepBodyDetail = ctl->keep_body_detail != 0;
+
+    // host SoA in input row order; component k of owner row o sits at row o
+    std::vector<double> hd((size_t)NDF * n, 0.0);
+    std::vector<int> hi((size_t)NIF * n, 0);
+    auto D = [&](int f, long p) -> double& { return hd[(size_t)f * n + p]; };
+    auto Iv = [&](int f, long p) -> int& { return hi[(size_t)f * n + p]; };
+    s->compIds.assign(n, 0); s->ownerIds.assign(n, 0); s->compR.assign(n, 0.0); s->compB.assign(3 * n, 0.0);
+    for (int k : order) {
+        const long p = C->owner[k];
+        D(F_CR, p) = C->radius[k];
+        D(F_BX, p) = C->pos_body[3 * k]; D(F_BY, p) = C->pos_body[3 * k + 1]; D(F_BZ, p) = C->pos_body[3 * k + 2];
+        Iv(I_CID, p) = C->id[k];
+        if (D(F_BX, p) != 0.0 || D(F_BY, p) != 0.0 || D(F_BZ, p) != 0.0) s->offs = true;
+        s->compIds[p] = C->id[k]; s->ownerIds[p] = P->id[p]; s->compR[p] = C->radius[k];
+        s->compB[3 * p] = D(F_BX, p); s->compB[3 * p + 1] = D(F_BY, p); s->compB[3 * p + 2] = D(F_BZ, p);
+    }
+    for (long p = 0; p < n; ++p) {
+        D(F_X, p) = P->pos[3 * p]; D(F_Y, p) = P->pos[3 * p + 1]; D(F_Z, p) = P->pos[3 * p + 2];
+        D(F_VX, p) = P->vel[3 * p]; D(F_VY, p) = P->vel[3 * p + 1]; D(F_VZ, p) = P->vel[3 * p + 2];
+        D(F_WX, p) = P->ang_vel[3 * p]; D(F_WY, p) = P->ang_vel[3 * p + 1]; D(F_WZ, p) = P->ang_vel[3 * p + 2];
+        D(F_QW, p) = P->quat[4 * p]; D(F_QX, p) = P->quat[4 * p + 1];
+        D(F_QY, p) = P->quat[4 * p + 2]; D(F_QZ, p) = P->quat[4 * p + 3];
+        D(F_MASS, p) = P->mass[p];
+        D(F_IX, p) = P->inertia[3 * p]; D(F_IY, p) = P->inertia[3 * p + 1]; D(F_IZ, p) = P->inertia[3 * p + 2];
+        Iv(I_PID, p) = P->id[p];
+        Iv(I_STATUS, p) = P->status[p];
+        Iv(I_ORIG, p) = (int)p;
+    }
+    for (int b = 0; b < 2; ++b) {
+        TRY(dalloc(s->dState[b], (size_t)NDF * n, "hipMalloc(particle state)"));
+        TRY(dalloc(s->iState[b], (size_t)NIF * n, "hipMalloc(particle ints)"));
+    }
+    HIP_TRY(hipMemcpy(s->dState[0], hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice),
+            "upload particle state");
+    HIP_TRY(hipMemcpy(s->iState[0], hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice),
+            "upload particle ints");
+    s->cur = 0;
+    TRY(dalloc(s->sc, 1, "hipMalloc(step scalars)"));
+    HIP_TRY(hipMemset(s->sc, 0, sizeof(Scalars)), "clear step scalars");
+    TRY(dalloc(s->keyA, n, "hipMalloc(grid keys)"));
+    TRY(dalloc(s->keyB, n, "hipMalloc(grid keys)"));
+    TRY(dalloc(s->valA, n, "hipMalloc(grid ids)"));
+    TRY(dalloc(s->valB, n, "hipMalloc(grid ids)"));
+    s->rsBlocks = (int)((n + RS_TILE - 1) / RS_TILE);
+    TRY(dalloc(s->rsCounts, (size_t)256 * s->rsBlocks, "hipMalloc(sort counts)"));
+    s->cellCap = (unsigned long long)std::max<long>(8 * n + 64, 1 << 20);
+    TRY(dalloc(s->cellStart, s->cellCap, "hipMalloc(cell table)"));
+    TRY(dalloc(s->posr, n, "hipMalloc(packed positions)"));
+    TRY(dalloc(s->nbr, (size_t)MAX_LIST * n, "hipMalloc(contact lists)"));
+    TRY(dalloc(s->ncount, n, "hipMalloc(contact counts)"));
+    TRY(dalloc(s->byorig, n, "hipMalloc(row index)"));
+    // history
+    TRY(dalloc(s->H.ppOcc, n, "hipMalloc(particle history)"));
+    TRY(dalloc(s->H.ppSeen, 2 * n, "hipMalloc(particle history)"));
+    TRY(dalloc(s->H.ppKey, (size_t)PP_SLOTS * n, "hipMalloc(particle history keys)"));
+    TRY(dalloc(s->H.ppVal, (size_t)PP_SLOTS * 9 * n, "hipMalloc(particle history values)"));
+    TRY(dalloc(s->H.pwOcc, n, "hipMalloc(wall history)"));
+    TRY(dalloc(s->H.pwSeen, 2 * n, "hipMalloc(wall history)"));
+    TRY(dalloc(s->H.pwKey, (size_t)PW_SLOTS * n, "hipMalloc(wall history keys)"));
+    TRY(dalloc(s->H.pwVal, (size_t)PW_SLOTS * 9 * n, "hipMalloc(wall history values)"));
+    HIP_TRY(hipMemset(s->H.ppOcc, 0, sizeof(unsigned long long) * n), "clear history");
+    HIP_TRY(hipMemset(s->H.ppSeen, 0, sizeof(unsigned long long) * 2 * n), "clear history");
+    HIP_TRY(hipMemset(s->H.pwOcc, 0, sizeof(unsigned) * n), "clear history");
+    HIP_TRY(hipMemset(s->H.pwSeen, 0, sizeof(unsigned) * 2 * n), "clear history");
+    HIP_TRY(hipMemset(s->H.ppKey, 0, sizeof(uint32_t) * PP_SLOTS * n), "clear history");
+    HIP_TRY(hipMemset(s->H.pwKey, 0, sizeof(uint32_t) * PW_SLOTS * n), "clear history");
+    // walls
+    s->nw = W ? W->count : 0;
+    TRY(dalloc(s->wallBuf, (size_t)6 * std::max(s->nw, 1), "hipMalloc(walls)"));
+    if (s->nw > 0) {
+        HIP_TRY(hipMemcpy(s->wallBuf, W->origin, sizeof(double) * 3 * s->nw, hipMemcpyHostToDevice), "upload walls");
+        HIP_TRY(hipMemcpy(s->wallBuf + 3 * s->nw, W->normal, sizeof(double) * 3 * s->nw, hipMemcpyHostToDevice),
+                "upload walls");
+    }
+    // outputs
+    TRY(dalloc(s->O.contact, 6 * n, "hipMalloc(contact forces)"));
+    TRY(dalloc(s->O.wall, 6 * n, "hipMalloc(wall forces)"));
+    TRY(dalloc(s->O.body, 6 * n, "hipMalloc(body forces)"));
+    TRY(dalloc(s->O.field, 3 * n, "hipMalloc(field forces)"));
+    TRY(dalloc(s->dl, (size_t)19 * n, "hipMalloc(download staging)"));
+    HIP_TRY(hipHostMalloc((void**)&s->hScalars, sizeof(Scalars), hipHostMallocDefault), "hipHostMalloc");
+    std::memset(s->hScalars, 0, sizeof(Scalars));
+    // bodies
+    s->nb = B ? B->count : 0;
+    long long ntl = 0;
+    for (int b = 0; b < s->nb; ++b) {
+        if (B->tri_count[b] < 0) return set_error(DEM_E_INVALID, "body triangle count cannot be negative");
+        ntl += B->tri_count[b];
+        if (ntl > INT_MAX) return set_error(DEM_E_CAPACITY, "total triangle count exceeds CUDA 32-bit index capacity");
+    }
+    s->nt = (int)ntl;
+    if (s->nb > 0) {
+        const int nb = s->nb;
+        TRY(dalloc(s->H.ptOcc, n, "hipMalloc(body history)"));
+        TRY(dalloc(s->H.ptSeen, 2 * n, "hipMalloc(body history)"));
+        TRY(dalloc(s->H.ptKey, (size_t)PT_SLOTS * n, "hipMalloc(body history keys)"));
+        TRY(dalloc(s->H.ptVal, (size_t)PT_SLOTS * 9 * n, "hipMalloc(body history values)"));
+        HIP_TRY(hipMemset(s->H.ptOcc, 0, sizeof(unsigned long long) * n), "clear history");
+        HIP_TRY(hipMemset(s->H.ptSeen, 0, sizeof(unsigned long long) * 2 * n), "clear history");
+        HIP_TRY(hipMemset(s->H.ptKey, 0, sizeof(uint32_t) * PT_SLOTS * n), "clear history");
+        TRY(dalloc(s->pose, (size_t)13 * nb, "hipMalloc(body poses)"));
+        HIP_TRY(hipMemcpy(s->pose, B->pose13, sizeof(double) * 13 * nb, hipMemcpyHostToDevice), "upload body poses");
+        std::vector<int> tb(s->nt);
+        for (int b = 0, t = 0; b < nb; ++b)
+            for (int k = 0; k < B->tri_count[b]; ++k, ++t) tb[t] = b;
+        TRY(dalloc(s->tBody, s->nt, "hipMalloc(triangles)"));
+        TRY(dalloc(s->tLocal, (size_t)15 * s->nt, "hipMalloc(triangles)"));
+        TRY(dalloc(s->tWorld, (size_t)15 * s->nt, "hipMalloc(triangles)"));
+        TRY(dalloc(s->tAabb, (size_t)6 * s->nt, "hipMalloc(triangles)"));
+        TRY(dalloc(s->tCounts, s->nt + 1, "hipMalloc(triangle grid)"));
+        TRY(dalloc(s->tOffsets, s->nt + 1, "hipMalloc(triangle grid)"));
+        if (s->nt > 0) {
+            HIP_TRY(hipMemcpy(s->tBody, tb.data(), sizeof(int) * s->nt, hipMemcpyHostToDevice), "upload triangles");
+            HIP_TRY(hipMemcpy(s->tLocal, B->tri_local, sizeof(double) * 15 * s->nt, hipMemcpyHostToDevice),
+                    "upload triangles");
+        }
+        TRY(dalloc(s->d64, 1, "hipMalloc"));
+        const int sixtyfour = 64;
+        HIP_TRY(hipMemcpy(s->d64, &sixtyfour, sizeof(int), hipMemcpyHostToDevice), "upload");
+        TRY(dalloc(s->regimeInstant, (size_t)7 * nb, "hipMalloc(regime counters)"));
+        TRY(dalloc(s->regimeInterval, (size_t)7 * nb, "hipMalloc(regime counters)"));
+        TRY(dalloc(s->microInstant, (size_t)22 * nb, "hipMalloc(microstate)"));
+        TRY(dalloc(s->microInterval, (size_t)22 * nb, "hipMalloc(microstate)"));
+        HIP_TRY(hipMemset(s->regimeInterval, 0, sizeof(unsigned long long) * 7 * nb), "clear regime");
+        HIP_TRY(hipMemset(s->microInterval, 0, sizeof(double) * 22 * nb), "clear microstate");
+        TRY(dalloc(s->bodyResult, (size_t)7 * nb, "hipMalloc(body resultant)"));
+        TRY(dalloc(s->O.bodyPart, (size_t)6 * nb * n, "hipMalloc(body load partials)"));
+        TRY(dalloc(s->O.fzPart, (size_t)nb * n, "hipMalloc(body load partials)"));
+        TRY(dalloc(s->O.regime, (size_t)nb * n, "hipMalloc(body regime state)"));
+        TRY(dalloc(s->O.micro, (size_t)22 * nb * n, "hipMalloc(body microstate)"));
+        s->redBlocks = (int)((n + RED_TILE - 1) / RED_TILE);
+        TRY(dalloc(s->redPartial, (size_t)22 * nb * s->redBlocks, "hipMalloc(reduction partials)"));
+        if (s->keepBodyParticle) TRY(dalloc(s->O.bodyParticle, (size_t)nb * n * 6, "hipMalloc(body-particle forces)"));
+        if (s->keepBodyDetail) TRY(dalloc(s->O.bodyDetail, (size_t)nb * n * 19, "hipMalloc(body contact details)"));
+        HIP_TRY(hipHostMalloc((void**)&s->hPose, sizeof(double) * 13 * nb, hipHostMallocDefault), "hipHostMalloc");
+        HIP_TRY(hipHostMalloc((void**)&s->hLoads, sizeof(double) * 7 * nb, hipHostMallocDefault), "hipHostMalloc");
+        std::vector<double> mass(B->mass, B->mass + nb);
+        TRY(dalloc(s->bodyMass, nb, "hipMalloc(body mass)"));
+        HIP_TRY(hipMemcpy(s->bodyMass, mass.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "upload");
+        s->cRegInst.assign(7 * nb, 0); s->cRegInt.assign(7 * nb, 0);
+        s->cMicInst.assign(22 * nb, 0.0); s->cMicInt.assign(22 * nb, 0.0);
+        HIP_TRY(hipDeviceSynchronize(), "initialize");
+        if (s->nt > 0) {
+            TriSet T{};
+            T.nt = s->nt; T.nb = nb; T.body = s->tBody; T.local = s->tLocal; T.world = s->tWorld;
+            T.aabb = s->tAabb; T.pose = s->pose;
+            hipLaunchKernelGGL(k_transform, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, s->stream, T);
+            TRY(launch_check("transform triangles"));
+        }
+    }
+    HIP_TRY(hipDeviceSynchronize(), "initialize");
+    const char* env = getenv("DEM_REORDER_INTERVAL");
+    if (env) s->reorderInterval = std::max(0, atoi(env));
+    s->initialized = true;
+    return DEM_OK;
+}
+
+namespace {
+
+TriSet triset(const dem_solver* s)
+{
+    TriSet T{};
+    T.nt = s->nt; T.nb = s->nb; T.body = s->tBody; T.local = s->tLocal; T.world = s->tWorld;
+    T.aabb = s->tAabb; T.pose = s->pose; T.entries = s->tEntries; T.tkey = s->tKeyA; T.ttri = s->tTriA;
+    return T;
+}
+
+int build_triangle_grid(dem_solver* s)
+{
+    TriSet T = triset(s);
+    hipStream_t st = s->stream;
+    hipLaunchKernelGGL(k_tri_count, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
+                       s->tCounts, s->sc);
+    TRY(launch_check("countTriangleCells"));
+    std::vector<int> counts(s->nt);
+    HIP_TRY(hipMemcpyAsync(counts.data(), s->tCounts, sizeof(int) * s->nt, hipMemcpyDeviceToHost, st),
+            "copy triangle grid counts");
+    HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
+    HIP_TRY(hipStreamSynchronize(st), "triangle grid counts");
+    unsigned long long total = 0;
+    std::vector<long long> off(s->nt);
+    for (int i = 0; i < s->nt; ++i) { off[i] = (long long)total; total += (unsigned long long)counts[i]; }
+    if (s->hScalars->tri_overflow != 0 || total > (unsigned long long)INT_MAX)
+        return set_error(DEM_E_CAPACITY, "triangle grid entry count exceeds the supported 32-bit index range");
+    s->tEntries = (int)total;
+    if ((long long)total > s->tCap) {
+        for (void* p : {(void*)s->tKeyA, (void*)s->tKeyB, (void*)s->tTriA, (void*)s->tTriB, (void*)s->tsCounts})
+            if (p) hipFree(p);
+        const long long cap = (long long)total + (long long)total / 4 + 64;
+        TRY(dalloc(s->tKeyA, cap, "hipMalloc(dynamic triangle grid keys)"));
+        TRY(dalloc(s->tKeyB, cap, "hipMalloc(dynamic triangle grid keys)"));
+        TRY(dalloc(s->tTriA, cap, "hipMalloc(dynamic triangle grid ids)"));
+        TRY(dalloc(s->tTriB, cap, "hipMalloc(dynamic triangle grid ids)"));
+        s->tsBlocks = (int)((cap + RS_TILE - 1) / RS_TILE);
+        TRY(dalloc(s->tsCounts, (size_t)256 * s->tsBlocks, "hipMalloc(triangle sort counts)"));
+        s->tCap = cap;
+    }
+    if (total == 0) return DEM_OK;
+    HIP_TRY(hipMemcpyAsync(s->tOffsets, off.data(), sizeof(long long) * s->nt, hipMemcpyHostToDevice, st),
+            "upload triangle offsets");
+    hipLaunchKernelGGL(k_tri_fill, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
+                       s->tCounts, s->tOffsets, s->tKeyA, s->tTriA);
+    TRY(launch_check("fillTriangleCells"));
+    const int nblk = (int)((total + RS_TILE - 1) / RS_TILE);
+    TRY(radix_sort(st, s->tKeyA, s->tTriA, s->tKeyB, s->tTriB, (long)total, s->tsCounts, nblk, s->d64));
+    HIP_TRY(hipStreamSynchronize(st), "triangle grid");  // host vector `off` goes out of scope
+    return DEM_OK;
+}
+
+}  // namespace
+
+int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int collect)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    hipStream_t st = s->stream;
+    const long n = s->n;
+    const int nb = s->nb;
+    // a range error seen on an earlier step aborts here (reference throws at the step)
+    if (s->hScalars->grid_overflow != 0) {
+        s->hScalars->grid_overflow = 0;
+        return set_error(DEM_E_CAPACITY, "component grid coordinate exceeds signed 21-bit hash range");
+    }
+    if (pose13 && nb > 0) {
+        std::memcpy(s->hPose, pose13, sizeof(double) * 13 * nb);
+        HIP_TRY(hipMemcpyAsync(s->pose, s->hPose, sizeof(double) * 13 * nb, hipMemcpyHostToDevice, st),
+                "upload body poses");
+        if (s->nt > 0) {
+            hipLaunchKernelGGL(k_transform, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, st, triset(s));
+            TRY(launch_check("transform triangles"));
+        }
+    }
+    HIP_TRY(hipEventRecord(s->ev[0], st), "event");
+    hipLaunchKernelGGL(k_step_init, dim3(1), dim3(64), 0, st, s->sc);
+    Soa S = s->soa();
+    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->inv, s->offs, s->sc);
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, st, s->sc, s->cellCap);
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->sc, s->keyA, s->valA);
+    TRY(launch_check("grid keys"));
+    TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits));
+    hipLaunchKernelGGL(k_fixup, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->keyA, s->valA, n,
+                       S.i + (long)I_ORIG * n);
+    const bool reorder = s->reorderInterval > 0 && (s->stepIndex % s->reorderInterval) == 0;
+    if (reorder) {
+        Soa D{s->dState[1 - s->cur], s->iState[1 - s->cur], n};
+        hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valA);
+        hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->valA, n);
+        s->cur = 1 - s->cur;
+        S = s->soa();
+    }
+    const int cellBlocks = (int)std::min<unsigned long long>((s->cellCap + BLOCK - 1) / BLOCK, 65535ull);
+    hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, n, s->sc, s->cellStart,
+                       s->cellCap);
+    hipLaunchKernelGGL(k_posr, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->offs, s->posr);
+    TRY(launch_check("grid build"));
+    if (s->nt > 0) TRY(build_triangle_grid(s));
+    HIP_TRY(hipEventRecord(s->ev[1], st), "event");
+    hipLaunchKernelGGL(k_scan, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->keyA,
+                       s->cellStart, s->sc, s->nbr, s->ncount);
+    TRY(launch_check("contact scan"));
+    HIP_TRY(hipEventRecord(s->ev[2], st), "event");
+    const int step = s->stepIndex++;
+    LawArgs A{};
+    A.S = S; A.sorted = s->valA; A.nbr = s->nbr; A.ncount = s->ncount; A.H = s->H;
+    A.W = Walls{s->nw, s->wallBuf, s->wallBuf + 3 * s->nw};
+    A.T = triset(s); A.O = s->O; A.sc = s->sc;
+    A.pp = s->pp; A.pt = s->pt; A.pw = s->pw;
+    if (step == 0) { A.pp.dt = 0.0; A.pt.dt = 0.0; A.pw.dt = 0.0; }
+    A.g[0] = s->g[0]; A.g[1] = s->g[1]; A.g[2] = s->g[2];
+    A.gravity = s->gravity; A.universal = s->universal; A.byorig = s->byorig;
+    A.step = step; A.offs = s->offs; A.nb = nb;
+    if (s->universal) hipLaunchKernelGGL(k_byorig, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->byorig);
+    if (nb > 0) {
+        const int fb = 1024;
+        hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(BLOCK), 0, st, (unsigned*)s->O.regime, (long)nb * n);
+        if (s->O.bodyParticle) hipLaunchKernelGGL(k_fill, dim3(fb), dim3(BLOCK), 0, st, s->O.bodyParticle, (long)nb * n * 6, 0.0);
+        if (s->O.bodyDetail) hipLaunchKernelGGL(k_fill, dim3(fb), dim3(BLOCK), 0, st, s->O.bodyDetail, (long)nb * n * 19, 0.0);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(BLOCK), 0, st, s->regimeInstant, (long)7 * nb);
+    }
+    hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+    TRY(launch_check("contact law"));
+    HIP_TRY(hipEventRecord(s->ev[3], st), "event");
+    const bool interval = A.pt.dt > 0.0;
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, S, s->H, A.T, s->O,
+                           A.pt, step, s->offs);
+        hipLaunchKernelGGL(k_regime, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, s->O.regime, n, nb,
+                           s->regimeInstant, s->regimeInterval, interval ? 1 : 0);
+        // body loads [nb*6] then fz [nb] (rows laid out contiguously)
+        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 6 * nb), dim3(BLOCK), 0, st, s->O.bodyPart, n,
+                           6 * nb, s->redPartial, s->redBlocks);
+        hipLaunchKernelGGL(k_reduce_final, dim3((6 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 6 * nb, s->redBlocks,
+                           s->bodyResult, 0);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, nb), dim3(BLOCK), 0, st, s->O.fzPart, n, nb,
+                           s->redPartial, s->redBlocks);
+        hipLaunchKernelGGL(k_reduce_final, dim3((nb + 255) / 256), dim3(256), 0, st, s->redPartial, nb, s->redBlocks,
+                           s->bodyResult + 6 * nb, 0);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 22 * nb), dim3(BLOCK), 0, st, s->O.micro, n,
+                           22 * nb, s->redPartial, s->redBlocks);
+        hipLaunchKernelGGL(k_reduce_final, dim3((22 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 22 * nb, s->redBlocks,
+                           s->microInstant, 0);
+        if (interval)
+            hipLaunchKernelGGL(k_reduce_final, dim3((22 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 22 * nb, s->redBlocks,
+                               s->microInterval, 1);
+        TRY(launch_check("body reductions"));
+        ++s->intervalSteps;
+        if (!interval) --s->intervalSteps;
+    }
+    HIP_TRY(hipEventRecord(s->ev[4], st), "event");
+    s->evReady = true;
+    HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
+    const bool needLoads = loads6 && nb > 0;
+    if (needLoads)
+        HIP_TRY(hipMemcpyAsync(s->hLoads, s->bodyResult, sizeof(double) * 7 * nb, hipMemcpyDeviceToHost, st),
+                "download body resultant forces");
+    if (collect) {
+        HIP_TRY(hipStreamSynchronize(st), "collect diagnostics");
+        const Scalars& h = *s->hScalars;
+        if (h.grid_overflow != 0) {
+            s->hScalars->grid_overflow = 0;
+            return set_error(DEM_E_CAPACITY, "component grid coordinate exceeds signed 21-bit hash range");
+        }
+        dem_step_stats stt{};
+        stt.particle_contacts = h.particle_contacts;
+        stt.body_contacts = h.body_contacts;
+        double md;
+        std::memcpy(&md, &h.body_max_depth_bits, sizeof md);
+        stt.body_max_depth = md;
+        stt.particle_history_overflows = h.pp_overflow;
+        stt.body_history_overflows = h.pw_overflow + h.pt_overflow;
+        stt.particle_history_high_water = h.pp_high;
+        stt.body_history_high_water = h.body_high;
+        stt.triangle_grid_overflows = h.tri_overflow;
+        if (nb > 0) {
+            std::vector<double> fz(nb);
+            HIP_TRY(hipMemcpy(fz.data(), s->bodyResult + 6 * nb, sizeof(double) * nb, hipMemcpyDeviceToHost), "download fz");
+            double sum = 0.0;
+            for (double v : fz) sum += v;
+            stt.body_contact_fz = sum;
+            HIP_TRY(hipMemcpy(s->cRegInst.data(), s->regimeInstant, sizeof(uint64_t) * 7 * nb, hipMemcpyDeviceToHost), "download regime");
+            HIP_TRY(hipMemcpy(s->cRegInt.data(), s->regimeInterval, sizeof(uint64_t) * 7 * nb, hipMemcpyDeviceToHost), "download regime");
+            HIP_TRY(hipMemcpy(s->cMicInst.data(), s->microInstant, sizeof(double) * 22 * nb, hipMemcpyDeviceToHost), "download microstate");
+            HIP_TRY(hipMemcpy(s->cMicInt.data(), s->microInterval, sizeof(double) * 22 * nb, hipMemcpyDeviceToHost), "download microstate");
+            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(BLOCK), 0, st, s->regimeInterval, (long)7 * nb);
+            hipLaunchKernelGGL(k_fill, dim3(1), dim3(BLOCK), 0, st, s->microInterval, (long)22 * nb, 0.0);
+            TRY(launch_check("reset body regime interval counters"));
+            stt.body_regime_interval_native_steps = s->intervalSteps;
+            s->intervalSteps = 0;
+        }
+        s->cached = stt;
+        if (stt.particle_history_overflows != 0)
+            return set_error(DEM_E_CAPACITY,
+                             "CUDA particle contact-history capacity exceeded; increase kGpuParticleHistorySlots");
+        if (stt.body_history_overflows != 0)
+            return set_error(DEM_E_CAPACITY,
+                             "CUDA wall/body contact-history capacity exceeded; increase the configured history slots");
+        if (stt.triangle_grid_overflows != 0)
+            return set_error(DEM_E_CAPACITY, "CUDA triangle-grid coordinate/count overflow");
+    }
+    if (needLoads) {
+        HIP_TRY(hipStreamSynchronize(st), "download body resultant forces");
+        std::vector<double> mass(nb);
+        HIP_TRY(hipMemcpy(mass.data(), s->bodyMass, sizeof(double) * nb, hipMemcpyDeviceToHost), "download mass");
+        for (int b = 0; b < nb; ++b) {
+            for (int k = 0; k < 6; ++k) loads6[6 * b + k] = s->hLoads[6 * b + k];
+            if (s->gravity) {
+                loads6[6 * b] += s->g[0] * mass[b];
+                loads6[6 * b + 1] += s->g[1] * mass[b];
+                loads6[6 * b + 2] += s->g[2] * mass[b];
+            }
+        }
+    }
+    return DEM_OK;
+}
+
+int dem_init_half_step(dem_solver* s, double h)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    hipLaunchKernelGGL(k_init_half, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), h);
+    return launch_check("initializeLeapfrog");
+}
+
+int dem_advance_particles(dem_solver* s, double dt, double* vmax)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
+    if (vmax) *vmax = 0.0;
+    return launch_check("advancePosition");
+}
+
+int dem_finish_particle_step(dem_solver* s, double dt, double)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    hipLaunchKernelGGL(k_finish, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
+    return launch_check("advanceHalfVelocity");
+}
+
+int dem_download_particle_state(const dem_solver* s, double* pos, double* vel, double* ang, double* quat)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    const long n = s->n;
+    hipLaunchKernelGGL(k_gather_state, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), s->dl);
+    TRY(launch_check("gather state"));
+    std::vector<double> h((size_t)13 * n);
+    HIP_TRY(hipMemcpyAsync(h.data(), s->dl, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream),
+            "download particle state");
+    HIP_TRY(hipStreamSynchronize(s->stream), "download particle state");
+    for (long i = 0; i < n; ++i) {
+        if (pos) for (int a = 0; a < 3; ++a) pos[3 * i + a] = h[(size_t)a * n + i];
+        if (vel) for (int a = 0; a < 3; ++a) vel[3 * i + a] = h[(size_t)(3 + a) * n + i];
+        if (ang) for (int a = 0; a < 3; ++a) ang[3 * i + a] = h[(size_t)(6 + a) * n + i];
+        if (quat) for (int a = 0; a < 4; ++a) quat[4 * i + a] = h[(size_t)(9 + a) * n + i];
+    }
+    return DEM_OK;
+}
+
+int dem_download_force_output(const dem_solver* s, double* contact, double* field, double* wall,
+                              double* total, double* bp, double* bd)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    const long n = s->n;
+    hipStream_t st = s->stream;
+    std::vector<double> c(6 * n), w(6 * n), b(6 * n), f(3 * n);
+    struct Item { const double* src; int rows; double* dst; };
+    Item items[] = {{s->O.contact, 6, c.data()}, {s->O.wall, 6, w.data()}, {s->O.body, 6, b.data()},
+                    {s->O.field, 3, f.data()}};
+    for (auto& it : items) {
+        hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->soa(), it.src, it.rows, s->dl);
+        TRY(launch_check("gather forces"));
+        HIP_TRY(hipMemcpyAsync(it.dst, s->dl, sizeof(double) * it.rows * n, hipMemcpyDeviceToHost, st),
+                "download force output");
+        HIP_TRY(hipStreamSynchronize(st), "download force output");
+    }
+    for (long i = 0; i < n; ++i) {
+        double cf[6], wf[6], bf[6], ff[6];
+        for (int k = 0; k < 6; ++k) { cf[k] = c[(size_t)k * n + i]; wf[k] = w[(size_t)k * n + i]; bf[k] = b[(size_t)k * n + i]; }
+        for (int k = 0; k < 3; ++k) ff[k] = f[(size_t)k * n + i];
+        ff[3] = ff[4] = ff[5] = 0.0;
+        // total = (contact + field) + (wall + body), torque = contact + (wall + body) (contact:1215-1217)
+        double tf[6];
+        for (int k = 0; k < 3; ++k) tf[k] = (cf[k] + ff[k]) + (wf[k] + bf[k]);
+        for (int k = 3; k < 6; ++k) tf[k] = cf[k] + (wf[k] + bf[k]);
+        for (int k = 0; k < 6; ++k) {
+            if (contact) contact[6 * i + k] = cf[k];
+            if (field) field[6 * i + k] = 0.0 + ff[k];
+            if (wall) wall[6 * i + k] = wf[k];
+            if (total) total[6 * i + k] = tf[k];
+        }
+    }
+    if (bp && s->nb > 0) {
+        if (!s->O.bodyParticle) return set_error(DEM_E_INVALID, "body-particle force output was not kept");
+        HIP_TRY(hipMemcpy(bp, s->O.bodyParticle, sizeof(double) * 6 * s->nb * n, hipMemcpyDeviceToHost),
+                "download body-particle force output");
+    }
+    if (bd && s->nb > 0) {
+        if (!s->O.bodyDetail) return set_error(DEM_E_INVALID, "body detail output was not kept");
+        HIP_TRY(hipMemcpy(bd, s->O.bodyDetail, sizeof(double) * 19 * s->nb * n, hipMemcpyDeviceToHost),
+                "download body detail output");
+    }
+    return DEM_OK;
+}
+
+int dem_get_step_stats(const dem_solver* s, dem_step_stats* stats, uint64_t* ri, uint64_t* rv, double* mi,
+                       double* mv)
+{
+    if (!s) return set_error(DEM_E_INVALID, "null solver");
+    if (stats) *stats = s->cached;
+    const size_t nb = (size_t)s->nb;
+    if (ri && nb) std::memcpy(ri, s->cRegInst.data(), sizeof(uint64_t) * 7 * nb);
+    if (rv && nb) std::memcpy(rv, s->cRegInt.data(), sizeof(uint64_t) * 7 * nb);
+    if (mi && nb) std::memcpy(mi, s->cMicInst.data(), sizeof(double) * 22 * nb);
+    if (mv && nb) std::memcpy(mv, s->cMicInt.data(), sizeof(double) * 22 * nb);
+    return DEM_OK;
+}
+
+int dem_particle_grid_entry_count(const dem_solver* s, size_t* out)
+{
+    if (!s || !out) return set_error(DEM_E_INVALID, "null argument");
+    *out = (size_t)s->n;
+    return DEM_OK;
+}
+
+int dem_triangle_grid_entry_count(const dem_solver* s, size_t* out)
+{
+    if (!s || !out) return set_error(DEM_E_INVALID, "null argument");
+    *out = (size_t)s->tEntries;
+    return DEM_OK;
+}
+
+int dem_device_memory_bytes(const dem_solver* s, size_t* out)
+{
+    if (!s || !out) return set_error(DEM_E_INVALID, "null argument");
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot), "query HIP memory usage");
+    *out = s->freeBefore > fr ? s->freeBefore - fr : 0;
+    return DEM_OK;
+}
+
+int dem_debug_grid(const dem_solver* s, uint64_t* keys, int32_t* ids)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    const long n = s->n;
+    unsigned long long* dk = nullptr;
+    int* di = nullptr;
+    TRY(dalloc(dk, n, "hipMalloc"));
+    TRY(dalloc(di, n, "hipMalloc"));
+    hipLaunchKernelGGL(k_debug_grid, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), s->valA, dk, di);
+    HIP_TRY(hipStreamSynchronize(s->stream), "debug grid");
+    if (keys) HIP_TRY(hipMemcpy(keys, dk, sizeof(uint64_t) * n, hipMemcpyDeviceToHost), "download");
+    if (ids) HIP_TRY(hipMemcpy(ids, di, sizeof(int) * n, hipMemcpyDeviceToHost), "download");
+    hipFree(dk);
+    hipFree(di);
+    return DEM_OK;
+}
+
+int dem_debug_history(const dem_solver* s, uint64_t* pk, double* pv, uint64_t* wk, double* wv, uint64_t* bk,
+                      double* bv)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    HIP_TRY(hipStreamSynchronize(s->stream), "debug history");
+    const long n = s->n;
+    struct Tab { const void* occ; bool wide; const uint32_t* key; const double* val; int slots; uint64_t* ok; double* ov; };
+    Tab tabs[] = {{s->H.ppOcc, true, s->H.ppKey, s->H.ppVal, PP_SLOTS, pk, pv},
+                  {s->H.pwOcc, false, s->H.pwKey, s->H.pwVal, PW_SLOTS, wk, wv},
+                  {s->H.ptOcc, true, s->H.ptKey, s->H.ptVal, PT_SLOTS, bk, bv}};
+    for (auto& T : tabs) {
+        if (!T.ok && !T.ov) continue;
+        if (!T.occ) {
+            if (T.ok) std::memset(T.ok, 0, sizeof(uint64_t) * T.slots * n);
+            if (T.ov) std::memset(T.ov, 0, sizeof(double) * T.slots * 9 * n);
+            continue;
+        }
+        std::vector<unsigned long long> occ(n);
+        if (T.wide) {
+            HIP_TRY(hipMemcpy(occ.data(), T.occ, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost), "download");
+        } else {
+            std::vector<unsigned> o32(n);
+            HIP_TRY(hipMemcpy(o32.data(), T.occ, sizeof(unsigned) * n, hipMemcpyDeviceToHost), "download");
+            for (long i = 0; i < n; ++i) occ[i] = o32[i];
+        }
+        std::vector<uint32_t> key((size_t)T.slots * n);
+        HIP_TRY(hipMemcpy(key.data(), T.key, sizeof(uint32_t) * key.size(), hipMemcpyDeviceToHost), "download");
+        std::vector<double> val;
+        if (T.ov) {
+            val.resize((size_t)T.slots * 9 * n);
+            HIP_TRY(hipMemcpy(val.data(), T.val, sizeof(double) * val.size(), hipMemcpyDeviceToHost), "download");
+        }
+        for (long i = 0; i < n; ++i)
+            for (int b = 0; b < T.slots; ++b) {
+                const bool on = (occ[i] >> b) & 1ull;
+                const size_t idx = (size_t)i * T.slots + b;
+                if (T.ok) T.ok[idx] = on ? key[idx] : 0;
+                if (T.ov)
+                    for (int q = 0; q < 9; ++q) T.ov[idx * 9 + q] = on ? val[idx * 9 + q] : 0.0;
+            }
+    }
+    return DEM_OK;
+}
+
+int dem_debug_triangle_grid(const dem_solver* s, uint64_t* keys, int32_t* tris)
+{
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    HIP_TRY(hipStreamSynchronize(s->stream), "debug triangle grid");
+    if (s->tEntries == 0) return DEM_OK;
+    if (keys) HIP_TRY(hipMemcpy(keys, s->tKeyA, sizeof(uint64_t) * s->tEntries, hipMemcpyDeviceToHost), "download");
+    if (tris) HIP_TRY(hipMemcpy(tris, s->tTriA, sizeof(int) * s->tEntries, hipMemcpyDeviceToHost), "download");
+    return DEM_OK;
+}
+
+int dem_synchronize(const dem_solver* s)
+{
+    if (!s || !s->stream) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    return DEM_OK;
+}
+
+int dem_debug_last_timings(const dem_solver* s, double* ms, int capacity, int* count)
+{
+    if (!s || !count) return set_error(DEM_E_INVALID, "null argument");
+    *count = 0;
+    if (!s->evReady) return DEM_OK;
+    HIP_TRY(hipStreamSynchronize(s->stream), "timings");
+    for (int i = 0; i + 1 < 5 && i < capacity; ++i) {
+        float v = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&v, s->ev[i], s->ev[i + 1]), "hipEventElapsedTime");
+        ms[i] = v;
+        *count = i + 1;
+    }
+    return DEM_OK;
+}
+
+int dem_load_contact_history(dem_solver* s, const char* path)
+{
+    (void)path;
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "cannot load contact history before CUDA initialization");
+    return set_error(DEM_E_UNSUPPORTED, "DMHIST2 checkpoint reading is not implemented yet");
+}
+
+int dem_save_contact_history(const dem_solver* s, const char* path)
+{
+    (void)path;
+    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "cannot save contact history before CUDA initialization");
+    return set_error(DEM_E_UNSUPPORTED, "DMHIST2 checkpoint writing is not implemented yet");
+}
+
+int dem_has_restored_leapfrog_state(const dem_solver* s, int* out)
+{
+    if (!s || !out) return set_error(DEM_E_INVALID, "null argument");
+    *out = s->restored ? 1 : 0;
+    return DEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,370 @@
+"""Scene description: the inputs of one DEM run in the solver's array layout.
+
+A Scene holds what the reference host reads from `world.par`,
+`Particles.bt`, `Wall.bt`, `CouplePolyFileName.csv` and the `.bt` meshes
+(reference src/simulation.cpp:158-307, src/particle.cpp:18-106,
+src/wall.cpp:8-38, src/multibody.cpp:13-145,295-435), as numpy arrays in
+the layout of include/dem_hip.h.  It also builds the seeded synthetic
+packings used by the benchmarks.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+# MECH defaults of the reference (include/simulation.h:48-64).
+MECH_DEFAULTS = dict(epsS=1.0, mu=0.5, epsN=1.0, cohesion=1.0, beta=1.0, muT=1.3, muR=1.0, kN=1.0)
+
+
+def mech(**kw):
+    m = dict(MECH_DEFAULTS)
+    m.update(kw)
+    return m
+
+
+@dataclasses.dataclass
+class Scene:
+    pid: np.ndarray
+    status: np.ndarray
+    mass: np.ndarray
+    radius: np.ndarray
+    inertia: np.ndarray
+    pos: np.ndarray
+    vel: np.ndarray
+    angvel: np.ndarray
+    quat: np.ndarray
+    comp_id: np.ndarray = None
+    comp_owner: np.ndarray = None
+    comp_radius: np.ndarray = None
+    comp_pos_body: np.ndarray = None
+    wall_origin: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3)))
+    wall_normal: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3)))
+    body_mass: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    body_pose: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 13)))
+    body_tri_count: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+    tri_local: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 15)))
+    body_state: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+    body_inertia: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3, 3)))
+    pp: dict = dataclasses.field(default_factory=lambda: mech())
+    pt: dict = dataclasses.field(default_factory=lambda: mech())
+    pw: dict = dataclasses.field(default_factory=lambda: mech())
+    dt: float = 1e-6
+    gravity: tuple = (0.0, 0.0, 0.0)
+    gravity_enabled: int = 0
+    universal_gravity: int = 0
+    keep_body_particle: int = 0
+    keep_body_detail: int = 0
+    mesh_scale: float = 0.0
+    steps: int = 0
+    output_interval: int = 0
+
+    def __post_init__(self):
+        n = len(self.pid)
+        if self.comp_id is None:
+            # default one-sphere topology (reference src/particle.cpp:182-198)
+            self.comp_id = np.asarray(self.pid, np.int32).copy()
+            self.comp_owner = np.arange(n, dtype=np.int32)
+            self.comp_radius = np.asarray(self.radius, np.float64).copy()
+            self.comp_pos_body = np.zeros((n, 3))
+
+    @property
+    def n(self):
+        return len(self.pid)
+
+    def save(self, path):
+        d = {f.name: getattr(self, f.name) for f in dataclasses.fields(self)}
+        flat = {}
+        for k, v in d.items():
+            if isinstance(v, dict):
+                for kk, vv in v.items():
+                    flat[f"{k}__{kk}"] = np.asarray(vv)
+            else:
+                flat[k] = np.asarray(v)
+        np.savez_compressed(path, **flat)
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path, allow_pickle=False)
+        kw = {}
+        mechs = {"pp": {}, "pt": {}, "pw": {}}
+        for k in z.files:
+            if "__" in k:
+                a, b = k.split("__", 1)
+                mechs[a][b] = float(z[k])
+            else:
+                v = z[k]
+                kw[k] = v.item() if v.ndim == 0 else v
+        kw.update(mechs)
+        kw["gravity"] = tuple(np.asarray(kw["gravity"]).tolist())
+        return cls(**kw)
+
+    def copy(self):
+        return dataclasses.replace(self, **{f.name: (getattr(self, f.name).copy()
+                                                      if isinstance(getattr(self, f.name), np.ndarray)
+                                                      else getattr(self, f.name))
+                                            for f in dataclasses.fields(self)})
+
+
+def sphere_particles(pos, radius, density=None, mass=None, ids=None, status=None):
+    """Owner arrays for solid spheres: I = 0.4 m r^2 (reference particle.cpp:86-90)."""
+    pos = np.asarray(pos, np.float64).reshape(-1, 3)
+    radius = np.asarray(radius, np.float64)
+    n = len(radius)
+    if mass is None:
+        mass = density * (4.0 / 3.0) * np.pi * radius ** 3
+    mass = np.asarray(mass, np.float64) * np.ones(n)
+    inertia = 0.4 * mass * radius * radius
+    return dict(pid=np.arange(n, dtype=np.int32) if ids is None else np.asarray(ids, np.int32),
+                status=np.zeros(n, np.int32) if status is None else np.asarray(status, np.int32),
+                mass=mass, radius=radius, inertia=np.repeat(inertia[:, None], 3, axis=1),
+                pos=pos.copy(), vel=np.zeros((n, 3)), angvel=np.zeros((n, 3)),
+                quat=np.tile([1.0, 0.0, 0.0, 0.0], (n, 1)))
+
+
+def box_walls(lo, hi):
+    """Six planar walls of an axis-aligned box, normals pointing inward."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    origin, normal = [], []
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = 1.0
+        origin.append(lo.copy()); normal.append(e.copy())
+        origin.append(hi.copy()); normal.append(-e)
+    return np.array(origin), np.array(normal)
+
+
+_scene_lib = None
+
+
+def _rsa_lib():
+    global _scene_lib
+    if _scene_lib is None:
+        path = os.path.join(_HERE, "libdemscene.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        lib.dem_rsa_box.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.c_uint64, C.c_int, C.c_double,
+                                    C.POINTER(C.c_double)]
+        lib.dem_rsa_box.restype = C.c_int
+        _scene_lib = lib
+    return _scene_lib
+
+
+def rsa_positions(radius, lo, hi, seed, max_attempts=2000, gap=0.0):
+    radius = np.ascontiguousarray(radius, np.float64)
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray(hi, np.float64)
+    pos = np.zeros((len(radius), 3))
+    dp = C.POINTER(C.c_double)
+    placed = _rsa_lib().dem_rsa_box(len(radius), radius.ctypes.data_as(dp), lo.ctypes.data_as(dp),
+                                    hi.ctypes.data_as(dp), int(seed) & ((1 << 64) - 1),
+                                    int(max_attempts), float(gap), pos.ctypes.data_as(dp))
+    if placed != len(radius):
+        raise RuntimeError(f"random sequential insertion placed only {placed}/{len(radius)} spheres")
+    return pos
+
+
+def box_scene(n=2000, seed=1, box=(0.04, 0.04, 0.08), rmin=1.0e-3, rmax=1.5e-3, density=2500.0,
+              E=1.0e7, mu=0.5, restitution=0.5, dt=1.0e-6, steps=2000, muT=0.1, muR=0.1, beta=0.5,
+              gravity=(0.0, 0.0, -9.81)):
+    """Pure-DEM box: n spheres, radius U[rmin, rmax], seeded RSA, 6 walls.
+
+    The reference's law takes a normal stiffness kN [N/m] (ν is unused); the
+    mapping kN = E * mean radius is used, with epsS = epsN = restitution and
+    no cohesion.
+    """
+    rng = np.random.RandomState(seed)
+    radius = rng.uniform(rmin, rmax, n)
+    lo, hi = np.zeros(3), np.asarray(box, float)
+    pos = rsa_positions(radius, lo, hi, seed=seed * 7919 + 17)
+    kN = E * 0.5 * (rmin + rmax)
+    m = mech(epsS=restitution, mu=mu, epsN=restitution, cohesion=0.0, beta=beta, muT=muT, muR=muR,
+             kN=kN)
+    wo, wn = box_walls(lo, hi)
+    return Scene(**sphere_particles(pos, radius, density), wall_origin=wo, wall_normal=wn,
+                 pp=m, pt=dict(m), pw=dict(m), dt=dt, gravity=tuple(gravity), gravity_enabled=1,
+                 steps=steps)
+
+
+def hopper_walls(half_width, angle_deg):
+    """Floor z=0 plus four planes leaning outward by angle_deg from vertical."""
+    t = np.tan(np.radians(angle_deg))
+    origin = [[0.0, 0.0, 0.0]]
+    normal = [[0.0, 0.0, 1.0]]
+    for a in (0, 1):
+        for sgn in (1.0, -1.0):
+            o = np.zeros(3)
+            o[a] = -sgn * half_width
+            nrm = np.zeros(3)
+            nrm[a] = sgn
+            nrm[2] = t
+            origin.append(o)
+            normal.append(nrm / np.linalg.norm(nrm))
+    return np.array(origin), np.array(normal)
+
+
+def hopper_scene(n=1_000_000, seed=2, width=0.2, fill=0.36, median=1.0e-3, sigma=0.2,
+                 rclip=(0.5e-3, 2.0e-3), density=2500.0, E=1.0e7, mu=0.5, restitution=0.5,
+                 dt=5.0e-7, steps=5000, angle_deg=10.0, muT=0.1, muR=0.1, beta=0.5,
+                 mesh_scale=0.0, gravity=(0.0, 0.0, -9.81)):
+    """Pure-DEM hopper: lognormal radii, seeded RSA into the hopper's footprint
+    column at volume fraction `fill`, floor + 4 inclined walls, gravity.
+
+    Radii are placed largest first so the insertion reaches a dense column.
+    """
+    rng = np.random.RandomState(seed)
+    radius = np.clip(median * np.exp(sigma * rng.standard_normal(n)), rclip[0], rclip[1])
+    radius = np.sort(radius)[::-1].copy()
+    vol = float(np.sum(4.0 / 3.0 * np.pi * radius ** 3))
+    height = vol / (fill * width * width)
+    lo = np.array([-0.5 * width, -0.5 * width, 0.0])
+    hi = np.array([0.5 * width, 0.5 * width, height])
+    pos = rsa_positions(radius, lo, hi, seed=seed * 104729 + 3, max_attempts=20000)
+    kN = E * median
+    m = mech(epsS=restitution, mu=mu, epsN=restitution, cohesion=0.0, beta=beta, muT=muT, muR=muR,
+             kN=kN)
+    wo, wn = hopper_walls(0.5 * width, angle_deg)
+    return Scene(**sphere_particles(pos, radius, density), wall_origin=wo, wall_normal=wn,
+                 pp=m, pt=dict(m), pw=dict(m), dt=dt, gravity=tuple(gravity), gravity_enabled=1,
+                 mesh_scale=mesh_scale, steps=steps)
+
+
+# ---------------------------------------------------------------------------
+# Readers for the reference's input formats.
+# ---------------------------------------------------------------------------
+
+def _strip_comment(line):
+    return line.split("#", 1)[0].strip()
+
+
+def read_world_par(path):
+    """KEY = value pairs with '#' comments (reference simulation.cpp:158-307)."""
+    out = {}
+    with open(path, encoding="utf-8", errors="replace") as f:
+        for raw in f:
+            line = _strip_comment(raw)
+            if not line or "=" not in line:
+                continue
+            k, v = line.split("=", 1)
+            out[k.strip()] = v.strip()
+    return out
+
+
+def mechs_from_par(par):
+    """MECH_ keys for all three blocks plus MECH_PP_/PT_/PW_ overrides
+    (reference simulation.cpp:24-55)."""
+    names = dict(epsS="epsS", mu="mu", epsN="epsN", cohesive="cohesion", beta="beta",
+                 mu_T="muT", mu_R="muR", kN="kN")
+    blocks = {"pp": mech(), "pt": mech(), "pw": mech()}
+    for key, value in par.items():
+        mobj = re.match(r"MECH_(PP_|PT_|PW_)?(\w+)$", key)
+        if not mobj or mobj.group(2) not in names:
+            continue
+        field = names[mobj.group(2)]
+        targets = ["pp", "pt", "pw"] if mobj.group(1) is None else [mobj.group(1)[:2].lower()]
+        for t in targets:
+            blocks[t][field] = float(value)
+    return blocks
+
+
+def read_particles_bt(path):
+    """17 whitespace-separated fields per particle (reference particle.cpp:66-73)."""
+    data = np.loadtxt(path, ndmin=2)
+    pid = data[:, 0].astype(np.int32)
+    status = data[:, 1].astype(np.int32)
+    mass, radius = data[:, 2].copy(), data[:, 3].copy()
+    inertia = 0.4 * mass * radius * radius
+    return dict(pid=pid, status=status, mass=mass, radius=radius,
+                inertia=np.repeat(inertia[:, None], 3, axis=1), pos=data[:, 4:7].copy(),
+                vel=data[:, 7:10].copy(), quat=data[:, 10:14].copy(), angvel=data[:, 14:17].copy())
+
+
+def read_walls_bt(path):
+    tok = open(path).read().split()
+    n = int(tok[0])
+    vals = np.array(tok[1:1 + 6 * n], float).reshape(n, 6)
+    return vals[:, :3].copy(), vals[:, 3:].copy()
+
+
+def read_mesh_bt(path):
+    """Vertex count, triangle count, vertices, 1-based faces; per triangle the
+    inner apex v1 = centroid - normal * mean edge (reference multibody.cpp:54-145).
+    Returns [t][15] = v1, v2, v3, v4, unit normal in the body frame."""
+    tok = open(path).read().split()
+    nv, nt = int(tok[0]), int(tok[1])
+    pts = np.array(tok[2:2 + 3 * nv], float).reshape(nv, 3)
+    tri = np.array(tok[2 + 3 * nv:2 + 3 * nv + 3 * nt], int).reshape(nt, 3) - 1
+    out = np.zeros((nt, 15))
+    for k in range(nt):
+        v2, v3, v4 = pts[tri[k, 0]], pts[tri[k, 1]], pts[tri[k, 2]]
+        a1, a2, a3 = v3 - v2, v4 - v2, v3 - v4
+        center = (v2 + v3 + v4) / 3
+        n1, n2, n3 = (float(np.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])) for a in (a1, a2, a3))
+        mean = (n1 + n2 + n3) / 3
+        c = np.array([a1[1] * a2[2] - a1[2] * a2[1], a1[2] * a2[0] - a1[0] * a2[2],
+                      a1[0] * a2[1] - a1[1] * a2[0]])
+        length = float(np.sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]))
+        normal = c * (1.0 / length) if length > 1e-14 else np.zeros(3)
+        v1 = center - normal * mean
+        out[k] = np.concatenate([v1, v2, v3, v4, normal])
+    return out
+
+
+def read_bodies_csv(path, resolve):
+    """Count line, header, then 24 or 25 columns per body; the optional 25th
+    column is the motion state 0/1/2 (reference multibody.cpp:295-435)."""
+    lines = [ln.rstrip("\r\n") for ln in open(path, encoding="utf-8", errors="replace")]
+    count = int(lines[0].split(",")[0])
+    rows = [ln.split(",") for ln in lines[2:2 + count]]
+    mass, pose, state, inertia, tris, counts = [], [], [], [], [], []
+    for r in rows:
+        vals = [float(x) for x in r[1:24]]
+        st = int(r[24]) if len(r) >= 25 and r[24].strip() != "" else 0
+        vel = vals[4:7] if st != 1 else [0.0, 0.0, 0.0]
+        omg = vals[7:10] if st != 1 else [0.0, 0.0, 0.0]
+        mass.append(vals[0])
+        pose.append(vals[1:4] + vals[10:14] + vel + omg)
+        inertia.append(np.array(vals[14:23]).reshape(3, 3))
+        state.append(st)
+        mesh = read_mesh_bt(resolve(r[0]))
+        tris.append(mesh)
+        counts.append(len(mesh))
+    return dict(body_mass=np.array(mass), body_pose=np.array(pose),
+                body_state=np.array(state, np.int32), body_inertia=np.array(inertia),
+                tri_local=np.concatenate(tris) if tris else np.zeros((0, 15)),
+                body_tri_count=np.array(counts, np.int32))
+
+
+def read_case(case_dir):
+    """Loads a reference case directory (InputFile/...) into a Scene."""
+    par = read_world_par(os.path.join(case_dir, "InputFile", "SettingData", "world.par"))
+
+    def resolve(p):
+        p = p.strip().replace("\\", "/")
+        p = re.sub(r"^Data/DATA/", "", p)
+        return os.path.join(case_dir, p)
+
+    parts = read_particles_bt(resolve(par["CONTROL_PARTICLES_FILE"]))
+    kw = dict(parts)
+    if int(par.get("CONTROL_WALL_FLAG", "0")):
+        kw["wall_origin"], kw["wall_normal"] = read_walls_bt(resolve(par["CONTROL_WALL_FILE"]))
+    if int(par.get("CONTROL_MULTIBODY_FLAG", "0")):
+        kw.update(read_bodies_csv(resolve(par["CONTROL_MULTIBODY_FILE"]), resolve))
+    blocks = mechs_from_par(par)
+    g = [float(par.get(f"CONTROL_GRAVITY_{a}", "0")) for a in "XYZ"]
+    return Scene(**kw, pp=blocks["pp"], pt=blocks["pt"], pw=blocks["pw"],
+                 dt=float(par["ODE_StepSize"]), gravity=tuple(g),
+                 gravity_enabled=1 if int(par.get("CONTROL_GRAVITY_FLAG", "0")) else 0,
+                 universal_gravity=int(par.get("CONTROL_UNIVERSAL_GRAVITATION_FLAG", "0")),
+                 keep_body_particle=int(par.get("CONTROL_OUTPUT_PARTICLE_FORCE", "1")),
+                 keep_body_detail=int(par.get("CONTROL_OUTPUT_BODY_FORCE", "1")),
+                 steps=int(par["ODE_EndStep"]) - int(par.get("ODE_StartStep", "0")),
+                 output_interval=int(par.get("ODE_OutputInterval", "1")))

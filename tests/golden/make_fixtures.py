@@ -1,0 +1,86 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own
+validation inputs (data files, not source) and from the reference's CPU
+binary built in oracle/_ref/.  Run in the build container, where the
+reference tree exists:
+
+    python tests/golden/make_fixtures.py
+
+Outputs (committed):
+  case_<name>.npz     the five analytic validation cases (validation/cases/*)
+  data1_mbd.npz       DATA1 inputs (validation/ctest_data1_mbd), 1,704
+                      spheres, 5 walls, 2 bodies x 1,416 triangles
+  data1_pure.npz      the same without bodies (validation/ctest_data1_pure_dem)
+  data1_ref_cpu.npz   ph/F/At values written by the reference's legacy CPU
+                      binary (oracle/_ref/dem_mbd_cpu) after 5 steps on both
+                      DATA1 cases (particle state + force files, body state)
+"""
+from __future__ import annotations
+
+import glob
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+sys.path.insert(0, ROOT)
+
+from dem_mbd_cuda_amd import scene as S  # noqa: E402
+
+CASES = ["free_fall", "wall_contact", "two_particle", "triangle_contact", "body_states"]
+
+
+def run_reference_cpu(case_dir, work):
+    """Runs oracle/_ref/dem_mbd_cpu on a copy of case_dir with all outputs on."""
+    dst = os.path.join(work, os.path.basename(case_dir))
+    shutil.copytree(case_dir, dst)
+    par = os.path.join(dst, "InputFile", "SettingData", "world.par")
+    lines = open(par, encoding="utf-8", errors="replace").read().splitlines()
+    out = []
+    for ln in lines:
+        key = ln.split("=")[0].strip()
+        if key.startswith("CONTROL_OUTPUT_") and key != "CONTROL_OUTPUT_SUMMARY":
+            ln = f"{key} = 1"
+        out.append(ln)
+    open(par, "w").write("\n".join(out) + "\n")
+    exe = os.path.join(ROOT, "oracle", "_ref", "dem_mbd_cpu")
+    res = subprocess.run([exe, dst], cwd=work, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                         text=True, timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError(res.stdout[-4000:])
+    return dst
+
+
+def read_outputs(case_out):
+    d = {}
+    for sub, pref in (("state_particles", "ph"), ("force_particles", "F"), ("state_bodys", "At")):
+        for f in sorted(glob.glob(os.path.join(case_out, "OutputFile", sub, pref + ".*.bt"))):
+            idx = int(os.path.basename(f).split(".")[1])
+            d[f"{pref}_{idx}"] = np.loadtxt(f, ndmin=2)
+    return d
+
+
+def main():
+    for name in CASES:
+        sc = S.read_case(os.path.join(REF, "validation", "cases", name))
+        sc.save(os.path.join(HERE, f"case_{name}.npz"))
+    for tag, d in (("mbd", "ctest_data1_mbd"), ("pure", "ctest_data1_pure_dem")):
+        sc = S.read_case(os.path.join(REF, "validation", d))
+        sc.save(os.path.join(HERE, f"data1_{tag}.npz"))
+    ref = {}
+    with tempfile.TemporaryDirectory() as work:
+        for tag, d in (("mbd", "ctest_data1_mbd"), ("pure", "ctest_data1_pure_dem")):
+            out = run_reference_cpu(os.path.join(REF, "validation", d), work)
+            for k, v in read_outputs(out).items():
+                ref[f"{tag}__{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "data1_ref_cpu.npz"), **ref)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
