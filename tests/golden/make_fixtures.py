@@ -13,6 +13,8 @@ Outputs (committed):
   data1_ref_cpu.npz   ph/F/At values written by the reference's legacy CPU
                       binary (oracle/_ref/dem_mbd_cpu) after 5 steps on both
                       DATA1 cases (particle state + force files, body state)
+  data1_zero_vel_ref_cpu.npz  the initial (frame 0) outputs of the same binary
+                      on DATA1 with all particle velocities set to zero
 """
 from __future__ import annotations
 
@@ -35,10 +37,19 @@ from dem_mbd_cuda_amd import scene as S  # noqa: E402
 CASES = ["free_fall", "wall_contact", "two_particle", "triangle_contact", "body_states"]
 
 
-def run_reference_cpu(case_dir, work):
-    """Runs oracle/_ref/dem_mbd_cpu on a copy of case_dir with all outputs on."""
-    dst = os.path.join(work, os.path.basename(case_dir))
+def run_reference_cpu(case_dir, work, zero_velocity=False, tag=""):
+    """Runs oracle/_ref/dem_mbd_cpu on a copy of case_dir with all outputs on
+    (optionally with every particle velocity and angular velocity zeroed)."""
+    dst = os.path.join(work, os.path.basename(case_dir) + tag)
     shutil.copytree(case_dir, dst)
+    if zero_velocity:
+        pf = os.path.join(dst, "InputFile", "Particles", "Particles.bt")
+        rows = np.loadtxt(pf, ndmin=2)
+        rows[:, 7:10] = 0.0
+        rows[:, 14:17] = 0.0
+        with open(pf, "w") as f:
+            for r in rows:
+                f.write("%d %d " % (int(r[0]), int(r[1])) + " ".join("%.17e" % v for v in r[2:]) + "\n")
     par = os.path.join(dst, "InputFile", "SettingData", "world.par")
     lines = open(par, encoding="utf-8", errors="replace").read().splitlines()
     out = []
@@ -79,6 +90,14 @@ def main():
             for k, v in read_outputs(out).items():
                 ref[f"{tag}__{k}"] = v
     np.savez_compressed(os.path.join(HERE, "data1_ref_cpu.npz"), **ref)
+    zref = {}
+    with tempfile.TemporaryDirectory() as work:
+        for tag, d in (("mbd", "ctest_data1_mbd"), ("pure", "ctest_data1_pure_dem")):
+            out = run_reference_cpu(os.path.join(REF, "validation", d), work, zero_velocity=True, tag="_zv")
+            for k, v in read_outputs(out).items():
+                if k.endswith("_0"):
+                    zref[f"{tag}__{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "data1_zero_vel_ref_cpu.npz"), **zref)
     print("fixtures written to", HERE)
 
 
