@@ -4,7 +4,8 @@ Workload (default): pure DEM, 1,000,000 spheres, radius lognormal (median
 1 mm, sigma 0.2, clipped to [0.5, 2] mm), density 2500 kg/m^3, kN = E * median
 radius with E = 1e7 Pa, mu 0.5, restitution 0.5, seeded random sequential
 insertion into a 200 x 200 mm hopper (floor + 4 walls inclined 10 degrees),
-gravity -9.81 z, dt = 5e-7 s, cell = 2.5 x max radius (the reference's grid).
+gravity -9.81 z, dt = 5e-7 s, grid cell = 2 x max radius.  --packing dense
+replaces the loose insertion with a near-touching bed (contact-heavy).
 A step is the reference's pure-DEM loop body: advanceParticles,
 computeForces, finishParticleStep, all state resident in HBM.
 
@@ -44,7 +45,10 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mesh-scale", type=float, default=0.0)
+    ap.add_argument("--mesh-scale", type=float, default=2.0,
+                    help="grid cell / max radius (the workload specifies 2; the reference uses 2.5)")
+    ap.add_argument("--packing", choices=("rsa", "dense"), default="rsa",
+                    help="rsa: the specified seeded insertion (loose); dense: near-touching bed")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -57,7 +61,7 @@ def main():
     from dem_mbd_cuda_amd import scene
 
     t0 = time.time()
-    sc = scene.hopper_scene(n=args.n, seed=2, mesh_scale=args.mesh_scale)
+    sc = scene.hopper_scene(n=args.n, seed=2, mesh_scale=args.mesh_scale, packing=args.packing)
     print(f"# scene built in {time.time() - t0:.1f}s, {sc.n} spheres", file=sys.stderr, flush=True)
     dt = sc.dt
     s = dm.CudaDemSolver()
@@ -127,9 +131,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded RSA packing)",
-        "config": {"workload": "pure DEM hopper, 1M spheres lognormal r (median 1 mm, sigma 0.2), "
-                               "floor + 4 inclined walls, gravity, dt 5e-7, reference 2.5 r_max grid",
-                   "particles": sc.n, "dt": dt, "contacts_per_particle": cbar},
+        "config": {"workload": f"pure DEM hopper, {sc.n} spheres lognormal r (median 1 mm, sigma 0.2, "
+                               f"clipped 0.5-2 mm), floor + 4 walls inclined 10 deg, gravity, dt 5e-7, "
+                               f"grid cell {args.mesh_scale} x max radius, "
+                               + ("seeded random sequential insertion" if args.packing == "rsa"
+                                  else "dense near-touching packing"),
+                   "particles": sc.n, "dt": dt, "packing": args.packing,
+                   "contacts_per_particle": cbar},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_law", "kernel_ms": law_ms,

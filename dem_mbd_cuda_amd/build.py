@@ -37,7 +37,7 @@ def build_demhip():
 def build_scene():
     src = os.path.join(HERE, "csrc", "scene_rsa.c")
     out = os.path.join(HERE, "libdemscene.so")
-    _run(["gcc", "-O2", "-fPIC", "-shared", "-o", out, src, "-lm"])
+    _run(["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-o", out, src, "-lm"])
     return out
 
 

@@ -242,6 +242,58 @@ __device__ __forceinline__ void pw_law(const Body& I, V3 outward, double depth,
     tsum = add(tsum, add(couple, cross(I.off, cf)));
 }
 
+// ---- arc tangent --------------------------------------------------------
+// Built only from correctly rounded +, -, *, /, sqrt so the particle-triangle
+// path evaluates identically on any IEEE-754 machine (the yield test of the
+// read-only triangle pass compares a stored value of norm exactly mu|Fn|
+// against mu|Fn|; a last-bit difference in atan2 would flip that branch).
+// One half-angle reduction atan(t) = 2 atan(t / (1 + sqrt(1 + t^2))) takes
+// t <= 1 below tan(pi/8); the Taylor series to z^43/43 follows (within 3 ulp).
+__device__ __forceinline__ double series_atan(double t)
+{
+    t = t / (1.0 + sqrt(1.0 + t * t));
+    const double z = t * t;
+    double p = -1.0 / 43.0;
+    p = 1.0 / 41.0 + z * p;
+    p = -1.0 / 39.0 + z * p;
+    p = 1.0 / 37.0 + z * p;
+    p = -1.0 / 35.0 + z * p;
+    p = 1.0 / 33.0 + z * p;
+    p = -1.0 / 31.0 + z * p;
+    p = 1.0 / 29.0 + z * p;
+    p = -1.0 / 27.0 + z * p;
+    p = 1.0 / 25.0 + z * p;
+    p = -1.0 / 23.0 + z * p;
+    p = 1.0 / 21.0 + z * p;
+    p = -1.0 / 19.0 + z * p;
+    p = 1.0 / 17.0 + z * p;
+    p = -1.0 / 15.0 + z * p;
+    p = 1.0 / 13.0 + z * p;
+    p = -1.0 / 11.0 + z * p;
+    p = 1.0 / 9.0 + z * p;
+    p = -1.0 / 7.0 + z * p;
+    p = 1.0 / 5.0 + z * p;
+    p = -1.0 / 3.0 + z * p;
+    p = 1.0 + z * p;
+    return (t * p) * 2.0;
+}
+
+__device__ inline double arc_tan2(double y, double x)
+{
+    const double kHalfPi = 1.5707963267948966192313216916398;
+    if (isnan(x) || isnan(y)) return x + y;
+    const double ax = fabs(x), ay = fabs(y);
+    double a;
+    if (ax == 0.0 && ay == 0.0) a = 0.0;
+    else if (isinf(ax) && isinf(ay)) a = 0.5 * kHalfPi;
+    else if (isinf(ax)) a = 0.0;
+    else if (isinf(ay)) a = kHalfPi;
+    else if (ay <= ax) a = series_atan(ay / ax);
+    else a = kHalfPi - series_atan(ax / ay);
+    if (signbit(x)) a = kPi - a;
+    return signbit(y) ? -a : a;
+}
+
 // ---- sphere-section / triangle overlap area (contact:313-407) -----------
 struct V2 {
     double x, y;
@@ -285,7 +337,7 @@ __device__ inline double circle_edge_area(V2 a, V2 b, double radius)
         if (dot2(mid, mid) <= radius * radius)
             area += 0.5 * cross2(u, v);
         else
-            area += 0.5 * radius * radius * atan2(cross2(u, v), dot2(u, v));
+            area += 0.5 * radius * radius * arc_tan2(cross2(u, v), dot2(u, v));
     }
     return area;
 }

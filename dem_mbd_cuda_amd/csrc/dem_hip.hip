@@ -236,8 +236,10 @@ __global__ void k_step_init(Scalars* sc)
     }
 }
 
-// component centre + cell + range check + bbox (grid:51-71)
-__global__ void __launch_bounds__(BLOCK) k_cells(Soa S, double inv, bool offs, Scalars* sc)
+// component centre + cell + range check + bbox (grid:51-71).  The bounding
+// box is reduced per block (wave shuffles, then LDS) into part[block][6];
+// k_grid_setup folds the block partials.
+__global__ void __launch_bounds__(BLOCK) k_cells(Soa S, double inv, bool offs, Scalars* sc, int* part)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
@@ -264,26 +266,49 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, double inv, bool offs, S
         }
         S.k(I_VALID, p) = ok ? 1 : 0;
     }
+    __shared__ int red[BLOCK / 64][6];
+    const int wave = threadIdx.x >> 6;
     for (int a = 0; a < 3; ++a) {
         const int l = wave_min(lo[a]), h = wave_max(hi[a]);
-        if ((threadIdx.x & 63) == 0 && l <= h) {
-            atomicMin(&sc->lo[a], l);
-            atomicMax(&sc->hi[a], h);
-        }
+        if ((threadIdx.x & 63) == 0) { red[wave][a] = l; red[wave][3 + a] = h; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int v = red[0][threadIdx.x];
+        for (int w = 1; w < BLOCK / 64; ++w)
+            v = threadIdx.x < 3 ? min(v, red[w][threadIdx.x]) : max(v, red[w][threadIdx.x]);
+        part[(long)blockIdx.x * 6 + threadIdx.x] = v;
     }
 }
 
-__global__ void k_grid_setup(Scalars* sc, unsigned long long cap)
+// Folds the per-block boxes and sizes the grid (one block).
+__global__ void __launch_bounds__(BLOCK) k_grid_setup(Scalars* sc, const int* part, int nblocks,
+                                                      unsigned long long cap)
 {
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (int b = threadIdx.x; b < nblocks; b += BLOCK)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], part[(long)b * 6 + a]);
+            hi[a] = max(hi[a], part[(long)b * 6 + 3 + a]);
+        }
+    __shared__ int red[BLOCK / 64][6];
+    const int wave = threadIdx.x >> 6;
+    for (int a = 0; a < 3; ++a) {
+        const int l = wave_min(lo[a]), h = wave_max(hi[a]);
+        if ((threadIdx.x & 63) == 0) { red[wave][a] = l; red[wave][3 + a] = h; }
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     unsigned long long nc = 1ull;
-    bool empty = false;
     for (int a = 0; a < 3; ++a) {
-        if (sc->lo[a] > sc->hi[a]) { empty = true; sc->lo[a] = 0; sc->hi[a] = 0; }
-        sc->dims[a] = (unsigned long long)((long long)sc->hi[a] - sc->lo[a] + 1);
+        int l = red[0][a], h = red[0][3 + a];
+        for (int w = 1; w < BLOCK / 64; ++w) { l = min(l, red[w][a]); h = max(h, red[w][3 + a]); }
+        if (l > h) { l = 0; h = 0; }
+        sc->lo[a] = l;
+        sc->hi[a] = h;
+        sc->dims[a] = (unsigned long long)((long long)h - l + 1);
         nc *= sc->dims[a];
     }
-    (void)empty;
     sc->ncells = nc;
     const unsigned long long top = nc - 1ull;
     sc->nbits = top == 0ull ? 0 : 64 - __clzll((long long)top);
@@ -310,12 +335,11 @@ __global__ void __launch_bounds__(BLOCK) k_keys(Soa S, const Scalars* sc, unsign
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = BLOCK * RS_ITEMS;
 
-// A pass over bits [shift, shift+8) runs only if the pair of passes it belongs
-// to is needed: pairs keep the executed pass count even, so the sorted data
-// always ends in the first buffer.
+// A pass over bits [shift, shift+8) runs only if the key has significant
+// bits there; *nbits is computed on the device, so no host round trip.
 __device__ __forceinline__ bool pass_active(const int* nbits, int shift)
 {
-    return (shift & ~15) < *nbits;
+    return shift < *nbits;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_rs_hist(const unsigned long long* key, long n, int shift,
@@ -334,29 +358,53 @@ __global__ void __launch_bounds__(BLOCK) k_rs_hist(const unsigned long long* key
     counts[(long)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the [256][nblocks] digit-major counts, one block.
-__global__ void __launch_bounds__(1024) k_rs_scan(unsigned* counts, long total, int shift,
+// Exclusive scan of the [256][nblocks] digit-major counts in one block of
+// 1024 threads: four threads per digit each fold a quarter of the blocks,
+// the 256 digit totals are scanned in LDS, then each thread writes the
+// running offsets of its quarter.
+__global__ void __launch_bounds__(1024) k_rs_scan(unsigned* counts, int nblocks, int shift,
                                                   const int* nbits)
 {
     if (!pass_active(nbits, shift)) return;
-    __shared__ unsigned part[1024];
-    const long per = (total + 1023) / 1024;
-    const long b = threadIdx.x * per, e = min(total, b + per);
-    unsigned s = 0;
-    for (long i = b; i < e; ++i) s += counts[i];
-    part[threadIdx.x] = s;
+    __shared__ unsigned part[4][256];
+    __shared__ unsigned base[256];
+    const int d = threadIdx.x & 255, q = threadIdx.x >> 8;
+    const int cb = (nblocks + 3) / 4;
+    const int b0 = q * cb, b1 = min(nblocks, b0 + cb);
+    unsigned* row = counts + (long)d * nblocks;
+    unsigned sum = 0;
+    for (int b = b0; b < b1; ++b) sum += row[b];
+    part[q][d] = sum;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const unsigned v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    if (threadIdx.x < 256) base[d] = part[0][d] + part[1][d] + part[2][d] + part[3][d];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        unsigned v = 0;
+        if (threadIdx.x < 256 && d >= off) v = base[d - off];
         __syncthreads();
-        part[threadIdx.x] += v;
+        if (threadIdx.x < 256) base[d] += v;
         __syncthreads();
     }
-    unsigned run = part[threadIdx.x] - s;
-    for (long i = b; i < e; ++i) {
-        const unsigned c = counts[i];
-        counts[i] = run;
+    unsigned run = base[d] - (part[0][d] + part[1][d] + part[2][d] + part[3][d]);
+    for (int k = 0; k < q; ++k) run += part[k][d];
+    for (int b = b0; b < b1; ++b) {
+        const unsigned c = row[b];
+        row[b] = run;
         run += c;
+    }
+}
+
+// After the pass slots the data sits in the second buffer when an odd number
+// of passes ran; move it back so callers always read the first buffer.
+__global__ void __launch_bounds__(BLOCK) k_rs_copyback(unsigned long long* ka, int* va,
+                                                       const unsigned long long* kb, const int* vb,
+                                                       long n, const int* nbits)
+{
+    const int passes = (*nbits + 7) / 8;
+    if ((passes & 1) == 0) return;
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * BLOCK) {
+        ka[i] = kb[i];
+        va[i] = vb[i];
     }
 }
 
@@ -618,7 +666,7 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const Soa& S = A.S;
     const long n = S.n;
-    int pcount = 0, ppHigh = 0, bodyHigh = 0, ppOver = 0, pwOver = 0, ptOver = 0, bcount = 0;
+    int pcount = 0, ppHigh = 0, bodyHigh = 0, ppOver = 0, pwOver = 0;
     if (t < n) {
         const long p = A.sorted[t];
         const long o = S.k(I_ORIG, p);
@@ -671,8 +719,71 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
             ++pcount;
         }
         if (A.ncount[t] > MAX_LIST) ppOver += A.ncount[t] - MAX_LIST;
-        // triangles of the own cell (contact:1148-1208)
+        // triangle contacts were evaluated by k_law_pt (separate accumulator)
         if (A.nb > 0) {
+            const double* ob_ = A.O.body;
+            bF = mk(ob_[0 * n + p], ob_[1 * n + p], ob_[2 * n + p]);
+            bT = mk(ob_[3 * n + p], ob_[4 * n + p], ob_[5 * n + p]);
+        }
+        H.ppOcc[o] = ppOcc;
+        H.ppSeen[(long)(s & 1) * n + o] = ppCur;
+        H.pwOcc[o] = pwOcc;
+        H.pwSeen[(long)(s & 1) * n + o] = pwCur;
+        // outputs (contact:1211-1220)
+        double* oc = A.O.contact;
+        oc[0 * n + p] = cF.x; oc[1 * n + p] = cF.y; oc[2 * n + p] = cF.z;
+        oc[3 * n + p] = cT.x; oc[4 * n + p] = cT.y; oc[5 * n + p] = cT.z;
+        double* ow = A.O.wall;
+        ow[0 * n + p] = wF.x; ow[1 * n + p] = wF.y; ow[2 * n + p] = wF.z;
+        ow[3 * n + p] = wT.x; ow[4 * n + p] = wT.y; ow[5 * n + p] = wT.z;
+        double* of = A.O.field;
+        of[0 * n + p] = fF.x; of[1 * n + p] = fF.y; of[2 * n + p] = fF.z;
+        const V3 tF = add(add(cF, fF), add(wF, bF));
+        const V3 tT = add(cT, add(wT, bT));
+        // accelerationKernel (particle:71-101)
+        const double m = I.mass;
+        S.f(F_AX, p) = tF.x / m;
+        S.f(F_AY, p) = tF.y / m;
+        S.f(F_AZ, p) = tF.z / m;
+        const double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+        const double tx = (1.0 - 2.0 * (qy * qy + qz * qz)) * tT.x + 2.0 * (qx * qy + qw * qz) * tT.y +
+                          2.0 * (qx * qz - qw * qy) * tT.z;
+        const double ty = 2.0 * (qx * qy - qw * qz) * tT.x + (1.0 - 2.0 * (qx * qx + qz * qz)) * tT.y +
+                          2.0 * (qy * qz + qw * qx) * tT.z;
+        const double tz = 2.0 * (qx * qz + qw * qy) * tT.x + 2.0 * (qy * qz - qw * qx) * tT.y +
+                          (1.0 - 2.0 * (qx * qx + qy * qy)) * tT.z;
+        const double wx = S.f(F_WX, p), wy = S.f(F_WY, p), wz = S.f(F_WZ, p);
+        const double ix = S.f(F_IX, p), iy = S.f(F_IY, p), iz = S.f(F_IZ, p);
+        S.f(F_AWX, p) = (tx - (iz - iy) * wz * wy) / ix;
+        S.f(F_AWY, p) = (ty - (ix - iz) * wx * wz) / iy;
+        S.f(F_AWZ, p) = (tz - (iy - ix) * wy * wx) / iz;
+    }
+    atomic_add_wave(&A.sc->particle_contacts, pcount);
+    atomic_max_wave(&A.sc->pp_high, ppHigh);
+    atomic_max_wave(&A.sc->body_high, bodyHigh);
+    atomic_add_wave(&A.sc->pp_overflow, ppOver);
+    atomic_add_wave(&A.sc->pw_overflow, pwOver);
+}
+
+// Particle-triangle part of computeContactsKernel (contact:1148-1208): per
+// body, the area-weighted manifold over the triangles of the component's own
+// cell advances one history slot, then every triangle contributes its
+// read-only contact.  Writes the body accumulator (bF, bT) read by k_law.
+__global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const Soa& S = A.S;
+    const long n = S.n;
+    int bodyHigh = 0, ptOver = 0, bcount = 0;
+    if (t < n) {
+        const long p = A.sorted[t];
+        const long o = S.k(I_ORIG, p);
+        const int s = A.step;
+        const History& H = A.H;
+        const Body I = load_body(S, p, A.offs);
+        V3 bF = zero3(), bT = zero3();
+        // triangles of the own cell (contact:1148-1208)
+        {
             unsigned long long ptOcc = H.ptOcc[o] & H.ptSeen[(long)((s + 1) & 1) * n + o];
             unsigned long long ptCur = 0;
             const TriSet& T = A.T;
@@ -762,64 +873,30 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
             H.ptOcc[o] = ptOcc;
             H.ptSeen[(long)(s & 1) * n + o] = ptCur;
         }
-        H.ppOcc[o] = ppOcc;
-        H.ppSeen[(long)(s & 1) * n + o] = ppCur;
-        H.pwOcc[o] = pwOcc;
-        H.pwSeen[(long)(s & 1) * n + o] = pwCur;
-        // outputs (contact:1211-1220)
-        double* oc = A.O.contact;
-        oc[0 * n + p] = cF.x; oc[1 * n + p] = cF.y; oc[2 * n + p] = cF.z;
-        oc[3 * n + p] = cT.x; oc[4 * n + p] = cT.y; oc[5 * n + p] = cT.z;
-        double* ow = A.O.wall;
-        ow[0 * n + p] = wF.x; ow[1 * n + p] = wF.y; ow[2 * n + p] = wF.z;
-        ow[3 * n + p] = wT.x; ow[4 * n + p] = wT.y; ow[5 * n + p] = wT.z;
         double* ob = A.O.body;
         ob[0 * n + p] = bF.x; ob[1 * n + p] = bF.y; ob[2 * n + p] = bF.z;
         ob[3 * n + p] = bT.x; ob[4 * n + p] = bT.y; ob[5 * n + p] = bT.z;
-        double* of = A.O.field;
-        of[0 * n + p] = fF.x; of[1 * n + p] = fF.y; of[2 * n + p] = fF.z;
-        const V3 tF = add(add(cF, fF), add(wF, bF));
-        const V3 tT = add(cT, add(wT, bT));
-        // accelerationKernel (particle:71-101)
-        const double m = I.mass;
-        S.f(F_AX, p) = tF.x / m;
-        S.f(F_AY, p) = tF.y / m;
-        S.f(F_AZ, p) = tF.z / m;
-        const double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
-        const double tx = (1.0 - 2.0 * (qy * qy + qz * qz)) * tT.x + 2.0 * (qx * qy + qw * qz) * tT.y +
-                          2.0 * (qx * qz - qw * qy) * tT.z;
-        const double ty = 2.0 * (qx * qy - qw * qz) * tT.x + (1.0 - 2.0 * (qx * qx + qz * qz)) * tT.y +
-                          2.0 * (qy * qz + qw * qx) * tT.z;
-        const double tz = 2.0 * (qx * qz + qw * qy) * tT.x + 2.0 * (qy * qz - qw * qx) * tT.y +
-                          (1.0 - 2.0 * (qx * qx + qy * qy)) * tT.z;
-        const double wx = S.f(F_WX, p), wy = S.f(F_WY, p), wz = S.f(F_WZ, p);
-        const double ix = S.f(F_IX, p), iy = S.f(F_IY, p), iz = S.f(F_IZ, p);
-        S.f(F_AWX, p) = (tx - (iz - iy) * wz * wy) / ix;
-        S.f(F_AWY, p) = (ty - (ix - iz) * wx * wz) / iy;
-        S.f(F_AWZ, p) = (tz - (iy - ix) * wy * wx) / iz;
     }
-    atomic_add_wave(&A.sc->particle_contacts, pcount);
     atomic_add_wave(&A.sc->body_contacts, bcount);
-    atomic_max_wave(&A.sc->pp_high, ppHigh);
     atomic_max_wave(&A.sc->body_high, bodyHigh);
-    atomic_add_wave(&A.sc->pp_overflow, ppOver);
-    atomic_add_wave(&A.sc->pw_overflow, pwOver);
     atomic_add_wave(&A.sc->pt_overflow, ptOver);
 }
 
 // observeBodyMicrostateKernel (contact:1226-1356), one thread per
-// (body, particle); writes [body][22][n] by physical index.
-__global__ void __launch_bounds__(BLOCK) k_observe(Soa S, History H, TriSet T, Outputs O,
-                                                   Material m, int step, bool offs)
+// (body, particle in cell order); writes [body][22][t] so the per-body sums
+// run over the same order as the body loads.
+__global__ void __launch_bounds__(BLOCK) k_observe(Soa S, const int* sorted, History H, TriSet T,
+                                                   Outputs O, Material m, int step, bool offs)
 {
     const long n = S.n;
     const long idx = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (idx >= (long)T.nb * n) return;
     const int body = (int)(idx / n);
-    const long p = idx - (long)body * n;
-    double* ms = O.micro + (long)body * 22 * n + p;
+    const long t = idx - (long)body * n;
+    const long p = sorted[t];
+    double* ms = O.micro + (long)body * 22 * n + t;
     for (int f = 0; f < 22; ++f) ms[(long)f * n] = 0.0;
-    if ((O.regime[idx] & 1) == 0) return;
+    if ((O.regime[(long)body * n + p] & 1) == 0) return;
     const Body I = load_body(S, p, offs);
     const unsigned long long key = pack_cell(S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p));
     const long b0 = lower_bound_u64(T.tkey, T.entries, key);
@@ -1184,6 +1261,7 @@ struct dem_solver {
     unsigned* rsCounts = nullptr;
     int rsBlocks = 0;
     unsigned* cellStart = nullptr;
+    int* boxPart = nullptr;
     unsigned long long cellCap = 0;
     double4* posr = nullptr;
     int* nbr = nullptr;
@@ -1240,7 +1318,7 @@ struct dem_solver {
     {
         if (stream) hipStreamSynchronize(stream);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
-                        rsCounts, cellStart, posr, nbr, ncount, byorig, H.ppOcc, H.ppSeen, H.ppKey,
+                        rsCounts, cellStart, boxPart, posr, nbr, ncount, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart, O.fzPart,
                         O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, tLocal, tWorld, tAabb,
@@ -1287,7 +1365,6 @@ int radix_sort(hipStream_t st, unsigned long long* ka, int* va, unsigned long lo
                long n, unsigned* counts, int nblocks, const int* nbits_dev)
 {
     if (n <= 0) return DEM_OK;
-    const long total = 256L * nblocks;
     for (int shift = 0; shift < 64; shift += 8) {
         const bool even = ((shift / 8) & 1) == 0;
         const unsigned long long* kin = even ? ka : kb;
@@ -1296,10 +1373,12 @@ int radix_sort(hipStream_t st, unsigned long long* ka, int* va, unsigned long lo
         int* vout = even ? vb : va;
         hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(BLOCK), 0, st, kin, n, shift, nbits_dev,
                            counts, nblocks);
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, counts, total, shift, nbits_dev);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, counts, nblocks, shift, nbits_dev);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(BLOCK), 0, st, kin, vin, kout, vout, n,
                            shift, nbits_dev, counts, nblocks);
     }
+    const int cb = (int)std::min<long>((n + BLOCK - 1) / BLOCK, 4096);
+    hipLaunchKernelGGL(k_rs_copyback, dim3(cb), dim3(BLOCK), 0, st, ka, va, kb, vb, n, nbits_dev);
     return launch_check("radix sort");
 }
 
@@ -1466,6 +1545,7 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
     s->cellCap = (unsigned long long)std::max<long>(8 * n + 64, 1 << 20);
     TRY(dalloc(s->cellStart, s->cellCap, "hipMalloc(cell table)"));
     TRY(dalloc(s->posr, n, "hipMalloc(packed positions)"));
+    TRY(dalloc(s->boxPart, (size_t)6 * blocks_for(n), "hipMalloc(grid box partials)"));
     TRY(dalloc(s->nbr, (size_t)MAX_LIST * n, "hipMalloc(contact lists)"));
     TRY(dalloc(s->ncount, n, "hipMalloc(contact counts)"));
     TRY(dalloc(s->byorig, n, "hipMalloc(row index)"));
@@ -1652,8 +1732,10 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     HIP_TRY(hipEventRecord(s->ev[0], st), "event");
     hipLaunchKernelGGL(k_step_init, dim3(1), dim3(64), 0, st, s->sc);
     Soa S = s->soa();
-    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->inv, s->offs, s->sc);
-    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, st, s->sc, s->cellCap);
+    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->inv, s->offs, s->sc,
+                       s->boxPart);
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(BLOCK), 0, st, s->sc, s->boxPart, blocks_for(n),
+                       s->cellCap);
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->sc, s->keyA, s->valA);
     TRY(launch_check("grid keys"));
     TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits));
@@ -1696,13 +1778,17 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         if (s->O.bodyDetail) hipLaunchKernelGGL(k_fill, dim3(fb), dim3(BLOCK), 0, st, s->O.bodyDetail, (long)nb * n * 19, 0.0);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(BLOCK), 0, st, s->regimeInstant, (long)7 * nb);
     }
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_law_pt, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+        TRY(launch_check("particle-triangle contact law"));
+    }
     hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     TRY(launch_check("contact law"));
     HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     const bool interval = A.pt.dt > 0.0;
     if (nb > 0) {
-        hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, S, s->H, A.T, s->O,
-                           A.pt, step, s->offs);
+        hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, S, s->valA, s->H,
+                           A.T, s->O, A.pt, step, s->offs);
         hipLaunchKernelGGL(k_regime, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, s->O.regime, n, nb,
                            s->regimeInstant, s->regimeInterval, interval ? 1 : 0);
         // body loads [nb*6] then fz [nb] (rows laid out contiguously)
@@ -1844,6 +1930,7 @@ int dem_download_force_output(const dem_solver* s, double* contact, double* fiel
     Item items[] = {{s->O.contact, 6, c.data()}, {s->O.wall, 6, w.data()}, {s->O.body, 6, b.data()},
                     {s->O.field, 3, f.data()}};
     for (auto& it : items) {
+        if (it.src == s->O.body && s->nb == 0) continue;  // no bodies: body force is zero
         hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->soa(), it.src, it.rows, s->dl);
         TRY(launch_check("gather forces"));
         HIP_TRY(hipMemcpyAsync(it.dst, s->dl, sizeof(double) * it.rows * n, hipMemcpyDeviceToHost, st),

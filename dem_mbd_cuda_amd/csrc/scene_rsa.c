@@ -83,3 +83,89 @@ int dem_rsa_box(int n, const double* radius, const double lo[3], const double hi
     free(next);
     return placed;
 }
+
+/*
+ * Overlap relaxation of a packing (positions in/out): Jacobi sweeps in which
+ * every overlapping pair moves each sphere half the overlap apart along the
+ * centre line and the box faces (x, y on both sides, z below) push spheres
+ * back inside.  Stops when the largest overlap falls below tol * (r_i + r_j)
+ * or after max_iter sweeps; returns the sweeps used (-1: allocation failure).
+ * Threads split each sweep (OpenMP).
+ */
+int dem_relax_packing(int n, const double* radius, const double lo[3], const double hi[3],
+                      int max_iter, double tol, double* pos)
+{
+    double rmax = 0.0, top = lo[2];
+    for (int i = 0; i < n; ++i) rmax = radius[i] > rmax ? radius[i] : rmax;
+    for (int i = 0; i < n; ++i) top = pos[3 * i + 2] + radius[i] > top ? pos[3 * i + 2] + radius[i] : top;
+    const double cell = 2.0 * rmax;
+    int dims[3];
+    dims[0] = (int)floor((hi[0] - lo[0]) / cell) + 1;
+    dims[1] = (int)floor((hi[1] - lo[1]) / cell) + 1;
+    dims[2] = (int)floor((top - lo[2]) / cell) + 16;
+    const size_t ncell = (size_t)dims[0] * dims[1] * dims[2];
+    int* start = malloc(sizeof(int) * (ncell + 1));
+    int* order = malloc(sizeof(int) * (size_t)n);
+    int* cellof = malloc(sizeof(int) * (size_t)n);
+    double* disp = malloc(sizeof(double) * 3 * (size_t)n);
+    if (!start || !order || !cellof || !disp) {
+        free(start); free(order); free(cellof); free(disp);
+        return -1;
+    }
+    int it = 0;
+    for (; it < max_iter; ++it) {
+        for (size_t c = 0; c <= ncell; ++c) start[c] = 0;
+        for (int i = 0; i < n; ++i) {
+            int c3[3];
+            for (int d = 0; d < 3; ++d) {
+                c3[d] = (int)floor((pos[3 * i + d] - lo[d]) / cell);
+                if (c3[d] < 0) c3[d] = 0;
+                if (c3[d] >= dims[d]) c3[d] = dims[d] - 1;
+            }
+            cellof[i] = (int)(((size_t)c3[0] * dims[1] + c3[1]) * dims[2] + c3[2]);
+            ++start[cellof[i] + 1];
+        }
+        for (size_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
+        for (int i = 0; i < n; ++i) order[start[cellof[i]]++] = i;
+        for (size_t c = ncell; c > 0; --c) start[c] = start[c - 1];
+        start[0] = 0;
+        double worst = 0.0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : worst)
+        for (int i = 0; i < n; ++i) {
+            double dx = 0.0, dy = 0.0, dz = 0.0;
+            const int c = cellof[i];
+            const int cz = c % dims[2], cy = (c / dims[2]) % dims[1], cx = c / (dims[2] * dims[1]);
+            for (int ox = -1; ox <= 1; ++ox)
+                for (int oy = -1; oy <= 1; ++oy)
+                    for (int oz = -1; oz <= 1; ++oz) {
+                        const int X = cx + ox, Y = cy + oy, Z = cz + oz;
+                        if (X < 0 || Y < 0 || Z < 0 || X >= dims[0] || Y >= dims[1] || Z >= dims[2]) continue;
+                        const size_t cc = ((size_t)X * dims[1] + Y) * dims[2] + Z;
+                        for (int k = start[cc]; k < start[cc + 1]; ++k) {
+                            const int j = order[k];
+                            if (j == i) continue;
+                            const double ex = pos[3 * i] - pos[3 * j], ey = pos[3 * i + 1] - pos[3 * j + 1],
+                                         ez = pos[3 * i + 2] - pos[3 * j + 2];
+                            const double d2 = ex * ex + ey * ey + ez * ez, rs = radius[i] + radius[j];
+                            if (d2 >= rs * rs || d2 <= 0.0) continue;
+                            const double d = sqrt(d2), ov = rs - d;
+                            if (ov / rs > worst) worst = ov / rs;
+                            const double k2 = 0.5 * ov / d;
+                            dx += k2 * ex; dy += k2 * ey; dz += k2 * ez;
+                        }
+                    }
+            const double r = radius[i];
+            const double* pi = pos + 3 * i;
+            if (pi[0] - r < lo[0]) dx += lo[0] - (pi[0] - r);
+            if (pi[0] + r > hi[0]) dx -= pi[0] + r - hi[0];
+            if (pi[1] - r < lo[1]) dy += lo[1] - (pi[1] - r);
+            if (pi[1] + r > hi[1]) dy -= pi[1] + r - hi[1];
+            if (pi[2] - r < lo[2]) dz += lo[2] - (pi[2] - r);
+            disp[3 * i] = dx; disp[3 * i + 1] = dy; disp[3 * i + 2] = dz;
+        }
+        for (size_t k = 0; k < 3 * (size_t)n; ++k) pos[k] += 0.9 * disp[k];
+        if (worst < tol) break;
+    }
+    free(start); free(order); free(cellof); free(disp);
+    return it;
+}

@@ -158,6 +158,49 @@ def _rsa_lib():
     return _scene_lib
 
 
+def layered_lattice(radius, lo, hi_xy, factor=1.0):
+    """Spheres sorted by decreasing radius, laid on square layers from the
+    floor up; a layer's spacing is factor * twice its largest radius.
+    Returns (radius in placement order, positions)."""
+    r = np.sort(np.asarray(radius, float))[::-1].copy()
+    n = len(r)
+    pos = np.zeros((n, 3))
+    wx, wy = hi_xy[0] - lo[0], hi_xy[1] - lo[1]
+    z = lo[2]
+    prev = 0.0
+    i = 0
+    while i < n:
+        r0 = r[i]
+        s = 2.0 * r0 * factor
+        nx, ny = max(1, int(wx // s)), max(1, int(wy // s))
+        m = min(nx * ny, n - i)
+        z = z + (prev + r0) * factor if i else lo[2] + r0
+        k = np.arange(m)
+        pos[i:i + m, 0] = lo[0] + (wx - nx * s) / 2 + (k % nx + 0.5) * s
+        pos[i:i + m, 1] = lo[1] + (wy - ny * s) / 2 + (k // nx + 0.5) * s
+        pos[i:i + m, 2] = z
+        prev = r0
+        i += m
+    return r, pos
+
+
+def relax_packing(radius, pos, lo, hi_xy, max_iter=200, tol=1e-3):
+    """Removes overlaps in place (see dem_relax_packing); returns sweeps used."""
+    lib = _rsa_lib()
+    fn = lib.dem_relax_packing
+    dp = C.POINTER(C.c_double)
+    fn.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_double, dp]
+    fn.restype = C.c_int
+    radius = np.ascontiguousarray(radius, np.float64)
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray([hi_xy[0], hi_xy[1], 1e30], np.float64)
+    it = fn(len(radius), radius.ctypes.data_as(dp), lo.ctypes.data_as(dp), hi.ctypes.data_as(dp),
+            int(max_iter), float(tol), pos.ctypes.data_as(dp))
+    if it < 0:
+        raise RuntimeError("packing relaxation: out of memory")
+    return it
+
+
 def rsa_positions(radius, lo, hi, seed, max_attempts=2000, gap=0.0):
     radius = np.ascontiguousarray(radius, np.float64)
     lo = np.ascontiguousarray(lo, np.float64)
@@ -214,20 +257,29 @@ def hopper_walls(half_width, angle_deg):
 def hopper_scene(n=1_000_000, seed=2, width=0.2, fill=0.36, median=1.0e-3, sigma=0.2,
                  rclip=(0.5e-3, 2.0e-3), density=2500.0, E=1.0e7, mu=0.5, restitution=0.5,
                  dt=5.0e-7, steps=5000, angle_deg=10.0, muT=0.1, muR=0.1, beta=0.5,
-                 mesh_scale=0.0, gravity=(0.0, 0.0, -9.81)):
-    """Pure-DEM hopper: lognormal radii, seeded RSA into the hopper's footprint
-    column at volume fraction `fill`, floor + 4 inclined walls, gravity.
+                 mesh_scale=0.0, gravity=(0.0, 0.0, -9.81), packing="rsa"):
+    """Pure-DEM hopper: lognormal radii, floor + 4 inclined walls, gravity.
 
-    Radii are placed largest first so the insertion reaches a dense column.
+    packing="rsa": seeded random sequential insertion into the hopper's
+    footprint column at volume fraction `fill` (largest radii first; loose,
+    no initial contacts).  packing="dense": jittered lattice at `fill`
+    relaxed to a near-touching packing (a settled-bed stand-in with contacts).
     """
     rng = np.random.RandomState(seed)
     radius = np.clip(median * np.exp(sigma * rng.standard_normal(n)), rclip[0], rclip[1])
-    radius = np.sort(radius)[::-1].copy()
-    vol = float(np.sum(4.0 / 3.0 * np.pi * radius ** 3))
-    height = vol / (fill * width * width)
     lo = np.array([-0.5 * width, -0.5 * width, 0.0])
-    hi = np.array([0.5 * width, 0.5 * width, height])
-    pos = rsa_positions(radius, lo, hi, seed=seed * 104729 + 3, max_attempts=20000)
+    if packing == "dense":
+        radius, pos = layered_lattice(radius, lo, (0.5 * width, 0.5 * width), factor=0.985)
+        relax_packing(radius, pos, lo, (0.5 * width, 0.5 * width), max_iter=60, tol=2e-3)
+    else:
+        radius = np.sort(radius)[::-1].copy()
+        vol = float(np.sum(4.0 / 3.0 * np.pi * radius ** 3))
+        height = vol / (fill * width * width)
+        hi = np.array([0.5 * width, 0.5 * width, height])
+        pos = rsa_positions(radius, lo, hi, seed=seed * 104729 + 3, max_attempts=20000)
+    # input row order carries no spatial locality (the solver sorts by cell)
+    perm = np.random.RandomState(seed + 1).permutation(n)
+    radius, pos = radius[perm].copy(), pos[perm].copy()
     kN = E * median
     m = mech(epsS=restitution, mu=mu, epsN=restitution, cohesion=0.0, beta=beta, muT=muT, muR=muR,
              kN=kN)

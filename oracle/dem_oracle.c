@@ -158,6 +158,8 @@ struct ora_solver {
     /* forces (contact.cuh:54-78) */
     int keepBodyParticle, keepBodyDetail;
     double *fcontact, *ffield, *fwall, *ftotal, *fbp, *fbd, *fbr;
+    double *bodyRow, *fzRow; /* [nb][6][n], [nb][n]: per-owner body load and Fz */
+    double* rowScratch;      /* [n] rows in cell order for the tree sums */
     int pcount, bcount, pover, bover, phigh, bhigh;
     double bmaxdepth, bfz;
     int32_t* regimeState;
@@ -195,6 +197,7 @@ void ora_destroy(ora_solver* s)
                     s->gkey, s->gid, s->tkey, s->ttri, s->hpk, s->hwk, s->hbk,
                     s->hpl, s->hwl, s->hbl, s->hpv, s->hwv, s->hbv,
                     s->fcontact, s->ffield, s->fwall, s->ftotal, s->fbp, s->fbd, s->fbr,
+                    s->bodyRow, s->fzRow, s->rowScratch,
                     s->regimeState, s->regimeInstant, s->regimeInterval, s->microState,
                     s->microInstant, s->microInterval, s->cachedRegimeInstant,
                     s->cachedRegimeInterval, s->cachedMicroInstant, s->cachedMicroInterval};
@@ -362,6 +365,9 @@ int ora_initialize(ora_solver* s, const dem_particles_view* P,
     if (s->keepBodyParticle) ALLOC(s->fbp, (size_t)s->nb * n * 6);
     if (s->keepBodyDetail) ALLOC(s->fbd, (size_t)s->nb * n * 19);
     ALLOC(s->fbr, 6 * s->nb);
+    ALLOC(s->bodyRow, (size_t)6 * s->nb * n);
+    ALLOC(s->fzRow, (size_t)s->nb * n);
+    ALLOC(s->rowScratch, n);
     ALLOC(s->regimeState, (size_t)s->nb * nc);
     ALLOC(s->regimeInstant, NREG * s->nb); ALLOC(s->regimeInterval, NREG * s->nb);
     ALLOC(s->microState, (size_t)s->nb * nc * NMICRO);
@@ -542,6 +548,8 @@ static void clear_forces(ora_solver* s)
     memset(s->microState, 0, sizeof(double) * (size_t)s->nb * s->nc * NMICRO);
     memset(s->microInstant, 0, sizeof(double) * NMICRO * s->nb);
     memset(s->fbr, 0, sizeof(double) * 6 * s->nb);
+    memset(s->bodyRow, 0, sizeof(double) * 6 * (size_t)s->nb * s->n);
+    memset(s->fzRow, 0, sizeof(double) * (size_t)s->nb * s->n);
     s->pcount = s->bcount = s->phigh = s->bhigh = 0;
     s->bmaxdepth = 0.0; s->bfz = 0.0;
 }
@@ -797,6 +805,61 @@ static void wall_contact(ora_solver* s, const Mech* m, int comp, int w, int step
     *torqueSum = add(*torqueSum, add(couple, cross(comp_off(s, comp), cf)));
 }
 
+/* ---- arc tangent ---------------------------------------------------------
+ * The area routine needs atan2.  Library atan2 differs between the CPU and
+ * the GPU in the last bits, which can flip the yield branch of a
+ * particle-triangle contact (the manifold stores a tangential value of norm
+ * exactly mu|Fn|, which the per-triangle pass compares against mu|Fn|).  Both
+ * sides therefore use this one evaluation, built only from +, -, *, /, sqrt:
+ * one half-angle reduction atan(t) = 2 atan(t / (1 + sqrt(1 + t^2))) brings
+ * t <= 1 below tan(pi/8) ~ 0.414, then the Taylor series
+ * atan(z) = z - z^3/3 + ... - z^43/43 (truncation < 1e-17 relative).  The
+ * result is within 3 ulp of the correctly rounded value. */
+static double series_atan(double t)
+{
+    t = t / (1.0 + sqrt(1.0 + t * t));
+    const double z = t * t;
+    double p = -1.0 / 43.0;
+    p = 1.0 / 41.0 + z * p;
+    p = -1.0 / 39.0 + z * p;
+    p = 1.0 / 37.0 + z * p;
+    p = -1.0 / 35.0 + z * p;
+    p = 1.0 / 33.0 + z * p;
+    p = -1.0 / 31.0 + z * p;
+    p = 1.0 / 29.0 + z * p;
+    p = -1.0 / 27.0 + z * p;
+    p = 1.0 / 25.0 + z * p;
+    p = -1.0 / 23.0 + z * p;
+    p = 1.0 / 21.0 + z * p;
+    p = -1.0 / 19.0 + z * p;
+    p = 1.0 / 17.0 + z * p;
+    p = -1.0 / 15.0 + z * p;
+    p = 1.0 / 13.0 + z * p;
+    p = -1.0 / 11.0 + z * p;
+    p = 1.0 / 9.0 + z * p;
+    p = -1.0 / 7.0 + z * p;
+    p = 1.0 / 5.0 + z * p;
+    p = -1.0 / 3.0 + z * p;
+    p = 1.0 + z * p;
+    return (t * p) * 2.0;
+}
+
+double ora_atan2(double y, double x)
+{
+    static const double kHalfPi = 1.5707963267948966192313216916398;
+    if (isnan(x) || isnan(y)) return x + y;
+    const double ax = fabs(x), ay = fabs(y);
+    double a;
+    if (ax == 0.0 && ay == 0.0) a = 0.0;
+    else if (isinf(ax) && isinf(ay)) a = 0.5 * kHalfPi;
+    else if (isinf(ax)) a = 0.0;
+    else if (isinf(ay)) a = kHalfPi;
+    else if (ay <= ax) a = series_atan(ay / ax);
+    else a = kHalfPi - series_atan(ax / ay);
+    if (signbit(x)) a = kPi - a;
+    return signbit(y) ? -a : a;
+}
+
 /* ---- sphere section vs triangle area (contact:313-407) ------------------ */
 static double cross2(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
 static double dot2(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
@@ -830,7 +893,7 @@ static double circle_edge_area(V2 a, V2 b, double radius)
         const double h = 0.5 * (ta + tb);
         const V2 mid = {a.x + d.x * h, a.y + d.y * h};
         if (dot2(mid, mid) <= radius * radius) area += 0.5 * cross2(u, v);
-        else area += 0.5 * radius * radius * atan2(cross2(u, v), dot2(u, v));
+        else area += 0.5 * radius * radius * ora_atan2(cross2(u, v), dot2(u, v));
     }
     return area;
 }
@@ -951,12 +1014,16 @@ static void body_contact(ora_solver* s, const Mech* m, int comp, int t, int slot
         for (int k = 0; k < 6; ++k) { d[3 * k] += g[k].x; d[3 * k + 1] += g[k].y; d[3 * k + 2] += g[k].z; }
         if (d[18] < depth) d[18] = depth;
     }
-    double* r = s->fbr + 6 * body;
-    r[0] += bnf.x + bfr.x; r[1] += bnf.y + bfr.y; r[2] += bnf.z + bfr.z;
-    r[3] += btq.x; r[4] += btq.y; r[5] += btq.z;
+    /* the reference adds these with atomics (contact:843-852); here each
+       owner's contributions are summed in triangle order and the owners are
+       reduced afterwards in a fixed tree (tree_sum) */
+    const size_t N = (size_t)s->n;
+    double* r = s->bodyRow + (size_t)body * 6 * N + particle;
+    r[0] += bnf.x + bfr.x; r[N] += bnf.y + bfr.y; r[2 * N] += bnf.z + bfr.z;
+    r[3 * N] += btq.x; r[4 * N] += btq.y; r[5 * N] += btq.z;
+    s->fzRow[(size_t)body * N + particle] += bnf.z + bfr.z;
     ++s->bcount;
     if (s->bmaxdepth < depth) s->bmaxdepth = depth;
-    s->bfz += bnf.z + bfr.z;
 }
 
 /* advanceBodyManifoldHistory (contact:855-949) */
@@ -1165,7 +1232,8 @@ static void observe_microstate(ora_solver* s, const Mech* m, int step)
     }
 }
 
-/* accumulateBodyRegimeKernel (contact:530-567) */
+/* accumulateBodyRegimeKernel (contact:530-567): integer counters in any
+   order; the 22 microstate sums per body through tree_sum */
 static void accumulate_regime(ora_solver* s, int interval)
 {
     for (int index = 0; index < s->nb * s->nc; ++index) {
@@ -1180,15 +1248,69 @@ static void accumulate_regime(ora_solver* s, int interval)
         if (sl) ri[4]++;
         if (tw) ri[5]++;
         if (sl && tw) ri[6]++;
-        const double* ms = s->microState + (size_t)index * NMICRO;
-        for (int f = 0; f < NMICRO; ++f) s->microInstant[NMICRO * body + f] += ms[f];
         if (!interval) continue;
         rv[0]++; rv[primary]++;
         if (sl) rv[4]++;
         if (tw) rv[5]++;
         if (sl && tw) rv[6]++;
-        for (int f = 0; f < NMICRO; ++f) s->microInterval[NMICRO * body + f] += ms[f];
     }
+}
+
+static double tree_sum(ora_solver* s, const double* row);
+
+static void reduce_body_rows(ora_solver* s, int interval)
+{
+    const size_t N = (size_t)s->n;
+    double* row = malloc(sizeof(double) * N);
+    s->bfz = 0.0;
+    for (int b = 0; b < s->nb; ++b) {
+        for (int k = 0; k < 6; ++k) s->fbr[6 * b + k] = tree_sum(s, s->bodyRow + ((size_t)b * 6 + k) * N);
+        s->bfz += tree_sum(s, s->fzRow + (size_t)b * N);
+        for (int f = 0; f < NMICRO; ++f) {
+            for (size_t c = 0; c < N; ++c) row[c] = s->microState[((size_t)b * s->nc + c) * NMICRO + f];
+            const double v = tree_sum(s, row);
+            s->microInstant[NMICRO * b + f] = v;
+            if (interval) s->microInterval[NMICRO * b + f] += v;
+        }
+    }
+    free(row);
+}
+
+/* Sum of one per-owner row taken in cell order (the sorted grid order) with
+ * a fixed tree: tiles of 4096 rows; in a tile, 256 partial sums over rows
+ * tid, tid + 256, ...; each group of 64 partials folded pairwise with
+ * partner lane ^ 32, ^ 16, ..., ^ 1; the 4 group results added in order; the
+ * tile results added in order.  Any order is a faithful restatement of the
+ * reference's atomic sums; this one is deterministic. */
+static double tree_sum(ora_solver* s, const double* row)
+{
+    const long n = s->n;
+    double* v = s->rowScratch;
+    for (long t = 0; t < n; ++t) v[t] = row[s->cowner[s->gid[t]]];
+    double total = 0.0;
+    for (long base = 0; base < n; base += 4096) {
+        double part[256];
+        for (int tid = 0; tid < 256; ++tid) {
+            double acc = 0.0;
+            for (int k = 0; k < 16; ++k) {
+                const long e = base + (long)k * 256 + tid;
+                if (e < n) acc += v[e];
+            }
+            part[tid] = acc;
+        }
+        double blk = 0.0;
+        for (int w = 0; w < 4; ++w) {
+            double lane[64], nxt[64];
+            for (int l = 0; l < 64; ++l) lane[l] = part[w * 64 + l];
+            for (int o = 32; o > 0; o >>= 1) {
+                for (int l = 0; l < 64; ++l) nxt[l] = lane[l] + lane[l ^ o];
+                memcpy(lane, nxt, sizeof lane);
+            }
+            blk += lane[0];
+        }
+        total += blk;
+    }
+    return total;
 }
 
 /* accelerationKernel (particle:71-101) */
@@ -1234,6 +1356,7 @@ int ora_compute_forces(ora_solver* s, const double* pose13, double* loads6, int 
     if (s->nb > 0) {
         observe_microstate(s, &bm, step);
         accumulate_regime(s, bm.dt > 0.0);
+        reduce_body_rows(s, bm.dt > 0.0);
     }
     if (bm.dt > 0.0 && s->nb > 0) ++s->intervalSteps;
     acceleration(s);

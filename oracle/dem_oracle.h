@@ -63,6 +63,7 @@ double ora_circle_triangle_area(const double center[3], const double normal[3],
                                 const double a[3], const double b[3],
                                 const double c[3], double radius);
 uint64_t ora_pack_cell(int x, int y, int z);
+double ora_atan2(double y, double x);
 
 #ifdef __cplusplus
 }
