@@ -156,10 +156,10 @@ struct Body {
 };
 
 // particleContact (contact:579-706) after the overlap test has passed and
-// the history slot has been found.  Returns the force on i and adds its
-// torque contribution.
+// the history slot has been found: the contact force on i (cf) and the term
+// the owner adds to its torque sum (couple + offset x cf).
 __device__ __forceinline__ void pp_law(const Body& I, const Body& J, V3 delta, double d2,
-                                       const Material& m, double* slot, V3& fsum, V3& tsum)
+                                       const Material& m, double* slot, V3& cf_out, V3& tq_out)
 {
     const double ri = I.r, rj = J.r, rs = ri + rj;
     const double dist = sqrt(d2);
@@ -202,8 +202,8 @@ __device__ __forceinline__ void pp_law(const Body& I, const Body& J, V3 delta, d
                           m.muR * ms, slot, 0, true, nullptr);
     const V3 cf = add(fn, fr);
     const V3 couple = add(add(tw, rl), scl(cross(dir, fr), li));
-    fsum = add(fsum, cf);
-    tsum = add(tsum, add(couple, cross(I.off, cf)));
+    cf_out = cf;
+    tq_out = add(couple, cross(I.off, cf));
 }
 
 // wallContact (contact:975-1063) after the depth test and slot lookup.

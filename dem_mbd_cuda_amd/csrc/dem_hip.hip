@@ -13,10 +13,15 @@
 //   k_permute    every reorder_interval steps the whole particle SoA is
 //                gathered into cell order so every later access is coalesced
 //   k_cellstart  dense cell -> first sorted index table (when it fits)
-//   k_posr       packed (x, y, z, r) in cell order for the candidate scan
+//   k_posr       packed (x, y, z, r) and component id in cell order
+//   k_pack       per-particle contact record (one 160-byte line per row)
 //   k_scan       27-cell candidate scan, nine contiguous column ranges per
-//                particle, writes each particle's contact list
-//   k_law        walls, particle pairs, triangles, history, accelerations
+//                particle, then history expiry and one slot per contact
+//   k_off_*      prefix sum of list lengths, compacted (owner, entry) list
+//   k_pairs      particle-particle law, one thread per contact
+//   k_law_pt     triangle manifold and contacts per body (coupled runs)
+//   k_law        walls, field, ordered sum of the pair results, triangle
+//                accumulator, accelerations
 //   k_reduce     deterministic row sums for the body loads and diagnostics
 //
 // History is indexed by input row, so it never moves when particles are
@@ -64,6 +69,10 @@ constexpr int PP_SLOTS = DEM_PARTICLE_HISTORY_SLOTS;
 constexpr int PW_SLOTS = DEM_WALL_HISTORY_SLOTS;
 constexpr int PT_SLOTS = DEM_BODY_HISTORY_SLOTS;
 constexpr int MAX_LIST = PP_SLOTS;  // contact-list capacity per component
+constexpr int SLOT_BITS = 6;        // list entry = partner row << SLOT_BITS | slot
+static_assert((1 << SLOT_BITS) == PP_SLOTS, "slot field width");
+constexpr long MAX_PARTICLES = (1L << (31 - SLOT_BITS)) - 1;
+constexpr int PAIR_BLOCKS = 8192;   // grid of the per-contact law (grid-stride)
 constexpr int BLOCK = 256;
 
 // Double fields of the particle SoA (one array of n doubles each).
@@ -101,6 +110,7 @@ struct Scalars {
     unsigned long long body_max_depth_bits;
     int tri_overflow;
     int pad;
+    long long total_pairs;
 };
 
 struct History {
@@ -358,40 +368,57 @@ __global__ void __launch_bounds__(BLOCK) k_rs_hist(const unsigned long long* key
     counts[(long)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the [256][nblocks] digit-major counts in one block of
-// 1024 threads: four threads per digit each fold a quarter of the blocks,
-// the 256 digit totals are scanned in LDS, then each thread writes the
-// running offsets of its quarter.
-__global__ void __launch_bounds__(1024) k_rs_scan(unsigned* counts, int nblocks, int shift,
-                                                  const int* nbits)
+// Block-wide exclusive scan of one value per thread (wave prefix sums with
+// shuffles, then the four wave totals through LDS).  Returns the exclusive
+// prefix; *total receives the block sum.
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* total)
+{
+    __shared__ unsigned wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    unsigned before = 0, all = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// Digit-major counts [256][nblocks]: block d turns row d into exclusive
+// offsets within the digit and records the digit total.
+__global__ void __launch_bounds__(BLOCK) k_rs_scan_rows(unsigned* counts, int nblocks, int shift,
+                                                        const int* nbits, unsigned* digitTotal)
 {
     if (!pass_active(nbits, shift)) return;
-    __shared__ unsigned part[4][256];
-    __shared__ unsigned base[256];
-    const int d = threadIdx.x & 255, q = threadIdx.x >> 8;
-    const int cb = (nblocks + 3) / 4;
-    const int b0 = q * cb, b1 = min(nblocks, b0 + cb);
-    unsigned* row = counts + (long)d * nblocks;
-    unsigned sum = 0;
-    for (int b = b0; b < b1; ++b) sum += row[b];
-    part[q][d] = sum;
-    __syncthreads();
-    if (threadIdx.x < 256) base[d] = part[0][d] + part[1][d] + part[2][d] + part[3][d];
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        unsigned v = 0;
-        if (threadIdx.x < 256 && d >= off) v = base[d - off];
-        __syncthreads();
-        if (threadIdx.x < 256) base[d] += v;
-        __syncthreads();
+    unsigned* row = counts + (long)blockIdx.x * nblocks;
+    unsigned carry = 0;
+    for (int b0 = 0; b0 < nblocks; b0 += BLOCK) {
+        const int b = b0 + threadIdx.x;
+        const unsigned v = b < nblocks ? row[b] : 0u;
+        unsigned tot;
+        const unsigned ex = block_exclusive_scan(v, &tot);
+        if (b < nblocks) row[b] = carry + ex;
+        carry += tot;
     }
-    unsigned run = base[d] - (part[0][d] + part[1][d] + part[2][d] + part[3][d]);
-    for (int k = 0; k < q; ++k) run += part[k][d];
-    for (int b = b0; b < b1; ++b) {
-        const unsigned c = row[b];
-        row[b] = run;
-        run += c;
-    }
+    if (threadIdx.x == 0) digitTotal[blockIdx.x] = carry;
+}
+
+// Exclusive scan of the 256 digit totals into digit bases.
+__global__ void __launch_bounds__(BLOCK) k_rs_scan_base(unsigned* digitTotal, int shift, const int* nbits)
+{
+    if (!pass_active(nbits, shift)) return;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan(digitTotal[threadIdx.x], &tot);
+    __syncthreads();
+    digitTotal[256 + threadIdx.x] = ex;
 }
 
 // After the pass slots the data sits in the second buffer when an odd number
@@ -411,13 +438,14 @@ __global__ void __launch_bounds__(BLOCK) k_rs_copyback(unsigned long long* ka, i
 __global__ void __launch_bounds__(BLOCK) k_rs_scatter(const unsigned long long* kin, const int* vin,
                                                       unsigned long long* kout, int* vout, long n,
                                                       int shift, const int* nbits,
-                                                      const unsigned* counts, int nblocks)
+                                                      const unsigned* counts, int nblocks,
+                                                      const unsigned* digitBase)
 {
     if (!pass_active(nbits, shift)) return;
     __shared__ unsigned base[256];
     __shared__ unsigned wc[BLOCK / 64][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    base[tid] = counts[(long)tid * nblocks + blockIdx.x];
+    base[tid] = digitBase[tid] + counts[(long)tid * nblocks + blockIdx.x];
     const long tile = (long)blockIdx.x * RS_TILE;
     const unsigned long long below = (1ull << lane) - 1ull;
     for (int r = 0; r < RS_ITEMS; ++r) {
@@ -511,7 +539,55 @@ __global__ void __launch_bounds__(BLOCK) k_cellstart(const unsigned long long* k
         start[c] = (unsigned)lower_bound_u64(key, n, c);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool offs, double4* posr)
+// Packed contact view of every particle, by physical index: the component
+// centre, endpoint and half-step velocities, world angular velocities, mass,
+// radius, status and component id (the fields particleContact reads for
+// both sides, contact:585-609), 160 B so a neighbour costs a few cache lines.
+struct alignas(32) Rec {
+    double px, py, pz, vx, vy, vz, hx, hy, hz, wx, wy, wz, ux, uy, uz, mass, r;
+    int status;
+    uint32_t cid;
+    double pad0, pad1;
+};
+static_assert(sizeof(Rec) == 160, "contact record layout");
+
+__device__ __forceinline__ Body rec_body(const Rec* R, long p)
+{
+    const Rec& q = R[p];
+    Body B;
+    B.pos = mk(q.px, q.py, q.pz);
+    B.vel = mk(q.vx, q.vy, q.vz);
+    B.vh = mk(q.hx, q.hy, q.hz);
+    B.w = mk(q.wx, q.wy, q.wz);
+    B.wh = mk(q.ux, q.uy, q.uz);
+    B.mass = q.mass;
+    B.r = q.r;
+    B.status = q.status;
+    B.cid = q.cid;
+    B.off = zero3();
+    return B;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Rec* R)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n) return;
+    const Body B = load_body(S, p, offs);
+    Rec q;
+    q.px = B.pos.x; q.py = B.pos.y; q.pz = B.pos.z;
+    q.vx = B.vel.x; q.vy = B.vel.y; q.vz = B.vel.z;
+    q.hx = B.vh.x; q.hy = B.vh.y; q.hz = B.vh.z;
+    q.wx = B.w.x; q.wy = B.w.y; q.wz = B.w.z;
+    q.ux = B.wh.x; q.uy = B.wh.y; q.uz = B.wh.z;
+    q.mass = B.mass; q.r = B.r; q.status = B.status; q.cid = B.cid;
+    q.pad0 = 0.0; q.pad1 = 0.0;
+    R[p] = q;
+}
+
+// Contact centre, radius and component id in sorted order (what the
+// candidate scan reads per candidate).
+__global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool offs, double4* posr,
+                                                uint32_t* cidS)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= S.n) return;
@@ -521,55 +597,7 @@ __global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool o
     else { v.x = S.f(F_X, p); v.y = S.f(F_Y, p); v.z = S.f(F_Z, p); }
     v.w = S.f(F_CR, p);
     posr[t] = v;
-}
-
-// 27-cell candidate scan -> contact list (particleContact's overlap test,
-// contact:579-591, evaluated in the reference's cell and entry order).
-__global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const double4* posr,
-                                                const unsigned long long* key, const unsigned* start,
-                                                const Scalars* sc, int* nbr, int* ncount)
-{
-    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
-    const long n = S.n;
-    if (t >= n) return;
-    const long p = sorted[t];
-    int count = 0;
-    if (S.k(I_VALID, p)) {
-        const double4 me = posr[t];
-        const long X = (long)sc->dims[0], Y = (long)sc->dims[1], Z = (long)sc->dims[2];
-        const long cx = S.k(I_CX, p) - sc->lo[0], cy = S.k(I_CY, p) - sc->lo[1],
-                   cz = S.k(I_CZ, p) - sc->lo[2];
-        const bool dense = sc->dense;
-        const long zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < Z ? cz + 1 : Z - 1;
-        for (long dx = -1; dx <= 1; ++dx) {
-            const long xx = cx + dx;
-            if (xx < 0 || xx >= X) continue;
-            for (long dy = -1; dy <= 1; ++dy) {
-                const long yy = cy + dy;
-                if (yy < 0 || yy >= Y) continue;
-                const unsigned long long col = ((unsigned long long)xx * Y + yy) * Z;
-                long b, e;
-                if (dense) {
-                    b = start[col + zlo];
-                    e = start[col + zhi + 1];
-                } else {
-                    b = lower_bound_u64(key, n, col + zlo);
-                    e = lower_bound_u64(key, n, col + zhi + 1);
-                }
-                for (long k = b; k < e; ++k) {
-                    if (k == t) continue;
-                    const double4 o = posr[k];
-                    const double dx_ = o.x - me.x, dy_ = o.y - me.y, dz_ = o.z - me.z;
-                    const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
-                    const double rs = me.w + o.w;
-                    if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
-                    if (count < MAX_LIST) nbr[(long)count * n + t] = sorted[k];
-                    ++count;
-                }
-            }
-        }
-    }
-    ncount[t] = count;
+    cidS[t] = (uint32_t)S.k(I_CID, p);
 }
 
 // History slot acquisition (acquireHistorySlot, contact:190-215) on an
@@ -606,9 +634,11 @@ __device__ __forceinline__ int acquire(M& occ, M& cur, uint32_t* keys, double* v
 
 struct LawArgs {
     Soa S;
+    const Rec* rec;
     const int* sorted;
-    const int* nbr;
     const int* ncount;
+    const unsigned* off;
+    const double* pairF;
     History H;
     Walls W;
     TriSet T;
@@ -623,6 +653,206 @@ struct LawArgs {
     bool offs;
     int nb;
 };
+
+// History lookup with the first 16 keys of the owner held in registers (kept
+// in step with memory when a slot is reassigned): a hit among them needs no
+// dependent load.  Same result as acquire().
+__device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long long& cur,
+                                          uint32_t (&k16)[16], uint32_t* keys, double* vals,
+                                          uint32_t key, int& high, int& overflow)
+{
+    unsigned match = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) match |= (k16[i] == key ? 1u : 0u) << i;
+    match &= (unsigned)(occ & 0xFFFFull);
+    int b = -1;
+    if (match) {
+        b = __ffs(match) - 1;
+    } else {
+        unsigned long long m = occ & ~0xFFFFull;
+        while (m) {
+            const int c = (int)__ffsll(m) - 1;
+            if (keys[c] == key) { b = c; break; }
+            m &= m - 1;
+        }
+    }
+    if (b >= 0) {
+        cur |= 1ull << b;
+        high = max(high, b + 1);
+        return b;
+    }
+    const unsigned long long freem = ~occ;
+    if (!freem) {
+        ++overflow;
+        return -1;
+    }
+    b = (int)__ffsll(freem) - 1;
+    occ |= 1ull << b;
+    cur |= 1ull << b;
+    keys[b] = key;
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        if (i == b) k16[i] = key;  // keep the register copy current (a reused slot)
+    double* v = vals + (long)b * 9;
+    for (int q = 0; q < 9; ++q) v[q] = 0.0;
+    high = max(high, b + 1);
+    return b;
+}
+
+// 27-cell candidate scan (particleContact's overlap test, contact:579-591, in
+// the reference's cell and entry order: the nine (dx, dy) columns, each one
+// contiguous run of the sorted array over dz = -1..1) followed by the
+// particle-particle history: expiry at the start of the owner's step
+// (expireHistory, contact:217-229) and one slot per contact in candidate
+// order (acquireHistorySlot, contact:190-215).  The scan first records the
+// overlapping candidates, so the slot lookups (dependent loads) run once per
+// contact rather than once per candidate iteration of the wave.  Leaves the
+// owner's contact list as entries (partner row << SLOT_BITS | slot), k-major.
+__global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const double4* posr,
+                                                const uint32_t* cidS, const unsigned long long* key,
+                                                const unsigned* start, Scalars* sc, History H, int step,
+                                                int* nbr, int* ncount)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long n = S.n;
+    int found = 0, stored = 0, high = 0, over = 0;
+    if (t < n) {
+        const long p = sorted[t];
+        if (S.k(I_VALID, p)) {
+            const double4 me = posr[t];
+            const long X = (long)sc->dims[0], Y = (long)sc->dims[1], Z = (long)sc->dims[2];
+            const long cx = S.k(I_CX, p) - sc->lo[0], cy = S.k(I_CY, p) - sc->lo[1],
+                       cz = S.k(I_CZ, p) - sc->lo[2];
+            const bool dense = sc->dense;
+            const long zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < Z ? cz + 1 : Z - 1;
+            for (long dx = -1; dx <= 1; ++dx) {
+                const long xx = cx + dx;
+                if (xx < 0 || xx >= X) continue;
+                for (long dy = -1; dy <= 1; ++dy) {
+                    const long yy = cy + dy;
+                    if (yy < 0 || yy >= Y) continue;
+                    const unsigned long long col = ((unsigned long long)xx * Y + yy) * Z;
+                    long b, e;
+                    if (dense) {
+                        b = start[col + zlo];
+                        e = start[col + zhi + 1];
+                    } else {
+                        b = lower_bound_u64(key, n, col + zlo);
+                        e = lower_bound_u64(key, n, col + zhi + 1);
+                    }
+                    for (long k = b; k < e; ++k) {
+                        if (k == t) continue;
+                        const double4 c = posr[k];
+                        const double dx_ = c.x - me.x, dy_ = c.y - me.y, dz_ = c.z - me.z;
+                        const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+                        const double rs = me.w + c.w;
+                        if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
+                        if (found < MAX_LIST) nbr[(long)found * n + t] = (int)k;
+                        ++found;
+                    }
+                }
+            }
+        }
+        if (found > MAX_LIST) {
+            atomicAdd(&sc->list_overflow, 1);
+            found = MAX_LIST;
+        }
+        const long o = S.k(I_ORIG, p);
+        unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
+        unsigned long long cur = 0;
+        uint32_t* keys = H.ppKey + o * PP_SLOTS;
+        double* vals = H.ppVal + o * PP_SLOTS * 9;
+        uint32_t k16[16];
+        {
+            const uint4* kv = reinterpret_cast<const uint4*>(keys);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint4 v = kv[i];
+                k16[4 * i] = v.x; k16[4 * i + 1] = v.y; k16[4 * i + 2] = v.z; k16[4 * i + 3] = v.w;
+            }
+        }
+        for (int c = 0; c < found; ++c) {
+            const long k = nbr[(long)c * n + t];
+            const int slot = acquire_pp(occ, cur, k16, keys, vals, cidS[k] + 1u, high, over);
+            if (slot < 0) continue;  // slot overflow: counted, no force
+            nbr[(long)stored * n + t] = (sorted[k] << SLOT_BITS) | slot;
+            ++stored;
+        }
+        H.ppOcc[o] = occ;
+        H.ppSeen[(long)(step & 1) * n + o] = cur;
+        ncount[t] = stored;
+    }
+    atomic_add_wave(&sc->particle_contacts, stored);
+    atomic_max_wave(&sc->pp_high, high);
+    atomic_add_wave(&sc->pp_overflow, over);
+}
+
+// Compacts the k-major contact lists into one array ordered by owner (CSR):
+// exclusive prefix sum of the list lengths in three passes (block sums, one
+// block scanning the sums, per-block offsets), the last also writing each
+// contact's (owner, entry) at its position.
+__global__ void __launch_bounds__(BLOCK) k_off_blocks(const int* cnt, long n, unsigned* bsum)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    unsigned tot;
+    block_exclusive_scan(t < n ? (unsigned)cnt[t] : 0u, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_off_scan(unsigned* bsum, int nblocks, Scalars* sc)
+{
+    unsigned carry = 0;
+    for (int b0 = 0; b0 < nblocks; b0 += BLOCK) {
+        const int b = b0 + threadIdx.x;
+        const unsigned v = b < nblocks ? bsum[b] : 0u;
+        unsigned tot;
+        const unsigned ex = block_exclusive_scan(v, &tot);
+        if (b < nblocks) bsum[b] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) sc->total_pairs = carry;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_off_apply(const int* cnt, long n, const unsigned* bsum,
+                                                     const int* nbr, unsigned* off, int2* pairs)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const int c = t < n ? cnt[t] : 0;
+    unsigned tot;
+    const unsigned ex = block_exclusive_scan((unsigned)c, &tot);
+    if (t < n) {
+        const unsigned base = bsum[blockIdx.x] + ex;
+        off[t] = base;
+        for (int k = 0; k < c; ++k) pairs[base + k] = make_int2((int)t, nbr[(long)k * n + t]);
+    }
+}
+
+// particleContact (contact:591-706) for every listed contact, one thread per
+// contact (grid-stride over the compacted list), so uneven contact counts do
+// not idle lanes.  Writes the contact force on the owner and its torque term.
+__global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const Rec* rec,
+                                                 const int2* pairs, History H, Material m, bool offs,
+                                                 const Scalars* sc, double* pairF)
+{
+    const long total = sc->total_pairs;
+    for (long e = (long)blockIdx.x * BLOCK + threadIdx.x; e < total; e += (long)gridDim.x * BLOCK) {
+        const int2 pr = pairs[e];
+        const long p = sorted[pr.x];
+        const long q = pr.y >> SLOT_BITS;
+        const int slot = pr.y & (PP_SLOTS - 1);
+        Body I = rec_body(rec, p);
+        if (offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
+        const Body J = rec_body(rec, q);
+        const V3 delta = sub(J.pos, I.pos);
+        const double d2 = dot(delta, delta);
+        const long o = S.k(I_ORIG, p);
+        V3 cf, tq;
+        pp_law(I, J, delta, d2, m, H.ppVal + (o * PP_SLOTS + slot) * 9, cf, tq);
+        double* out = pairF + e * 6;
+        out[0] = cf.x; out[1] = cf.y; out[2] = cf.z;
+        out[3] = tq.x; out[4] = tq.y; out[5] = tq.z;
+    }
+}
 
 __device__ __forceinline__ Tri load_tri(const TriSet& T, int t)
 {
@@ -666,12 +896,13 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const Soa& S = A.S;
     const long n = S.n;
-    int pcount = 0, ppHigh = 0, bodyHigh = 0, ppOver = 0, pwOver = 0;
+    int bodyHigh = 0, pwOver = 0;
     if (t < n) {
         const long p = A.sorted[t];
         const long o = S.k(I_ORIG, p);
         const int s = A.step;
-        const Body I = load_body(S, p, A.offs);
+        Body I = rec_body(A.rec, p);
+        if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
         V3 cF = zero3(), cT = zero3(), fF = zero3(), wF = zero3(), wT = zero3(), bF = zero3(),
            bT = zero3();
         if (A.gravity) fF = mk(I.mass * A.g[0], I.mass * A.g[1], I.mass * A.g[2]);
@@ -686,11 +917,8 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
                     fF = add(fF, scl(d, kGravityConstant * I.mass * S.f(F_MASS, j) / (d2 * sqrt(d2))));
             }
         }
-        // expiry (expireHistory, contact:217-229): drop slots not seen last step
+        // wall expiry (expireHistory, contact:217-229): drop slots not seen last step
         const History& H = A.H;
-        unsigned long long ppOcc = H.ppOcc[o];
-        ppOcc &= H.ppSeen[(long)((s + 1) & 1) * n + o];
-        unsigned long long ppCur = 0;
         unsigned pwOcc = H.pwOcc[o] & H.pwSeen[(long)((s + 1) & 1) * n + o];
         unsigned pwCur = 0;
         // walls
@@ -705,28 +933,20 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
             if (slot < 0) continue;
             pw_law(I, outward, depth, A.pw, H.pwVal + (o * PW_SLOTS + slot) * 9, wF, wT);
         }
-        // particle pairs, in candidate order
-        const int cnt = min(A.ncount[t], MAX_LIST);
+        // particle pairs, summed in candidate order from the per-contact results
+        const int cnt = A.ncount[t];
+        const long base = A.off[t];
         for (int c = 0; c < cnt; ++c) {
-            const long q = A.nbr[(long)c * n + t];
-            const Body J = load_body(S, q, A.offs);
-            const V3 delta = sub(J.pos, I.pos);
-            const double d2 = dot(delta, delta);
-            const int slot = acquire(ppOcc, ppCur, H.ppKey + o * PP_SLOTS, H.ppVal + o * PP_SLOTS * 9,
-                                     PP_SLOTS, J.cid + 1u, ppHigh, ppOver);
-            if (slot < 0) continue;
-            pp_law(I, J, delta, d2, A.pp, H.ppVal + (o * PP_SLOTS + slot) * 9, cF, cT);
-            ++pcount;
+            const double* pf = A.pairF + (base + c) * 6;
+            cF = add(cF, mk(pf[0], pf[1], pf[2]));
+            cT = add(cT, mk(pf[3], pf[4], pf[5]));
         }
-        if (A.ncount[t] > MAX_LIST) ppOver += A.ncount[t] - MAX_LIST;
         // triangle contacts were evaluated by k_law_pt (separate accumulator)
         if (A.nb > 0) {
             const double* ob_ = A.O.body;
             bF = mk(ob_[0 * n + p], ob_[1 * n + p], ob_[2 * n + p]);
             bT = mk(ob_[3 * n + p], ob_[4 * n + p], ob_[5 * n + p]);
         }
-        H.ppOcc[o] = ppOcc;
-        H.ppSeen[(long)(s & 1) * n + o] = ppCur;
         H.pwOcc[o] = pwOcc;
         H.pwSeen[(long)(s & 1) * n + o] = pwCur;
         // outputs (contact:1211-1220)
@@ -758,10 +978,7 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
         S.f(F_AWY, p) = (ty - (ix - iz) * wx * wz) / iy;
         S.f(F_AWZ, p) = (tz - (iy - ix) * wy * wx) / iz;
     }
-    atomic_add_wave(&A.sc->particle_contacts, pcount);
-    atomic_max_wave(&A.sc->pp_high, ppHigh);
     atomic_max_wave(&A.sc->body_high, bodyHigh);
-    atomic_add_wave(&A.sc->pp_overflow, ppOver);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
 }
 
@@ -780,7 +997,8 @@ __global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
         const long o = S.k(I_ORIG, p);
         const int s = A.step;
         const History& H = A.H;
-        const Body I = load_body(S, p, A.offs);
+        Body I = rec_body(A.rec, p);
+        if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
         V3 bF = zero3(), bT = zero3();
         // triangles of the own cell (contact:1148-1208)
         {
@@ -1085,6 +1303,44 @@ __global__ void __launch_bounds__(BLOCK) k_finish(Soa S, double dt)
     half_quat(S, p, 0.5 * dt);
 }
 
+// advanceHalfVelocityKernel of the previous step followed by
+// advancePositionKernel of the next one (particle:213-265), in one pass over
+// the state: the reference's loop calls them back to back with only host work
+// (the body solve) in between.
+__global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, double dt)
+{
+    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= S.n) return;
+    double vhx = S.f(F_VHX, p) + S.f(F_AX, p) * dtf, vhy = S.f(F_VHY, p) + S.f(F_AY, p) * dtf,
+           vhz = S.f(F_VHZ, p) + S.f(F_AZ, p) * dtf;
+    double whx = S.f(F_WHX, p) + S.f(F_AWX, p) * dtf, why = S.f(F_WHY, p) + S.f(F_AWY, p) * dtf,
+           whz = S.f(F_WHZ, p) + S.f(F_AWZ, p) * dtf;
+    double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+    const double h = 0.5 * dtf;
+    double dw, dx, dy, dz;
+    quat_dot(qw, qx, qy, qz, S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p), dw, dx, dy, dz);
+    double hw = qw + dw * h, hx = qx + dx * h, hy = qy + dy * h, hz = qz + dz * h;
+    quat_unit(hw, hx, hy, hz);
+    double gw, gx, gy, gz;
+    quat_dot(hw, hx, hy, hz, whx, why, whz, gw, gx, gy, gz);
+    S.f(F_VHX, p) = vhx; S.f(F_VHY, p) = vhy; S.f(F_VHZ, p) = vhz;
+    S.f(F_WHX, p) = whx; S.f(F_WHY, p) = why; S.f(F_WHZ, p) = whz;
+    S.f(F_HQW, p) = gw; S.f(F_HQX, p) = gx; S.f(F_HQY, p) = gy; S.f(F_HQZ, p) = gz;
+    if (S.k(I_STATUS, p) == -1) return;
+    S.f(F_X, p) += vhx * dt;
+    S.f(F_Y, p) += vhy * dt;
+    S.f(F_Z, p) += vhz * dt;
+    qw = qw + gw * dt; qx = qx + gx * dt; qy = qy + gy * dt; qz = qz + gz * dt;
+    quat_unit(qw, qx, qy, qz);
+    S.f(F_QW, p) = qw; S.f(F_QX, p) = qx; S.f(F_QY, p) = qy; S.f(F_QZ, p) = qz;
+    S.f(F_VX, p) = vhx + S.f(F_AX, p) * (0.5 * dt);
+    S.f(F_VY, p) = vhy + S.f(F_AY, p) * (0.5 * dt);
+    S.f(F_VZ, p) = vhz + S.f(F_AZ, p) * (0.5 * dt);
+    S.f(F_WX, p) = whx + S.f(F_AWX, p) * (0.5 * dt);
+    S.f(F_WY, p) = why + S.f(F_AWY, p) * (0.5 * dt);
+    S.f(F_WZ, p) = whz + S.f(F_AWZ, p) * (0.5 * dt);
+}
+
 // ------------------------------------------------------------ triangles
 // transformTrianglesKernel (body:71-112)
 __global__ void __launch_bounds__(BLOCK) k_transform(TriSet T)
@@ -1232,6 +1488,10 @@ int blocks_for(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
 }  // namespace
 
 // ============================================================ solver object
+// timing events: grid build | contact scan + list compaction | pair law |
+// owner sums (triangles, walls, field, acceleration) | body reductions
+constexpr int NEV = 6;
+
 struct dem_solver {
     bool initialized = false;
     int device = 0;
@@ -1259,13 +1519,22 @@ struct dem_solver {
     int* valA = nullptr;
     int* valB = nullptr;
     unsigned* rsCounts = nullptr;
+    unsigned* rsDigits = nullptr;  // [256] digit totals, [256] digit bases
     int rsBlocks = 0;
     unsigned* cellStart = nullptr;
     int* boxPart = nullptr;
     unsigned long long cellCap = 0;
     double4* posr = nullptr;
-    int* nbr = nullptr;
-    int* ncount = nullptr;
+    uint32_t* cidS = nullptr;
+    Rec* rec = nullptr;
+    bool pendingFinish = false;
+    double pendingDt = 0.0;
+    int* nbr = nullptr;      // k-major contact lists [MAX_LIST][n]
+    int* ncount = nullptr;   // list length per owner (sorted index)
+    unsigned* listOff = nullptr;  // [n] owner's first position in the compacted list
+    unsigned* listBlk = nullptr;  // [blocks] per-block offsets
+    int2* pairs = nullptr;   // compacted (owner, entry) [MAX_LIST * n]
+    double* pairF = nullptr; // per-contact force and torque term [MAX_LIST * n][6]
     int* byorig = nullptr;
     History H{};
     double* wallBuf = nullptr;
@@ -1306,7 +1575,7 @@ struct dem_solver {
     std::vector<double> cMicInst, cMicInt;
     bool keepBodyParticle = false, keepBodyDetail = false;
     // timing
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[NEV] = {};
     bool evReady = false;
     // metadata for checkpoints
     std::vector<int> compIds, ownerIds;
@@ -1318,7 +1587,7 @@ struct dem_solver {
     {
         if (stream) hipStreamSynchronize(stream);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
-                        rsCounts, cellStart, boxPart, posr, nbr, ncount, byorig, H.ppOcc, H.ppSeen, H.ppKey,
+                        rec, rsCounts, rsDigits, cellStart, boxPart, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart, O.fzPart,
                         O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, tLocal, tWorld, tAabb,
@@ -1362,7 +1631,7 @@ int launch_check(const char* what)
 // Stable LSD radix sort of (key, val) pairs held in (ka, va); (kb, vb) is
 // scratch.  Passes over bits >= *nbits (device) are skipped in pairs.
 int radix_sort(hipStream_t st, unsigned long long* ka, int* va, unsigned long long* kb, int* vb,
-               long n, unsigned* counts, int nblocks, const int* nbits_dev)
+               long n, unsigned* counts, int nblocks, const int* nbits_dev, unsigned* digits)
 {
     if (n <= 0) return DEM_OK;
     for (int shift = 0; shift < 64; shift += 8) {
@@ -1373,9 +1642,11 @@ int radix_sort(hipStream_t st, unsigned long long* ka, int* va, unsigned long lo
         int* vout = even ? vb : va;
         hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(BLOCK), 0, st, kin, n, shift, nbits_dev,
                            counts, nblocks);
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, counts, nblocks, shift, nbits_dev);
+        hipLaunchKernelGGL(k_rs_scan_rows, dim3(256), dim3(BLOCK), 0, st, counts, nblocks, shift, nbits_dev,
+                           digits);
+        hipLaunchKernelGGL(k_rs_scan_base, dim3(1), dim3(BLOCK), 0, st, digits, shift, nbits_dev);
         hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(BLOCK), 0, st, kin, vin, kout, vout, n,
-                           shift, nbits_dev, counts, nblocks);
+                           shift, nbits_dev, counts, nblocks, digits + 256);
     }
     const int cb = (int)std::min<long>((n + BLOCK - 1) / BLOCK, 4096);
     hipLaunchKernelGGL(k_rs_copyback, dim3(cb), dim3(BLOCK), 0, st, ka, va, kb, vb, n, nbits_dev);
@@ -1441,6 +1712,8 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
     if (s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver already initialized");
     if (!dem_backend_available()) return set_error(DEM_E_DEVICE, "no CUDA device is available");
     if (P->count <= 0) return set_error(DEM_E_INVALID, "CUDA DEM solver requires particles");
+    if (P->count > MAX_PARTICLES)
+        return set_error(DEM_E_CAPACITY, "particle count exceeds the contact-list row range (2^25 - 1)");
     const long n = P->count;
     {
         std::vector<int> ids(P->id, P->id + n);
@@ -1542,12 +1815,19 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
     TRY(dalloc(s->valB, n, "hipMalloc(grid ids)"));
     s->rsBlocks = (int)((n + RS_TILE - 1) / RS_TILE);
     TRY(dalloc(s->rsCounts, (size_t)256 * s->rsBlocks, "hipMalloc(sort counts)"));
+    TRY(dalloc(s->rsDigits, 512, "hipMalloc(sort digit totals)"));
     s->cellCap = (unsigned long long)std::max<long>(8 * n + 64, 1 << 20);
     TRY(dalloc(s->cellStart, s->cellCap, "hipMalloc(cell table)"));
     TRY(dalloc(s->posr, n, "hipMalloc(packed positions)"));
+    TRY(dalloc(s->cidS, n, "hipMalloc(sorted component ids)"));
+    TRY(dalloc(s->rec, n, "hipMalloc(contact records)"));
     TRY(dalloc(s->boxPart, (size_t)6 * blocks_for(n), "hipMalloc(grid box partials)"));
     TRY(dalloc(s->nbr, (size_t)MAX_LIST * n, "hipMalloc(contact lists)"));
     TRY(dalloc(s->ncount, n, "hipMalloc(contact counts)"));
+    TRY(dalloc(s->listOff, n + 1, "hipMalloc(contact list offsets)"));
+    TRY(dalloc(s->listBlk, blocks_for(n), "hipMalloc(contact list offsets)"));
+    TRY(dalloc(s->pairs, (size_t)MAX_LIST * n, "hipMalloc(compacted contacts)"));
+    TRY(dalloc(s->pairF, (size_t)6 * MAX_LIST * n, "hipMalloc(contact forces)"));
     TRY(dalloc(s->byorig, n, "hipMalloc(row index)"));
     // history
     TRY(dalloc(s->H.ppOcc, n, "hipMalloc(particle history)"));
@@ -1657,6 +1937,8 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
 
 namespace {
 
+int flush_finish(const dem_solver* cs);
+
 TriSet triset(const dem_solver* s)
 {
     TriSet T{};
@@ -1702,7 +1984,8 @@ int build_triangle_grid(dem_solver* s)
                        s->tCounts, s->tOffsets, s->tKeyA, s->tTriA);
     TRY(launch_check("fillTriangleCells"));
     const int nblk = (int)((total + RS_TILE - 1) / RS_TILE);
-    TRY(radix_sort(st, s->tKeyA, s->tTriA, s->tKeyB, s->tTriB, (long)total, s->tsCounts, nblk, s->d64));
+    TRY(radix_sort(st, s->tKeyA, s->tTriA, s->tKeyB, s->tTriB, (long)total, s->tsCounts, nblk, s->d64,
+                   s->rsDigits));
     HIP_TRY(hipStreamSynchronize(st), "triangle grid");  // host vector `off` goes out of scope
     return DEM_OK;
 }
@@ -1712,6 +1995,7 @@ int build_triangle_grid(dem_solver* s)
 int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int collect)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    TRY(flush_finish(s));
     hipStream_t st = s->stream;
     const long n = s->n;
     const int nb = s->nb;
@@ -1738,7 +2022,8 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                        s->cellCap);
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->sc, s->keyA, s->valA);
     TRY(launch_check("grid keys"));
-    TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits));
+    TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits,
+                   s->rsDigits));
     hipLaunchKernelGGL(k_fixup, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->keyA, s->valA, n,
                        S.i + (long)I_ORIG * n);
     const bool reorder = s->reorderInterval > 0 && (s->stepIndex % s->reorderInterval) == 0;
@@ -1752,17 +2037,33 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     const int cellBlocks = (int)std::min<unsigned long long>((s->cellCap + BLOCK - 1) / BLOCK, 65535ull);
     hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, n, s->sc, s->cellStart,
                        s->cellCap);
-    hipLaunchKernelGGL(k_posr, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->offs, s->posr);
+    hipLaunchKernelGGL(k_posr, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->offs, s->posr,
+                       s->cidS);
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->offs, s->rec);
     TRY(launch_check("grid build"));
     if (s->nt > 0) TRY(build_triangle_grid(s));
     HIP_TRY(hipEventRecord(s->ev[1], st), "event");
-    hipLaunchKernelGGL(k_scan, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->keyA,
-                       s->cellStart, s->sc, s->nbr, s->ncount);
+    const int step = s->stepIndex++;
+    const int nblk = blocks_for(n);
+    hipLaunchKernelGGL(k_scan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->cidS, s->keyA,
+                       s->cellStart, s->sc, s->H, step, s->nbr, s->ncount);
+    hipLaunchKernelGGL(k_off_blocks, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk);
+    hipLaunchKernelGGL(k_off_scan, dim3(1), dim3(BLOCK), 0, st, s->listBlk, nblk, s->sc);
+    hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk, s->nbr,
+                       s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
     HIP_TRY(hipEventRecord(s->ev[2], st), "event");
-    const int step = s->stepIndex++;
+    {
+        Material m = s->pp;
+        if (step == 0) m.dt = 0.0;
+        hipLaunchKernelGGL(k_pairs, dim3(PAIR_BLOCKS), dim3(BLOCK), 0, st, S, s->valA, s->rec, s->pairs,
+                           s->H, m, s->offs, s->sc, s->pairF);
+        TRY(launch_check("particle contact law"));
+    }
+    HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
-    A.S = S; A.sorted = s->valA; A.nbr = s->nbr; A.ncount = s->ncount; A.H = s->H;
+    A.S = S; A.rec = s->rec; A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
+    A.pairF = s->pairF; A.H = s->H;
     A.W = Walls{s->nw, s->wallBuf, s->wallBuf + 3 * s->nw};
     A.T = triset(s); A.O = s->O; A.sc = s->sc;
     A.pp = s->pp; A.pt = s->pt; A.pw = s->pw;
@@ -1784,7 +2085,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     }
     hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     TRY(launch_check("contact law"));
-    HIP_TRY(hipEventRecord(s->ev[3], st), "event");
+    HIP_TRY(hipEventRecord(s->ev[4], st), "event");
     const bool interval = A.pt.dt > 0.0;
     if (nb > 0) {
         hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, S, s->valA, s->H,
@@ -1811,7 +2112,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         ++s->intervalSteps;
         if (!interval) --s->intervalSteps;
     }
-    HIP_TRY(hipEventRecord(s->ev[4], st), "event");
+    HIP_TRY(hipEventRecord(s->ev[5], st), "event");
     s->evReady = true;
     HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
     const bool needLoads = loads6 && nb > 0;
@@ -1825,6 +2126,8 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             s->hScalars->grid_overflow = 0;
             return set_error(DEM_E_CAPACITY, "component grid coordinate exceeds signed 21-bit hash range");
         }
+        if (h.list_overflow != 0)
+            return set_error(DEM_E_CAPACITY, "contact list overflow (more partners than history slots)");
         dem_step_stats stt{};
         stt.particle_contacts = h.particle_contacts;
         stt.body_contacts = h.body_contacts;
@@ -1878,9 +2181,24 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     return DEM_OK;
 }
 
+namespace {
+// finishParticleStep is deferred so it can run fused with the next
+// advanceParticles; any other call that reads or writes particle state
+// completes it first.
+int flush_finish(const dem_solver* cs)
+{
+    dem_solver* s = const_cast<dem_solver*>(cs);
+    if (!s->pendingFinish) return DEM_OK;
+    s->pendingFinish = false;
+    hipLaunchKernelGGL(k_finish, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), s->pendingDt);
+    return launch_check("advanceHalfVelocity");
+}
+}  // namespace
+
 int dem_init_half_step(dem_solver* s, double h)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    TRY(flush_finish(s));
     hipLaunchKernelGGL(k_init_half, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), h);
     return launch_check("initializeLeapfrog");
 }
@@ -1888,7 +2206,13 @@ int dem_init_half_step(dem_solver* s, double h)
 int dem_advance_particles(dem_solver* s, double dt, double* vmax)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
+    if (s->pendingFinish) {
+        s->pendingFinish = false;
+        hipLaunchKernelGGL(k_finish_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(),
+                           s->pendingDt, dt);
+    } else {
+        hipLaunchKernelGGL(k_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
+    }
     if (vmax) *vmax = 0.0;
     return launch_check("advancePosition");
 }
@@ -1896,13 +2220,16 @@ int dem_advance_particles(dem_solver* s, double dt, double* vmax)
 int dem_finish_particle_step(dem_solver* s, double dt, double)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
-    hipLaunchKernelGGL(k_finish, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
-    return launch_check("advanceHalfVelocity");
+    TRY(flush_finish(s));
+    s->pendingFinish = true;
+    s->pendingDt = dt;
+    return DEM_OK;
 }
 
 int dem_download_particle_state(const dem_solver* s, double* pos, double* vel, double* ang, double* quat)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    TRY(flush_finish(s));
     const long n = s->n;
     hipLaunchKernelGGL(k_gather_state, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), s->dl);
     TRY(launch_check("gather state"));
@@ -2076,6 +2403,7 @@ int dem_debug_triangle_grid(const dem_solver* s, uint64_t* keys, int32_t* tris)
 int dem_synchronize(const dem_solver* s)
 {
     if (!s || !s->stream) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    TRY(flush_finish(s));
     HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     return DEM_OK;
 }
@@ -2086,7 +2414,7 @@ int dem_debug_last_timings(const dem_solver* s, double* ms, int capacity, int* c
     *count = 0;
     if (!s->evReady) return DEM_OK;
     HIP_TRY(hipStreamSynchronize(s->stream), "timings");
-    for (int i = 0; i + 1 < 5 && i < capacity; ++i) {
+    for (int i = 0; i + 1 < NEV && i < capacity; ++i) {
         float v = 0.0f;
         HIP_TRY(hipEventElapsedTime(&v, s->ev[i], s->ev[i + 1]), "hipEventElapsedTime");
         ms[i] = v;

@@ -214,8 +214,10 @@ int dem_debug_triangle_grid(const dem_solver* solver, uint64_t* keys,
  * reference counterpart: the reference's calls are synchronous, these are
  * ordered on one stream and complete at the next download). */
 int dem_synchronize(const dem_solver* solver);
-/* Device kernel timing of the last dem_compute_forces, in milliseconds
- * (grid build, contact scan, contact law); for measurement only. */
+/* Device kernel timing of the last dem_compute_forces, in milliseconds:
+ * [0] grid build, [1] contact scan + list compaction, [2] per-contact
+ * particle law, [3] per-particle sums (triangles, walls, field,
+ * acceleration), [4] body reductions.  For measurement only. */
 int dem_debug_last_timings(const dem_solver* solver, double* ms_out,
                            int capacity, int* count);
 
