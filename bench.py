@@ -37,6 +37,55 @@ def law_bytes_per_particle(cbar: float, walls: int) -> float:
     return 220.0 + 4.0 + 48.0 + 216.0 + 152.0 * cbar
 
 
+def run_steps(s, dt, steps):
+    for _ in range(steps):
+        s.advance(dt)
+        s.compute_forces(collect=False)
+        s.finish(dt)
+
+
+def measure_sweep(dm, scene, steps, warmup, seq_steps, count, seed):
+    """Parameter sweep of `count` independent box scenes: one batched solver
+    against the same scenes run one after another in single-scene solvers
+    (same GPU, same step loop).  Returns ms per step of each and the ratio."""
+    import time as _t
+
+    scenes = scene.sweep_scenes(count=count, seed=seed)
+    batch = scene.SceneBatch(scenes)
+    dt = batch.combined.dt
+    s = dm.CudaDemSolver()
+    s.initialize_batch(batch)
+    s.compute_forces(collect=True)
+    s.init_half_step(0.5 * dt)
+    run_steps(s, dt, warmup)
+    s.synchronize()
+    t0 = _t.perf_counter()
+    run_steps(s, dt, steps)
+    s.synchronize()
+    batched = (_t.perf_counter() - t0) / steps
+    contacts = s.stats()["particle_contacts"]
+    s.close()
+    seq = 0.0
+    for sc in scenes:
+        q = dm.CudaDemSolver()
+        q.initialize(sc)
+        q.compute_forces(collect=True)
+        q.init_half_step(0.5 * dt)
+        run_steps(q, dt, 3)
+        q.synchronize()
+        t0 = _t.perf_counter()
+        run_steps(q, dt, seq_steps)
+        q.synchronize()
+        seq += (_t.perf_counter() - t0) / seq_steps
+        q.close()
+    return {"scenes": count, "particles": batch.n, "batched_ms_per_step": 1e3 * batched,
+            "sequential_ms_per_step": 1e3 * seq, "speedup": seq / batched,
+            "batched_particle_steps_per_s": batch.n / batched, "contacts": contacts,
+            "steps": steps, "sequential_steps_per_scene": seq_steps,
+            "workload": f"{count} boxes, N ~ U[500, 20000] spheres r ~ U[1, 1.5] mm, seeded RSA, "
+                        "per-scene mu/restitution/E, dt 1e-6, gravity"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +98,10 @@ def main():
                     help="grid cell / max radius (the workload specifies 2; the reference uses 2.5)")
     ap.add_argument("--packing", choices=("rsa", "dense"), default="rsa",
                     help="rsa: the specified seeded insertion (loose); dense: near-touching bed")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter-sweep figure")
+    ap.add_argument("--sweep-scenes", type=int, default=256)
+    ap.add_argument("--sweep-steps", type=int, default=100)
+    ap.add_argument("--sweep-seq-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -102,6 +155,14 @@ def main():
           file=sys.stderr, flush=True)
     s.close()
 
+    sweep = None
+    if not args.no_sweep:
+        t0 = time.time()
+        sweep = measure_sweep(dm, scene, args.sweep_steps, 10, args.sweep_seq_steps, args.sweep_scenes, 4)
+        print(f"# sweep: {sweep['scenes']} scenes, {sweep['particles']} spheres: batched "
+              f"{sweep['batched_ms_per_step']:.3f} ms/step, sequential {sweep['sequential_ms_per_step']:.3f} "
+              f"ms/step, speedup {sweep['speedup']:.1f}x ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         from dem_mbd_cuda_amd import _abi, host
@@ -144,6 +205,7 @@ def main():
                      "kernel": "k_law", "kernel_ms": law_ms,
                      "bytes_per_particle": law_bytes_per_particle(cbar, len(sc.wall_origin))},
         "cpu_baseline": cpu,
+        "batched_sweep": sweep,
     }
     print(json.dumps(out))
 

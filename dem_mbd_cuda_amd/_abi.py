@@ -53,6 +53,11 @@ class Control(C.Structure):
                 ("keep_body_detail", C.c_int32), ("mesh_scale", C.c_double)]
 
 
+class BatchView(C.Structure):
+    _fields_ = [("scene_count", C.c_int32), ("particle_start", C.POINTER(C.c_int64)),
+                ("wall_start", _ip), ("pp", C.POINTER(Mech)), ("pw", C.POINTER(Mech))]
+
+
 class StepStats(C.Structure):
     _fields_ = [("particle_contacts", C.c_int32), ("body_contacts", C.c_int32),
                 ("body_max_depth", C.c_double), ("body_contact_fz", C.c_double),
@@ -83,6 +88,10 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+def _mech(m):
+    return Mech(*[float(m[k]) for k in ("epsS", "mu", "epsN", "cohesion", "beta", "muT", "muR", "kN")])
+
+
 class Library:
     """One loaded shared library exposing the dem_hip.h entry points."""
 
@@ -108,6 +117,9 @@ class Library:
         bind("initialize", [vp, C.POINTER(ParticlesView), C.POINTER(ComponentsView),
                             C.POINTER(WallsView), C.POINTER(BodiesView), C.POINTER(Mech),
                             C.POINTER(Mech), C.POINTER(Mech), C.POINTER(Control)])
+        if hasattr(L, p + "initialize_batch"):
+            bind("initialize_batch", [vp, C.POINTER(ParticlesView), C.POINTER(ComponentsView),
+                                      C.POINTER(WallsView), C.POINTER(BatchView), C.POINTER(Control)])
         bind("compute_forces", [vp, _dp, _dp, C.c_int])
         bind("init_half_step", [vp, C.c_double])
         bind("advance_particles", [vp, C.c_double, _dp])
@@ -175,10 +187,9 @@ class Solver:
         except Exception:
             pass
 
-    def initialize(self, scene):
-        s = scene
+    def _views(self, s):
         n = int(len(s.pid))
-        arrays = dict(
+        a = dict(
             pid=_i32(s.pid), status=_i32(s.status), mass=_f64(s.mass), radius=_f64(s.radius),
             inertia=_f64(s.inertia, (n, 3)), pos=_f64(s.pos, (n, 3)), vel=_f64(s.vel, (n, 3)),
             angvel=_f64(s.angvel, (n, 3)), quat=_f64(s.quat, (n, 4)),
@@ -187,8 +198,7 @@ class Solver:
             wo=_f64(s.wall_origin, (-1, 3)), wn=_f64(s.wall_normal, (-1, 3)),
             bm=_f64(s.body_mass), bp=_f64(s.body_pose, (-1, 13)), btc=_i32(s.body_tri_count),
             tl=_f64(s.tri_local, (-1, 15)))
-        a = arrays
-        self._keep = [arrays]
+        self._keep.append(a)
         pv = ParticlesView(n, _ptr(a["pid"], C.c_int32), _ptr(a["status"], C.c_int32),
                            _ptr(a["mass"], C.c_double), _ptr(a["radius"], C.c_double),
                            _ptr(a["inertia"], C.c_double), _ptr(a["pos"], C.c_double),
@@ -199,18 +209,42 @@ class Solver:
         wv = WallsView(len(a["wo"]), _ptr(a["wo"], C.c_double), _ptr(a["wn"], C.c_double))
         bv = BodiesView(len(a["bm"]), _ptr(a["bm"], C.c_double), _ptr(a["bp"], C.c_double),
                         _ptr(a["btc"], C.c_int32), _ptr(a["tl"], C.c_double))
-        mechs = [Mech(*[float(m[k]) for k in ("epsS", "mu", "epsN", "cohesion", "beta", "muT",
-                                               "muR", "kN")]) for m in (s.pp, s.pt, s.pw)]
         ctl = Control(float(s.dt), (C.c_double * 3)(*[float(g) for g in s.gravity]),
                       int(s.gravity_enabled), int(s.universal_gravity),
                       int(s.keep_body_particle), int(s.keep_body_detail), float(s.mesh_scale))
-        self.lib.call("initialize", self.h, C.byref(pv), C.byref(cv), C.byref(wv), C.byref(bv),
-                      C.byref(mechs[0]), C.byref(mechs[1]), C.byref(mechs[2]), C.byref(ctl))
+        return n, a, pv, cv, wv, bv, ctl
+
+    def _after_init(self, s, n, a):
         self.n = n
         self.nc = len(a["cid"])
         self.nb = len(a["bm"])
         self.keep_body_particle = bool(s.keep_body_particle)
         self.keep_body_detail = bool(s.keep_body_detail)
+
+    def initialize(self, scene):
+        s = scene
+        self._keep = []
+        n, a, pv, cv, wv, bv, ctl = self._views(s)
+        mechs = [_mech(m) for m in (s.pp, s.pt, s.pw)]
+        self.lib.call("initialize", self.h, C.byref(pv), C.byref(cv), C.byref(wv), C.byref(bv),
+                      C.byref(mechs[0]), C.byref(mechs[1]), C.byref(mechs[2]), C.byref(ctl))
+        self._after_init(s, n, a)
+
+    def initialize_batch(self, batch):
+        """Initialises the solver with a SceneBatch (scenes back to back)."""
+        self._keep = []
+        s = batch.combined
+        n, a, pv, cv, wv, bv, ctl = self._views(s)
+        k = len(batch.scenes)
+        rows = np.ascontiguousarray(batch.particle_start, dtype=np.int64)
+        walls = _i32(batch.wall_start)
+        pp = (Mech * k)(*[_mech(sc.pp) for sc in batch.scenes])
+        pw = (Mech * k)(*[_mech(sc.pw) for sc in batch.scenes])
+        self._keep.append((rows, walls, pp, pw))
+        view = BatchView(k, _ptr(rows, C.c_int64), _ptr(walls, C.c_int32), pp, pw)
+        self.lib.call("initialize_batch", self.h, C.byref(pv), C.byref(cv), C.byref(wv), C.byref(view),
+                      C.byref(ctl))
+        self._after_init(s, n, a)
 
     def compute_forces(self, body_pose=None, loads=False, collect=False):
         pose = None if body_pose is None else _f64(body_pose, (self.nb, 13))

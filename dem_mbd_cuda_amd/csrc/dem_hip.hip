@@ -85,7 +85,7 @@ enum DField {
     NDF
 };
 // Int fields.
-enum IField { I_PID, I_STATUS, I_CID, I_ORIG, I_CX, I_CY, I_CZ, I_VALID, NIF };
+enum IField { I_PID, I_STATUS, I_CID, I_ORIG, I_CX, I_CY, I_CZ, I_VALID, I_SCENE, NIF };
 
 struct Soa {
     double* d;  // [NDF][n]
@@ -95,11 +95,28 @@ struct Soa {
     __device__ __forceinline__ int& k(int field, long p) const { return i[field * n + p]; }
 };
 
-// Per-step device scalars.
-struct Scalars {
+// A solver holds one scene or a batch of independent scenes.  Scene s owns
+// the contiguous rows [row0, row1) (contiguous before and after any
+// reordering, since the cell index below is scene-major), its walls
+// [wall0, wall0 + nwall) and its own cell size.
+struct SceneParams {
+    double inv;  // 1 / cell size
+    long row0, row1;
+    int wall0, nwall;
+};
+
+// Per-step grid of one scene: cell bounding box and the scene's first index
+// in the global linear cell numbering.
+struct SceneGrid {
     int lo[3], hi[3];
     unsigned long long dims[3];
     unsigned long long ncells;
+    unsigned long long base;
+};
+
+// Per-step device scalars.
+struct Scalars {
+    unsigned long long ncells;  // all scenes
     int nbits;
     int dense;
     int grid_overflow;
@@ -231,10 +248,10 @@ __device__ __forceinline__ Body load_body(const Soa& S, long p, bool offs)
 }
 
 // ------------------------------------------------------------ step kernels
-__global__ void k_step_init(Scalars* sc)
+__global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc, int nscenes)
 {
+    for (int i = threadIdx.x; i < 6 * nscenes; i += BLOCK) sceneAcc[i] = (i % 6) < 3 ? INT_MAX : INT_MIN;
     if (threadIdx.x == 0) {
-        for (int a = 0; a < 3; ++a) { sc->lo[a] = INT_MAX; sc->hi[a] = INT_MIN; }
         sc->grid_overflow = 0;
         sc->list_overflow = 0;
         sc->pp_high = 0;
@@ -246,14 +263,20 @@ __global__ void k_step_init(Scalars* sc)
     }
 }
 
-// component centre + cell + range check + bbox (grid:51-71).  The bounding
-// box is reduced per block (wave shuffles, then LDS) into part[block][6];
-// k_grid_setup folds the block partials.
-__global__ void __launch_bounds__(BLOCK) k_cells(Soa S, double inv, bool offs, Scalars* sc, int* part)
+// component centre + cell + range check + bbox (grid:51-71), cell size of
+// the particle's scene.  The cell bounding box of each scene is reduced per
+// block (wave shuffles, then LDS) into part[block][6] when the block lies in
+// one scene; a block that straddles scenes adds its lanes to the scenes'
+// accumulators with atomics.  k_grid_setup folds both.
+__global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, bool offs, Scalars* sc,
+                                                 int* part, int* partScene, int* sceneAcc)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    int scene = -1;
     if (p < S.n) {
+        scene = S.k(I_SCENE, p);
+        const double inv = sp[scene].inv;
         double cx = S.f(F_X, p), cy = S.f(F_Y, p), cz = S.f(F_Z, p);
         if (offs) {
             const V3 o = rot(S.f(F_QW, p), S.f(F_QX, p), S.f(F_QY, p), S.f(F_QZ, p),
@@ -276,68 +299,142 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, double inv, bool offs, S
         }
         S.k(I_VALID, p) = ok ? 1 : 0;
     }
-    __shared__ int red[BLOCK / 64][6];
+    __shared__ int red[BLOCK / 64][8];
     const int wave = threadIdx.x >> 6;
+    {
+        const int smin = wave_min(scene < 0 ? INT_MAX : scene), smax = wave_max(scene);
+        if ((threadIdx.x & 63) == 0) { red[wave][6] = smin; red[wave][7] = smax; }
+    }
     for (int a = 0; a < 3; ++a) {
         const int l = wave_min(lo[a]), h = wave_max(hi[a]);
         if ((threadIdx.x & 63) == 0) { red[wave][a] = l; red[wave][3 + a] = h; }
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
-        int v = red[0][threadIdx.x];
-        for (int w = 1; w < BLOCK / 64; ++w)
-            v = threadIdx.x < 3 ? min(v, red[w][threadIdx.x]) : max(v, red[w][threadIdx.x]);
-        part[(long)blockIdx.x * 6 + threadIdx.x] = v;
-    }
-}
-
-// Folds the per-block boxes and sizes the grid (one block).
-__global__ void __launch_bounds__(BLOCK) k_grid_setup(Scalars* sc, const int* part, int nblocks,
-                                                      unsigned long long cap)
-{
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-    for (int b = threadIdx.x; b < nblocks; b += BLOCK)
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = min(lo[a], part[(long)b * 6 + a]);
-            hi[a] = max(hi[a], part[(long)b * 6 + 3 + a]);
+    int smin = red[0][6], smax = red[0][7];
+    for (int w = 1; w < BLOCK / 64; ++w) { smin = min(smin, red[w][6]); smax = max(smax, red[w][7]); }
+    if (smin == smax) {
+        if (threadIdx.x < 6) {
+            int v = red[0][threadIdx.x];
+            for (int w = 1; w < BLOCK / 64; ++w)
+                v = threadIdx.x < 3 ? min(v, red[w][threadIdx.x]) : max(v, red[w][threadIdx.x]);
+            part[(long)blockIdx.x * 6 + threadIdx.x] = v;
         }
+        if (threadIdx.x == 0) partScene[blockIdx.x] = smin;
+    } else {
+        if (scene >= 0 && lo[0] <= hi[0])
+            for (int a = 0; a < 3; ++a) {
+                atomicMin(&sceneAcc[scene * 6 + a], lo[a]);
+                atomicMax(&sceneAcc[scene * 6 + 3 + a], hi[a]);
+            }
+        if (threadIdx.x == 0) partScene[blockIdx.x] = -1;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan_t(T v, T* total)
+{
+    __shared__ T wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const T y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    T before = 0, all = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// Folds each scene's box (one block per scene) and sizes its grid; the last
+// block to finish numbers the scenes' cells consecutively and sizes the
+// global cell table.
+__global__ void __launch_bounds__(BLOCK) k_grid_setup(Scalars* sc, const SceneParams* sp, const int* part,
+                                                      const int* partScene, const int* sceneAcc,
+                                                      SceneGrid* grid, int nscenes, unsigned long long cap,
+                                                      unsigned* done)
+{
+    const int s = blockIdx.x;
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) { lo[a] = sceneAcc[s * 6 + a]; hi[a] = sceneAcc[s * 6 + 3 + a]; }
+    const long b0 = sp[s].row0 / BLOCK, b1 = sp[s].row1 > sp[s].row0 ? (sp[s].row1 - 1) / BLOCK : b0 - 1;
+    for (long b = b0 + threadIdx.x; b <= b1; b += BLOCK)
+        if (partScene[b] == s)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], part[b * 6 + a]);
+                hi[a] = max(hi[a], part[b * 6 + 3 + a]);
+            }
     __shared__ int red[BLOCK / 64][6];
+    __shared__ bool last;
     const int wave = threadIdx.x >> 6;
     for (int a = 0; a < 3; ++a) {
         const int l = wave_min(lo[a]), h = wave_max(hi[a]);
         if ((threadIdx.x & 63) == 0) { red[wave][a] = l; red[wave][3 + a] = h; }
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    unsigned long long nc = 1ull;
-    for (int a = 0; a < 3; ++a) {
-        int l = red[0][a], h = red[0][3 + a];
-        for (int w = 1; w < BLOCK / 64; ++w) { l = min(l, red[w][a]); h = max(h, red[w][3 + a]); }
-        if (l > h) { l = 0; h = 0; }
-        sc->lo[a] = l;
-        sc->hi[a] = h;
-        sc->dims[a] = (unsigned long long)((long long)h - l + 1);
-        nc *= sc->dims[a];
+    if (threadIdx.x == 0) {
+        unsigned long long nc = 1ull;
+        SceneGrid& g = grid[s];
+        for (int a = 0; a < 3; ++a) {
+            int l = red[0][a], h = red[0][3 + a];
+            for (int w = 1; w < BLOCK / 64; ++w) { l = min(l, red[w][a]); h = max(h, red[w][3 + a]); }
+            if (l > h) { l = 0; h = 0; }
+            g.lo[a] = l;
+            g.hi[a] = h;
+            g.dims[a] = (unsigned long long)((long long)h - l + 1);
+            nc *= g.dims[a];
+        }
+        g.ncells = nc;
+        __threadfence();
+        last = atomicAdd(done, 1u) == (unsigned)nscenes - 1u;
     }
-    sc->ncells = nc;
-    const unsigned long long top = nc - 1ull;
-    sc->nbits = top == 0ull ? 0 : 64 - __clzll((long long)top);
-    sc->dense = (nc + 1ull <= cap) ? 1 : 0;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    unsigned long long carry = 0ull;
+    bool wrapped = false;
+    for (int b = 0; b < nscenes; b += BLOCK) {
+        const int i = b + threadIdx.x;
+        const unsigned long long v = i < nscenes ? grid[i].ncells : 0ull;
+        unsigned long long tot;
+        const unsigned long long ex = block_exclusive_scan_t(v, &tot);
+        if (i < nscenes) grid[i].base = carry + ex;
+        if (carry + tot < carry) wrapped = true;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        if (wrapped) atomicAdd(&sc->grid_overflow, 1);
+        sc->ncells = carry;
+        const unsigned long long top = carry - 1ull;
+        sc->nbits = top == 0ull ? 0 : 64 - __clzll((long long)top);
+        sc->dense = (carry + 1ull <= cap) ? 1 : 0;
+        *done = 0u;
+    }
 }
 
-__device__ __forceinline__ unsigned long long rel_cell(const Scalars* sc, int cx, int cy, int cz)
+__device__ __forceinline__ unsigned long long rel_cell(const SceneGrid& g, int cx, int cy, int cz)
 {
-    return ((unsigned long long)(cx - sc->lo[0]) * sc->dims[1] + (unsigned long long)(cy - sc->lo[1])) *
-               sc->dims[2] +
-           (unsigned long long)(cz - sc->lo[2]);
+    return g.base + ((unsigned long long)(cx - g.lo[0]) * g.dims[1] + (unsigned long long)(cy - g.lo[1])) *
+                        g.dims[2] +
+           (unsigned long long)(cz - g.lo[2]);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_keys(Soa S, const Scalars* sc, unsigned long long* key,
+// Global linear cell index (scene-major, then the reference's x, y, z key
+// order within the scene).  A particle that failed the range check keeps its
+// scene's first cell (the step reports the error).
+__global__ void __launch_bounds__(BLOCK) k_keys(Soa S, const SceneGrid* grid, unsigned long long* key,
                                                 int* val)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (p >= S.n) return;
-    key[p] = S.k(I_VALID, p) ? rel_cell(sc, S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p)) : 0ull;
+    const SceneGrid& g = grid[S.k(I_SCENE, p)];
+    key[p] = S.k(I_VALID, p) ? rel_cell(g, S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p)) : g.base;
     val[p] = (int)p;
 }
 
@@ -547,7 +644,9 @@ struct alignas(32) Rec {
     double px, py, pz, vx, vy, vz, hx, hy, hz, wx, wy, wz, ux, uy, uz, mass, r;
     int status;
     uint32_t cid;
-    double pad0, pad1;
+    int scene;
+    int pad0;
+    double pad1;
 };
 static_assert(sizeof(Rec) == 160, "contact record layout");
 
@@ -580,7 +679,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Rec* R)
     q.wx = B.w.x; q.wy = B.w.y; q.wz = B.w.z;
     q.ux = B.wh.x; q.uy = B.wh.y; q.uz = B.wh.z;
     q.mass = B.mass; q.r = B.r; q.status = B.status; q.cid = B.cid;
-    q.pad0 = 0.0; q.pad1 = 0.0;
+    q.scene = S.k(I_SCENE, p); q.pad0 = 0; q.pad1 = 0.0;
     R[p] = q;
 }
 
@@ -644,7 +743,10 @@ struct LawArgs {
     TriSet T;
     Outputs O;
     Scalars* sc;
-    Material pp, pt, pw;
+    Material pt;                 // particle-triangle (single scene with bodies)
+    const Material* matPW;       // particle-wall, per scene
+    const SceneParams* sp;
+    bool first;                  // first force evaluation: no history advance (dt = 0)
     double g[3];
     int gravity;
     int universal;
@@ -710,8 +812,8 @@ __device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long
 // owner's contact list as entries (partner row << SLOT_BITS | slot), k-major.
 __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const double4* posr,
                                                 const uint32_t* cidS, const unsigned long long* key,
-                                                const unsigned* start, Scalars* sc, History H, int step,
-                                                int* nbr, int* ncount)
+                                                const unsigned* start, const SceneGrid* grid, Scalars* sc,
+                                                History H, int step, int* nbr, int* ncount)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
@@ -720,9 +822,10 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
         const long p = sorted[t];
         if (S.k(I_VALID, p)) {
             const double4 me = posr[t];
-            const long X = (long)sc->dims[0], Y = (long)sc->dims[1], Z = (long)sc->dims[2];
-            const long cx = S.k(I_CX, p) - sc->lo[0], cy = S.k(I_CY, p) - sc->lo[1],
-                       cz = S.k(I_CZ, p) - sc->lo[2];
+            const SceneGrid& g = grid[S.k(I_SCENE, p)];
+            const long X = (long)g.dims[0], Y = (long)g.dims[1], Z = (long)g.dims[2];
+            const long cx = S.k(I_CX, p) - g.lo[0], cy = S.k(I_CY, p) - g.lo[1],
+                       cz = S.k(I_CZ, p) - g.lo[2];
             const bool dense = sc->dense;
             const long zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < Z ? cz + 1 : Z - 1;
             for (long dx = -1; dx <= 1; ++dx) {
@@ -731,7 +834,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
                 for (long dy = -1; dy <= 1; ++dy) {
                     const long yy = cy + dy;
                     if (yy < 0 || yy >= Y) continue;
-                    const unsigned long long col = ((unsigned long long)xx * Y + yy) * Z;
+                    const unsigned long long col = g.base + ((unsigned long long)xx * Y + yy) * Z;
                     long b, e;
                     if (dense) {
                         b = start[col + zlo];
@@ -831,8 +934,8 @@ __global__ void __launch_bounds__(BLOCK) k_off_apply(const int* cnt, long n, con
 // contact (grid-stride over the compacted list), so uneven contact counts do
 // not idle lanes.  Writes the contact force on the owner and its torque term.
 __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const Rec* rec,
-                                                 const int2* pairs, History H, Material m, bool offs,
-                                                 const Scalars* sc, double* pairF)
+                                                 const int2* pairs, History H, const Material* mats,
+                                                 bool first, bool offs, const Scalars* sc, double* pairF)
 {
     const long total = sc->total_pairs;
     for (long e = (long)blockIdx.x * BLOCK + threadIdx.x; e < total; e += (long)gridDim.x * BLOCK) {
@@ -846,6 +949,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const
         const V3 delta = sub(J.pos, I.pos);
         const double d2 = dot(delta, delta);
         const long o = S.k(I_ORIG, p);
+        Material m = mats[rec[p].scene];
+        if (first) m.dt = 0.0;  // the first evaluation advances no history
         V3 cf, tq;
         pp_law(I, J, delta, d2, m, H.ppVal + (o * PP_SLOTS + slot) * 9, cf, tq);
         double* out = pairF + e * 6;
@@ -921,17 +1026,22 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
         const History& H = A.H;
         unsigned pwOcc = H.pwOcc[o] & H.pwSeen[(long)((s + 1) & 1) * n + o];
         unsigned pwCur = 0;
-        // walls
-        for (int w = 0; w < A.W.n; ++w) {
-            const V3 outward = mk(A.W.nrm[3 * w], A.W.nrm[3 * w + 1], A.W.nrm[3 * w + 2]);
-            const V3 origin = mk(A.W.o[3 * w], A.W.o[3 * w + 1], A.W.o[3 * w + 2]);
+        // walls of the particle's scene (history key: the scene's wall index + 1)
+        const int scene = S.k(I_SCENE, p);
+        const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
+        Material mw = A.matPW[scene];
+        if (A.first) mw.dt = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            const int g = w0 + w;
+            const V3 outward = mk(A.W.nrm[3 * g], A.W.nrm[3 * g + 1], A.W.nrm[3 * g + 2]);
+            const V3 origin = mk(A.W.o[3 * g], A.W.o[3 * g + 1], A.W.o[3 * g + 2]);
             const double distance = dot(sub(I.pos, origin), outward);
             const double depth = I.r - distance;
             if (depth <= 0.0) continue;
             const int slot = acquire(pwOcc, pwCur, H.pwKey + o * PW_SLOTS, H.pwVal + o * PW_SLOTS * 9,
                                      PW_SLOTS, (uint32_t)(w + 1), bodyHigh, pwOver);
             if (slot < 0) continue;
-            pw_law(I, outward, depth, A.pw, H.pwVal + (o * PW_SLOTS + slot) * 9, wF, wT);
+            pw_law(I, outward, depth, mw, H.pwVal + (o * PW_SLOTS + slot) * 9, wF, wT);
         }
         // particle pairs, summed in candidate order from the per-contact results
         const int cnt = A.ncount[t];
@@ -1523,6 +1633,15 @@ struct dem_solver {
     int rsBlocks = 0;
     unsigned* cellStart = nullptr;
     int* boxPart = nullptr;
+    int* partScene = nullptr;   // [blocks] scene of a one-scene block, -1 otherwise
+    int* sceneAcc = nullptr;    // [scenes][6] box of blocks that straddle scenes
+    unsigned* gridDone = nullptr;
+    int nscenes = 1;
+    SceneParams* dSp = nullptr;
+    SceneGrid* dGrid = nullptr;
+    Material* dMatPP = nullptr;
+    Material* dMatPW = nullptr;
+    std::vector<SceneParams> hsp;
     unsigned long long cellCap = 0;
     double4* posr = nullptr;
     uint32_t* cidS = nullptr;
@@ -1587,7 +1706,8 @@ struct dem_solver {
     {
         if (stream) hipStreamSynchronize(stream);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
-                        rec, rsCounts, rsDigits, cellStart, boxPart, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
+                        rec, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
+                        dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart, O.fzPart,
                         O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, tLocal, tWorld, tAabb,
@@ -1703,22 +1823,45 @@ int dem_create(dem_solver** out)
 
 void dem_destroy(dem_solver* s) { delete s; }
 
-int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_components_view* C,
-                   const dem_walls_view* W, const dem_bodies_view* B, const dem_mech* mpp,
-                   const dem_mech* mpt, const dem_mech* mpw, const dem_control* ctl)
+namespace {
+
+// Scene layout of a solver: rows and walls of each scene, per-scene
+// particle-particle and particle-wall laws.
+struct SceneSpec {
+    int count;
+    const int64_t* row;   // [count + 1]
+    const int32_t* wall;  // [count + 1]
+    const dem_mech* pp;   // [count]
+    const dem_mech* pw;   // [count]
+};
+
+int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_components_view* C,
+                    const dem_walls_view* W, const dem_bodies_view* B, const dem_mech* mpt,
+                    const dem_control* ctl, const SceneSpec& spec)
 {
-    if (!s || !P || !C || !valid_mech(mpp) || !valid_mech(mpt) || !valid_mech(mpw) || !ctl)
-        return set_error(DEM_E_INVALID, "null argument");
     if (s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver already initialized");
-    if (!dem_backend_available()) return set_error(DEM_E_DEVICE, "no CUDA device is available");
     if (P->count <= 0) return set_error(DEM_E_INVALID, "CUDA DEM solver requires particles");
     if (P->count > MAX_PARTICLES)
         return set_error(DEM_E_CAPACITY, "particle count exceeds the contact-list row range (2^25 - 1)");
     const long n = P->count;
-    {
-        std::vector<int> ids(P->id, P->id + n);
+    const int nsc = spec.count;
+    const int nwall = W ? W->count : 0;
+    if (nsc < 1 || spec.row[0] != 0 || spec.row[nsc] != n || spec.wall[0] != 0 || spec.wall[nsc] != nwall)
+        return set_error(DEM_E_INVALID, "scene row/wall ranges do not cover the particle and wall views");
+    for (int k = 0; k < nsc; ++k) {
+        if (spec.row[k + 1] <= spec.row[k] || spec.wall[k + 1] < spec.wall[k])
+            return set_error(DEM_E_INVALID, "scene %d has an empty or reversed row range", k);
+        if (!valid_mech(&spec.pp[k]) || !valid_mech(&spec.pw[k]))
+            return set_error(DEM_E_INVALID, "null argument");
+    }
+    if (nsc > 1 && B && B->count > 0)
+        return set_error(DEM_E_UNSUPPORTED, "batched scenes cannot carry bodies");
+    if (nsc > 1 && ctl->universal_gravity)
+        return set_error(DEM_E_UNSUPPORTED, "batched scenes cannot use universal gravitation");
+    for (int k = 0; k < nsc; ++k) {
+        std::vector<int> ids(P->id + spec.row[k], P->id + spec.row[k + 1]);
         std::sort(ids.begin(), ids.end());
-        for (long i = 1; i < n; ++i)
+        for (size_t i = 1; i < ids.size(); ++i)
             if (ids[i] == ids[i - 1])
                 return set_error(DEM_E_INVALID,
                                  "duplicate particle ID %d; CUDA contact history requires stable unique particle IDs",
@@ -1733,7 +1876,10 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
         if (C->owner[a] != C->owner[b]) return C->owner[a] < C->owner[b];
         return C->id[a] < C->id[b];
     });
-    std::vector<int> perOwner(n, 0);
+    std::vector<int> perOwner(n, 0), sceneOf(n, 0);
+    for (int k = 0; k < nsc; ++k)
+        for (int64_t r = spec.row[k]; r < spec.row[k + 1]; ++r) sceneOf[r] = k;
+    std::vector<double> maxrScene(nsc, 0.0);
     double maxr = 0.0;
     for (int k : order) {
         const int ow = C->owner[k];
@@ -1741,6 +1887,7 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
             return set_error(DEM_E_INVALID, "invalid native clump component topology");
         ++perOwner[ow];
         maxr = std::max(maxr, C->radius[k]);
+        maxrScene[sceneOf[ow]] = std::max(maxrScene[sceneOf[ow]], C->radius[k]);
     }
     for (long i = 0; i < n; ++i)
         if (perOwner[i] < 1 || perOwner[i] > 8)
@@ -1750,6 +1897,7 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
                          "multi-sphere clumps (more than one component per owner) are not supported by this build");
     if (n > (long)INT_MAX / PP_SLOTS)
         return set_error(DEM_E_CAPACITY, "particle count exceeds CUDA contact-history index capacity");
+    if (!dem_backend_available()) return set_error(DEM_E_DEVICE, "no CUDA device is available");
 
     HIP_TRY(hipSetDevice(0), "hipSetDevice");
     HIP_TRY(hipFree(nullptr), "initialize HIP context");
@@ -1759,12 +1907,26 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
     for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
 
     s->n = n;
+    s->nscenes = nsc;
+    const double scale = ctl->mesh_scale > 0.0 ? ctl->mesh_scale : 2.5;
     s->maxRadius = maxr;
-    s->meshSize = (ctl->mesh_scale > 0.0 ? ctl->mesh_scale : 2.5) * maxr;
+    s->meshSize = scale * maxr;
     s->inv = 1.0 / s->meshSize;
-    s->pp = make_material(*mpp, ctl->dt);
+    s->pp = make_material(spec.pp[0], ctl->dt);
     s->pt = make_material(*mpt, ctl->dt);
-    s->pw = make_material(*mpw, ctl->dt);
+    s->pw = make_material(spec.pw[0], ctl->dt);
+    s->hsp.resize(nsc);
+    std::vector<Material> hpp(nsc), hpw(nsc);
+    for (int k = 0; k < nsc; ++k) {
+        SceneParams& q = s->hsp[k];
+        q.inv = 1.0 / (scale * maxrScene[k]);
+        q.row0 = spec.row[k];
+        q.row1 = spec.row[k + 1];
+        q.wall0 = spec.wall[k];
+        q.nwall = spec.wall[k + 1] - spec.wall[k];
+        hpp[k] = make_material(spec.pp[k], ctl->dt);
+        hpw[k] = make_material(spec.pw[k], ctl->dt);
+    }
     s->g[0] = ctl->gravity[0]; s->g[1] = ctl->gravity[1]; s->g[2] = ctl->gravity[2];
     s->gravity = ctl->gravity_enabled ? 1 : 0;
     s->universal = ctl->universal_gravity ? 1 : 0;
@@ -1797,6 +1959,7 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
         Iv(I_PID, p) = P->id[p];
         Iv(I_STATUS, p) = P->status[p];
         Iv(I_ORIG, p) = (int)p;
+        Iv(I_SCENE, p) = sceneOf[p];
     }
     for (int b = 0; b < 2; ++b) {
         TRY(dalloc(s->dState[b], (size_t)NDF * n, "hipMalloc(particle state)"));
@@ -1822,6 +1985,17 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
     TRY(dalloc(s->cidS, n, "hipMalloc(sorted component ids)"));
     TRY(dalloc(s->rec, n, "hipMalloc(contact records)"));
     TRY(dalloc(s->boxPart, (size_t)6 * blocks_for(n), "hipMalloc(grid box partials)"));
+    TRY(dalloc(s->partScene, blocks_for(n), "hipMalloc(grid box partials)"));
+    TRY(dalloc(s->sceneAcc, (size_t)6 * nsc, "hipMalloc(scene boxes)"));
+    TRY(dalloc(s->gridDone, 1, "hipMalloc(grid counter)"));
+    HIP_TRY(hipMemset(s->gridDone, 0, sizeof(unsigned)), "clear grid counter");
+    TRY(dalloc(s->dSp, nsc, "hipMalloc(scene table)"));
+    TRY(dalloc(s->dGrid, nsc, "hipMalloc(scene grids)"));
+    TRY(dalloc(s->dMatPP, nsc, "hipMalloc(scene materials)"));
+    TRY(dalloc(s->dMatPW, nsc, "hipMalloc(scene materials)"));
+    HIP_TRY(hipMemcpy(s->dSp, s->hsp.data(), sizeof(SceneParams) * nsc, hipMemcpyHostToDevice), "upload scenes");
+    HIP_TRY(hipMemcpy(s->dMatPP, hpp.data(), sizeof(Material) * nsc, hipMemcpyHostToDevice), "upload scenes");
+    HIP_TRY(hipMemcpy(s->dMatPW, hpw.data(), sizeof(Material) * nsc, hipMemcpyHostToDevice), "upload scenes");
     TRY(dalloc(s->nbr, (size_t)MAX_LIST * n, "hipMalloc(contact lists)"));
     TRY(dalloc(s->ncount, n, "hipMalloc(contact counts)"));
     TRY(dalloc(s->listOff, n + 1, "hipMalloc(contact list offsets)"));
@@ -1935,6 +2109,30 @@ int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_compone
     return DEM_OK;
 }
 
+}  // namespace
+
+int dem_initialize(dem_solver* s, const dem_particles_view* P, const dem_components_view* C,
+                   const dem_walls_view* W, const dem_bodies_view* B, const dem_mech* mpp,
+                   const dem_mech* mpt, const dem_mech* mpw, const dem_control* ctl)
+{
+    if (!s || !P || !C || !valid_mech(mpp) || !valid_mech(mpt) || !valid_mech(mpw) || !ctl)
+        return set_error(DEM_E_INVALID, "null argument");
+    const int64_t rows[2] = {0, P->count};
+    const int32_t walls[2] = {0, W ? W->count : 0};
+    const SceneSpec spec{1, rows, walls, mpp, mpw};
+    return initialize_impl(s, P, C, W, B, mpt, ctl, spec);
+}
+
+int dem_initialize_batch(dem_solver* s, const dem_particles_view* P, const dem_components_view* C,
+                         const dem_walls_view* W, const dem_batch_view* batch, const dem_control* ctl)
+{
+    if (!s || !P || !C || !batch || !ctl || !batch->particle_start || !batch->wall_start || !batch->pp ||
+        !batch->pw || batch->scene_count < 1)
+        return set_error(DEM_E_INVALID, "null argument");
+    const SceneSpec spec{batch->scene_count, batch->particle_start, batch->wall_start, batch->pp, batch->pw};
+    return initialize_impl(s, P, C, W, nullptr, batch->pp, ctl, spec);
+}
+
 namespace {
 
 int flush_finish(const dem_solver* cs);
@@ -2014,13 +2212,13 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         }
     }
     HIP_TRY(hipEventRecord(s->ev[0], st), "event");
-    hipLaunchKernelGGL(k_step_init, dim3(1), dim3(64), 0, st, s->sc);
+    hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
     Soa S = s->soa();
-    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->inv, s->offs, s->sc,
-                       s->boxPart);
-    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(BLOCK), 0, st, s->sc, s->boxPart, blocks_for(n),
-                       s->cellCap);
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->sc, s->keyA, s->valA);
+    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dSp, s->offs, s->sc,
+                       s->boxPart, s->partScene, s->sceneAcc);
+    hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,
+                       s->partScene, s->sceneAcc, s->dGrid, s->nscenes, s->cellCap, s->gridDone);
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dGrid, s->keyA, s->valA);
     TRY(launch_check("grid keys"));
     TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits,
                    s->rsDigits));
@@ -2046,28 +2244,24 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     const int step = s->stepIndex++;
     const int nblk = blocks_for(n);
     hipLaunchKernelGGL(k_scan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->cidS, s->keyA,
-                       s->cellStart, s->sc, s->H, step, s->nbr, s->ncount);
+                       s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount);
     hipLaunchKernelGGL(k_off_blocks, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk);
     hipLaunchKernelGGL(k_off_scan, dim3(1), dim3(BLOCK), 0, st, s->listBlk, nblk, s->sc);
     hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk, s->nbr,
                        s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
     HIP_TRY(hipEventRecord(s->ev[2], st), "event");
-    {
-        Material m = s->pp;
-        if (step == 0) m.dt = 0.0;
-        hipLaunchKernelGGL(k_pairs, dim3(PAIR_BLOCKS), dim3(BLOCK), 0, st, S, s->valA, s->rec, s->pairs,
-                           s->H, m, s->offs, s->sc, s->pairF);
-        TRY(launch_check("particle contact law"));
-    }
+    hipLaunchKernelGGL(k_pairs, dim3(PAIR_BLOCKS), dim3(BLOCK), 0, st, S, s->valA, s->rec, s->pairs, s->H,
+                       s->dMatPP, step == 0, s->offs, s->sc, s->pairF);
+    TRY(launch_check("particle contact law"));
     HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
     A.S = S; A.rec = s->rec; A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
     A.pairF = s->pairF; A.H = s->H;
     A.W = Walls{s->nw, s->wallBuf, s->wallBuf + 3 * s->nw};
     A.T = triset(s); A.O = s->O; A.sc = s->sc;
-    A.pp = s->pp; A.pt = s->pt; A.pw = s->pw;
-    if (step == 0) { A.pp.dt = 0.0; A.pt.dt = 0.0; A.pw.dt = 0.0; }
+    A.pt = s->pt; A.matPW = s->dMatPW; A.sp = s->dSp; A.first = step == 0;
+    if (step == 0) A.pt.dt = 0.0;
     A.g[0] = s->g[0]; A.g[1] = s->g[1]; A.g[2] = s->g[2];
     A.gravity = s->gravity; A.universal = s->universal; A.byorig = s->byorig;
     A.step = step; A.offs = s->offs; A.nb = nb;

@@ -289,6 +289,82 @@ def hopper_scene(n=1_000_000, seed=2, width=0.2, fill=0.36, median=1.0e-3, sigma
                  mesh_scale=mesh_scale, steps=steps)
 
 
+class SceneBatch:
+    """Independent scenes stepped by one solver (include/dem_hip.h,
+    dem_initialize_batch).  The scenes share dt, gravity and the cell-size
+    factor; each keeps its own particles, walls and particle-particle /
+    particle-wall laws.  `combined` is the back-to-back Scene handed to the
+    solver; `split` cuts a downloaded per-particle array back into scenes."""
+
+    def __init__(self, scenes):
+        scenes = list(scenes)
+        if not scenes:
+            raise ValueError("a batch needs at least one scene")
+        s0 = scenes[0]
+        for sc in scenes:
+            if len(sc.body_mass):
+                raise ValueError("batched scenes cannot carry bodies")
+            if (sc.dt != s0.dt or tuple(sc.gravity) != tuple(s0.gravity)
+                    or sc.gravity_enabled != s0.gravity_enabled or sc.mesh_scale != s0.mesh_scale
+                    or sc.universal_gravity):
+                raise ValueError("batched scenes must share dt, gravity and mesh_scale "
+                                 "and cannot use universal gravitation")
+        self.scenes = scenes
+        self.particle_start = np.concatenate([[0], np.cumsum([sc.n for sc in scenes])]).astype(np.int64)
+        self.wall_start = np.concatenate([[0], np.cumsum([len(sc.wall_origin) for sc in scenes])]
+                                         ).astype(np.int32)
+        cat = lambda name: np.concatenate([np.asarray(getattr(sc, name)) for sc in scenes])
+        owner = np.concatenate([np.asarray(sc.comp_owner) + self.particle_start[k]
+                                for k, sc in enumerate(scenes)]).astype(np.int32)
+        self.combined = Scene(
+            pid=cat("pid"), status=cat("status"), mass=cat("mass"), radius=cat("radius"),
+            inertia=cat("inertia"), pos=cat("pos"), vel=cat("vel"), angvel=cat("angvel"),
+            quat=cat("quat"), comp_id=cat("comp_id"), comp_owner=owner,
+            comp_radius=cat("comp_radius"), comp_pos_body=cat("comp_pos_body"),
+            wall_origin=cat("wall_origin").reshape(-1, 3), wall_normal=cat("wall_normal").reshape(-1, 3),
+            pp=dict(s0.pp), pt=dict(s0.pt), pw=dict(s0.pw), dt=s0.dt, gravity=tuple(s0.gravity),
+            gravity_enabled=s0.gravity_enabled, mesh_scale=s0.mesh_scale, steps=s0.steps)
+
+    def __len__(self):
+        return len(self.scenes)
+
+    @property
+    def n(self):
+        return int(self.particle_start[-1])
+
+    def split(self, a):
+        return [a[self.particle_start[k]:self.particle_start[k + 1]] for k in range(len(self.scenes))]
+
+
+def sweep_scenes(count=256, seed=4, nmin=500, nmax=20000, rmin=1.0e-3, rmax=1.5e-3, density=2500.0,
+                 dt=1.0e-6, steps=2000, fill=0.2, mesh_scale=0.0, gravity=(0.0, 0.0, -9.81)):
+    """A parameter sweep: `count` boxes, N ~ U[nmin, nmax] spheres of radius
+    U[rmin, rmax], seeded RSA at volume fraction `fill` into a box twice as
+    tall as wide, 6 walls; per scene mu ~ U[0.1, 0.9], restitution
+    ~ U[0.2, 0.9], E log-uniform in [1e6, 1e8] (kN = E * mean radius),
+    muT = muR = 0.1, beta = 0.5, no cohesion."""
+    rng = np.random.RandomState(seed)
+    rbar = 0.5 * (rmin + rmax)
+    mean_r3 = (rmax ** 4 - rmin ** 4) / (4.0 * (rmax - rmin))
+    out = []
+    for k in range(count):
+        n = int(rng.randint(nmin, nmax + 1))
+        mu = float(rng.uniform(0.1, 0.9))
+        e = float(rng.uniform(0.2, 0.9))
+        E = float(10.0 ** rng.uniform(6.0, 8.0))
+        radius = rng.uniform(rmin, rmax, n)
+        vol = n * 4.0 / 3.0 * np.pi * mean_r3 / fill
+        side = (vol / 2.0) ** (1.0 / 3.0)
+        lo, hi = np.zeros(3), np.array([side, side, 2.0 * side])
+        pos = rsa_positions(radius, lo, hi, seed=seed * 1_000_003 + k, max_attempts=20000)
+        m = mech(epsS=e, mu=mu, epsN=e, cohesion=0.0, beta=0.5, muT=0.1, muR=0.1, kN=E * rbar)
+        wo, wn = box_walls(lo, hi)
+        out.append(Scene(**sphere_particles(pos, radius, density), wall_origin=wo, wall_normal=wn,
+                         pp=m, pt=dict(m), pw=dict(m), dt=dt, gravity=tuple(gravity), gravity_enabled=1,
+                         mesh_scale=mesh_scale, steps=steps))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Readers for the reference's input formats.
 # ---------------------------------------------------------------------------

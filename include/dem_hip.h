@@ -135,6 +135,34 @@ int dem_initialize(dem_solver* solver, const dem_particles_view* particles,
                    const dem_mech* pp, const dem_mech* pt, const dem_mech* pw,
                    const dem_control* control);
 
+/* A batch of independent scenes stepped together by one solver.  No
+ * reference counterpart: the reference runs one scene per process
+ * (cuda_simulation_integrator.cpp:248-285 drives one CudaDemSolver); a
+ * parameter sweep over many small scenes is launch-bound that way.  The
+ * particle, component and wall views hold the scenes back to back; scene k
+ * owns particle rows [particle_start[k], particle_start[k+1]) and walls
+ * [wall_start[k], wall_start[k+1]), has its own particle-particle and
+ * particle-wall MECH blocks and its own grid cell (mesh_scale times the
+ * scene's largest radius).  Particles interact only within their scene, and
+ * every per-scene result equals that of a solver initialised with the scene
+ * alone (bit for bit).  Particle ids must be unique within a scene.  Wall
+ * history keys are the wall's index within its scene + 1. */
+typedef struct dem_batch_view {
+    int32_t scene_count;
+    const int64_t* particle_start; /* [scene_count + 1], 0 .. particle count */
+    const int32_t* wall_start;     /* [scene_count + 1], 0 .. wall count      */
+    const dem_mech* pp;            /* [scene_count] particle-particle         */
+    const dem_mech* pw;            /* [scene_count] particle-wall             */
+} dem_batch_view;
+
+/* Initialises a solver with a batch of scenes (no bodies, no universal
+ * gravitation); afterwards every other entry point applies to the whole
+ * batch (downloads return the scenes back to back, statistics are totals). */
+int dem_initialize_batch(dem_solver* solver, const dem_particles_view* particles,
+                         const dem_components_view* components,
+                         const dem_walls_view* walls, const dem_batch_view* batch,
+                         const dem_control* control);
+
 /* CudaDemSolver::computeForces (cuda_backend.h:54-55; cuda_backend_cuda.cu:383-524).
  * body_pose13: [b][13] current body poses or NULL (no body upload, as when the
  * host passes a null BODYSET).  body_loads6: [b][6] out, contact load on each

@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -69,3 +70,42 @@ def test_reference_host_links_against_the_adapter():
         assert res.returncode == 0 and "CUDA device" in res.stdout, res.stdout
     else:
         assert res.returncode == 1 and "No usable NVIDIA CUDA device" in res.stdout, res.stdout
+
+
+def _small_batch():
+    from dem_mbd_cuda_amd import scene
+
+    scenes = scene.sweep_scenes(count=3, seed=3, nmin=50, nmax=120)
+    return scene, scenes
+
+
+def test_batch_argument_errors_need_no_device():
+    """dem_initialize_batch validates the scene layout before touching a
+    device: scene ranges must cover the views, scenes must be non-empty, and
+    batches reject universal gravitation."""
+    from dem_mbd_cuda_amd import _abi
+
+    scene, scenes = _small_batch()
+    lib = _abi.Library(LIB, "dem_")
+    b = scene.SceneBatch(scenes)
+    bad = scene.SceneBatch(scenes)
+    bad.particle_start = bad.particle_start.copy()
+    bad.particle_start[-1] -= 1
+    s = lib.solver()
+    with pytest.raises(_abi.DemError, match="scene row/wall ranges do not cover"):
+        s.initialize_batch(bad)
+    empty = scene.SceneBatch(scenes)
+    empty.particle_start = empty.particle_start.copy()
+    empty.particle_start[1] = empty.particle_start[0]
+    with pytest.raises(_abi.DemError, match="scene 0 has an empty or reversed row range"):
+        s.initialize_batch(empty)
+    uni = scene.SceneBatch(scenes)
+    uni.combined.universal_gravity = 1
+    with pytest.raises(_abi.DemError, match="cannot use universal gravitation"):
+        s.initialize_batch(uni)
+    s.close()
+    with pytest.raises(ValueError):
+        scene.SceneBatch([scenes[0], scene.Scene(**{**scenes[1].__dict__, "dt": 2e-6})])
+    assert b.n == sum(sc.n for sc in scenes)
+    parts = b.split(np.arange(b.n))
+    assert [len(p) for p in parts] == [sc.n for sc in scenes]
