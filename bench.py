@@ -86,6 +86,47 @@ def measure_sweep(dm, scene, steps, warmup, seq_steps, count, seed):
                         "per-scene mu/restitution/E, dt 1e-6, gravity"}
 
 
+def measure_adjoint(dm, scene, count, n, steps, K, seed):
+    """Differentiable batch: `count` settling beds of n spheres; the
+    gradient of bed height + plate load w.r.t. every scene's (mu,
+    restitution, E) through `steps` steps with checkpoints every K steps.
+    Cost ratio = (reverse sweep incl. recomputation) / (plain forward run of
+    the same steps)."""
+    import time as _t
+
+    import numpy as np
+    from dem_mbd_cuda_amd import diff, host
+
+    scenes = scene.settle_scenes(count=count, n=n, seed=seed)
+    batch = scene.SceneBatch(scenes)
+    s = dm.CudaDemSolver()
+    s.initialize_batch(batch)
+    s.synchronize()
+    t0 = _t.perf_counter()
+    host.run_pure(s, batch.combined, steps=steps)
+    s.synchronize()
+    t_fwd = _t.perf_counter() - t0
+    s.close()
+    q = dm.CudaDemSolver()
+    q.initialize_batch(batch)
+    q.synchronize()
+    t0 = _t.perf_counter()
+    h, f = q.diff_forward(steps, K, 4, scenes=count)
+    t_dfwd = _t.perf_counter() - t0
+    rng = np.random.RandomState(seed)
+    t0 = _t.perf_counter()
+    g = q.diff_backward(rng.normal(size=count), rng.normal(size=count))
+    t_bwd = _t.perf_counter() - t0
+    mem = q.device_memory_bytes()
+    q.close()
+    return {"scenes": count, "particles": batch.n, "steps": steps, "checkpoint_interval": K,
+            "forward_s": t_fwd, "forward_with_checkpoints_s": t_dfwd, "backward_s": t_bwd,
+            "adjoint_over_forward": t_bwd / t_fwd, "device_bytes": mem,
+            "finite_gradients": bool(np.all(np.isfinite(g))),
+            "workload": f"{count} settling beds x {n} spheres r ~ U[1, 1.5] mm, 30 mm box, dt 1e-6, "
+                        f"d(bed height + plate load)/d(mu, restitution, E) per scene"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +143,11 @@ def main():
     ap.add_argument("--sweep-scenes", type=int, default=256)
     ap.add_argument("--sweep-steps", type=int, default=100)
     ap.add_argument("--sweep-seq-steps", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-batch figure")
+    ap.add_argument("--adjoint-scenes", type=int, default=64)
+    ap.add_argument("--adjoint-n", type=int, default=5000)
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-K", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -163,6 +209,16 @@ def main():
               f"{sweep['batched_ms_per_step']:.3f} ms/step, sequential {sweep['sequential_ms_per_step']:.3f} "
               f"ms/step, speedup {sweep['speedup']:.1f}x ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
 
+    adjoint = None
+    if not args.no_adjoint:
+        t0 = time.time()
+        adjoint = measure_adjoint(dm, scene, args.adjoint_scenes, args.adjoint_n, args.adjoint_steps,
+                                  args.adjoint_K, 5)
+        print(f"# adjoint: {adjoint['scenes']} x {args.adjoint_n} spheres, {adjoint['steps']} steps: forward "
+              f"{adjoint['forward_s']:.2f}s, forward+checkpoints {adjoint['forward_with_checkpoints_s']:.2f}s, "
+              f"reverse {adjoint['backward_s']:.2f}s, ratio {adjoint['adjoint_over_forward']:.2f} "
+              f"({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         from dem_mbd_cuda_amd import _abi, host
@@ -206,6 +262,7 @@ def main():
                      "bytes_per_particle": law_bytes_per_particle(cbar, len(sc.wall_origin))},
         "cpu_baseline": cpu,
         "batched_sweep": sweep,
+        "adjoint": adjoint,
     }
     print(json.dumps(out))
 

@@ -146,6 +146,9 @@ class Library:
             bind("device_description", [], C.c_char_p)
         if hasattr(L, p + "synchronize"):
             bind("synchronize", [vp])
+        if hasattr(L, p + "diff_forward"):
+            bind("diff_forward", [vp, C.c_int, C.c_int, C.c_int, _dp, _dp])
+            bind("diff_backward", [vp, _dp, _dp, _dp])
         if hasattr(L, p + "debug_last_timings"):
             bind("debug_last_timings", [vp, _dp, C.c_int, C.POINTER(C.c_int)])
 
@@ -350,6 +353,24 @@ class Solver:
     def synchronize(self):
         if self.lib.has("synchronize"):
             self.lib.call("synchronize", self.h)
+
+    def diff_forward(self, steps, checkpoint_interval, plate_wall, scenes=1):
+        """Differentiable run; returns per-scene (bed height, plate load)."""
+        h = np.zeros(scenes)
+        f = np.zeros(scenes)
+        self.lib.call("diff_forward", self.h, int(steps), int(checkpoint_interval), int(plate_wall),
+                      _ptr(h, C.c_double), _ptr(f, C.c_double))
+        return h, f
+
+    def diff_backward(self, d_height, d_load):
+        """Per-scene d(objective)/d(MECH): [scenes][8] = pp (mu, epsN, epsS, kN),
+        pw (mu, epsN, epsS, kN)."""
+        dh = _f64(d_height)
+        dl = _f64(d_load)
+        out = np.zeros((len(dh), 8))
+        self.lib.call("diff_backward", self.h, _ptr(dh, C.c_double), _ptr(dl, C.c_double),
+                      _ptr(out, C.c_double))
+        return out
 
     def last_timings(self):
         buf = np.zeros(16)

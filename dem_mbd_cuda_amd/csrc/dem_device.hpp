@@ -10,6 +10,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
+// Law arithmetic compiles for the device and for the host (the host copies
+// serve the derivative checks; the kernels use the device copies).
+#define DEM_FN __host__ __device__ __forceinline__
+
 namespace dem {
 
 constexpr double kPi = 3.1415926535897932384626433832795;
@@ -19,26 +25,26 @@ struct V3 {
     double x, y, z;
 };
 
-__device__ __forceinline__ V3 mk(double x, double y, double z) { return V3{x, y, z}; }
-__device__ __forceinline__ V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ V3 scl(V3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 cross(V3 a, V3 b)
+DEM_FN V3 mk(double x, double y, double z) { return V3{x, y, z}; }
+DEM_FN V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+DEM_FN V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+DEM_FN V3 scl(V3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+DEM_FN double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DEM_FN V3 cross(V3 a, V3 b)
 {
     return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ double len(V3 a) { return sqrt(dot(a, a)); }
+DEM_FN double len(V3 a) { return sqrt(dot(a, a)); }
 // normalized(): zero below 1e-14 (contact:67-71)
-__device__ __forceinline__ V3 unit(V3 a)
+DEM_FN V3 unit(V3 a)
 {
     const double l = len(a);
     return l > 1.0e-14 ? scl(a, 1.0 / l) : mk(0.0, 0.0, 0.0);
 }
-__device__ __forceinline__ V3 zero3() { return mk(0.0, 0.0, 0.0); }
+DEM_FN V3 zero3() { return mk(0.0, 0.0, 0.0); }
 
 // world = R(q) body, R in the reference's rotateIv form (contact:105-112)
-__device__ __forceinline__ V3 rot(double qw, double qx, double qy, double qz, V3 v)
+DEM_FN V3 rot(double qw, double qx, double qy, double qz, V3 v)
 {
     return mk((1.0 - 2.0 * (qy * qy + qz * qz)) * v.x + 2.0 * (qx * qy - qw * qz) * v.y +
                   2.0 * (qx * qz + qw * qy) * v.z,
@@ -55,17 +61,17 @@ struct Material {
     double logN, logS;  // log(epsN), log(epsS)
 };
 
-__device__ __forceinline__ double damping(double l, double k, double m)
+DEM_FN double damping(double l, double k, double m)
 {
     return -2.0 * l * sqrt(k * m / (kPi * kPi + l * l));
 }
 
 // History slot values: [slot][roll(0), tangent(1), twist(2)][xyz]
-__device__ __forceinline__ V3 hget(const double* v, int cat)
+DEM_FN V3 hget(const double* v, int cat)
 {
     return mk(v[cat * 3], v[cat * 3 + 1], v[cat * 3 + 2]);
 }
-__device__ __forceinline__ void hput(double* v, int cat, V3 a)
+DEM_FN void hput(double* v, int cat, V3 a)
 {
     v[cat * 3] = a.x;
     v[cat * 3 + 1] = a.y;
@@ -73,7 +79,7 @@ __device__ __forceinline__ void hput(double* v, int cat, V3 a)
 }
 
 // transportTangentHistory / transportAxialHistory (contact:245-265)
-__device__ __forceinline__ V3 transport_tangent(V3 prev, V3 n)
+DEM_FN V3 transport_tangent(V3 prev, V3 n)
 {
     const double pn = len(prev);
     const V3 proj = sub(prev, scl(n, dot(prev, n)));
@@ -81,7 +87,7 @@ __device__ __forceinline__ V3 transport_tangent(V3 prev, V3 n)
     if (pn <= 1.0e-30 || qn <= 1.0e-30) return zero3();
     return scl(proj, pn / qn);
 }
-__device__ __forceinline__ V3 transport_axial(V3 prev, V3 n)
+DEM_FN V3 transport_axial(V3 prev, V3 n)
 {
     const double pn = len(prev);
     if (pn <= 1.0e-30) return zero3();
@@ -93,7 +99,7 @@ __device__ __forceinline__ V3 transport_axial(V3 prev, V3 n)
 // limitedHistoryResponseRates (contact:267-299): elastic trial from the
 // history rate, damping from the response rate, Coulomb-type cap on the
 // elastic trial, yield fallback rescaling the previous value.
-__device__ __forceinline__ V3 limited(V3 hrate, V3 rrate, double k, double c, double dt, V3 prev,
+DEM_FN V3 limited(V3 hrate, V3 rrate, double k, double c, double dt, V3 prev,
                                       double limit, double* slot, int cat, bool store,
                                       bool* yielded)
 {
@@ -122,7 +128,7 @@ __device__ __forceinline__ V3 limited(V3 hrate, V3 rrate, double k, double c, do
 }
 
 // wallHistoryResponse (contact:951-973)
-__device__ __forceinline__ V3 wall_limited(V3 rate, double k, double c, double dt, V3 prev,
+DEM_FN V3 wall_limited(V3 rate, double k, double c, double dt, V3 prev,
                                            double limit, bool checkDamped, bool zeroAtZero,
                                            double* slot, int cat)
 {
@@ -158,7 +164,7 @@ struct Body {
 // particleContact (contact:579-706) after the overlap test has passed and
 // the history slot has been found: the contact force on i (cf) and the term
 // the owner adds to its torque sum (couple + offset x cf).
-__device__ __forceinline__ void pp_law(const Body& I, const Body& J, V3 delta, double d2,
+DEM_FN void pp_law(const Body& I, const Body& J, V3 delta, double d2,
                                        const Material& m, double* slot, V3& cf_out, V3& tq_out)
 {
     const double ri = I.r, rj = J.r, rs = ri + rj;
@@ -207,7 +213,8 @@ __device__ __forceinline__ void pp_law(const Body& I, const Body& J, V3 delta, d
 }
 
 // wallContact (contact:975-1063) after the depth test and slot lookup.
-__device__ __forceinline__ void pw_law(const Body& I, V3 outward, double depth,
+// Accumulates the wall's force and torque contribution; returns the force.
+DEM_FN V3 pw_law(const Body& I, V3 outward, double depth,
                                        const Material& m, double* slot, V3& fsum, V3& tsum)
 {
     const V3 normal = scl(outward, -1.0);
@@ -240,6 +247,7 @@ __device__ __forceinline__ void pw_law(const Body& I, V3 outward, double depth,
     const V3 couple = add(add(tw, rl), scl(cross(normal, fr), lever));
     fsum = add(fsum, cf);
     tsum = add(tsum, add(couple, cross(I.off, cf)));
+    return cf;
 }
 
 // ---- arc tangent --------------------------------------------------------
@@ -249,7 +257,7 @@ __device__ __forceinline__ void pw_law(const Body& I, V3 outward, double depth,
 // against mu|Fn|; a last-bit difference in atan2 would flip that branch).
 // One half-angle reduction atan(t) = 2 atan(t / (1 + sqrt(1 + t^2))) takes
 // t <= 1 below tan(pi/8); the Taylor series to z^43/43 follows (within 3 ulp).
-__device__ __forceinline__ double series_atan(double t)
+DEM_FN double series_atan(double t)
 {
     t = t / (1.0 + sqrt(1.0 + t * t));
     const double z = t * t;
@@ -278,7 +286,7 @@ __device__ __forceinline__ double series_atan(double t)
     return (t * p) * 2.0;
 }
 
-__device__ inline double arc_tan2(double y, double x)
+DEM_FN double arc_tan2(double y, double x)
 {
     const double kHalfPi = 1.5707963267948966192313216916398;
     if (isnan(x) || isnan(y)) return x + y;
@@ -298,10 +306,10 @@ __device__ inline double arc_tan2(double y, double x)
 struct V2 {
     double x, y;
 };
-__device__ __forceinline__ double cross2(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
-__device__ __forceinline__ double dot2(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+DEM_FN double cross2(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
+DEM_FN double dot2(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
 
-__device__ inline double circle_edge_area(V2 a, V2 b, double radius)
+DEM_FN double circle_edge_area(V2 a, V2 b, double radius)
 {
     const V2 d{b.x - a.x, b.y - a.y};
     const double aa = dot2(d, d);
@@ -342,7 +350,7 @@ __device__ inline double circle_edge_area(V2 a, V2 b, double radius)
     return area;
 }
 
-__device__ inline double circle_triangle_area(V3 center, V3 normal, V3 a, V3 b, V3 c,
+DEM_FN double circle_triangle_area(V3 center, V3 normal, V3 a, V3 b, V3 c,
                                               double radius)
 {
     const V3 axis = fabs(normal.x) < 0.8 ? mk(1.0, 0.0, 0.0) : mk(0.0, 1.0, 0.0);
@@ -363,7 +371,7 @@ struct Tri {
 };
 
 // bodyTriangleAreaScale (contact:383-407)
-__device__ inline double tri_area_scale(V3 pos, double r, const Tri& t)
+DEM_FN double tri_area_scale(V3 pos, double r, const Tri& t)
 {
     const double distance = dot(t.n, sub(pos, t.a));
     if (distance <= 0.0) return 0.0;
@@ -392,7 +400,7 @@ struct PtOut {
 };
 
 // bodyContact with the manifold history read-only (dt = 0 in m).
-__device__ inline PtOut pt_law(const Body& I, const Tri& t, const Pose& P, const Material& m,
+DEM_FN PtOut pt_law(const Body& I, const Tri& t, const Pose& P, const Material& m,
                                const double* slot)
 {
     PtOut o;
@@ -451,7 +459,7 @@ __device__ inline PtOut pt_law(const Body& I, const Tri& t, const Pose& P, const
 }
 
 // advanceBodyManifoldHistory (contact:855-949) after the slot is found.
-__device__ inline void manifold_law(const Body& I, const Pose& P, const Material& m, V3 normal,
+DEM_FN void manifold_law(const Body& I, const Pose& P, const Material& m, V3 normal,
                                     V3 cc, double distance, double* slot, bool* slide,
                                     bool* twisty)
 {
@@ -485,7 +493,7 @@ __device__ inline void manifold_law(const Body& I, const Pose& P, const Material
 }
 
 // quatDerivative / normalizeQuat (particle:149-183)
-__device__ __forceinline__ void quat_dot(double qw, double qx, double qy, double qz, double wx,
+DEM_FN void quat_dot(double qw, double qx, double qy, double qz, double wx,
                                          double wy, double wz, double& dw, double& dx, double& dy,
                                          double& dz)
 {
@@ -494,7 +502,7 @@ __device__ __forceinline__ void quat_dot(double qw, double qx, double qy, double
     dy = 0.5 * (qw * wy - qx * wz + qz * wx);
     dz = 0.5 * (qw * wz + qx * wy - qy * wx);
 }
-__device__ __forceinline__ void quat_unit(double& qw, double& qx, double& qy, double& qz)
+DEM_FN void quat_unit(double& qw, double& qx, double& qy, double& qz)
 {
     const double n2 = qw * qw + qx * qx + qy * qy + qz * qz;
     if (n2 <= 1.0e-24) {

@@ -28,6 +28,7 @@
 // reordered; occupancy and "seen this step / last step" live in bit masks
 // instead of the reference's per-slot lastSeen words.
 #include "dem_device.hpp"
+#include "dem_adjoint.hpp"
 
 #include "../../include/dem_hip.h"
 
@@ -731,6 +732,20 @@ __device__ __forceinline__ int acquire(M& occ, M& cur, uint32_t* keys, double* v
     return b;
 }
 
+// What a differentiable run records of one step for the reverse sweep
+// (every pointer null when nothing is recorded).
+struct Tape {
+    unsigned long long* ppOcc;  // [n] particle-particle occupancy after expiry
+    unsigned* pwOcc;            // [n] particle-wall occupancy after expiry
+    int2* pair;                 // [contacts] (row, partner row << SLOT_BITS | slot)
+    double* pairPrev;           // [contacts][9] slot values the law read
+    int4* wall;                 // [wall contacts] (row, wall within the scene, slot, 0)
+    double* wallPrev;           // [wall contacts][9]
+    unsigned* wallCount;        // wall-contact append counter (counted even with wall == null)
+    double* plateAcc;           // [n] force of the plate wall on the particle (z), summed over steps
+    int plate;                  // plate wall index within each scene
+};
+
 struct LawArgs {
     Soa S;
     const Rec* rec;
@@ -754,6 +769,7 @@ struct LawArgs {
     int step;
     bool offs;
     int nb;
+    Tape tape;
 };
 
 // History lookup with the first 16 keys of the owner held in registers (kept
@@ -813,7 +829,8 @@ __device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long
 __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const double4* posr,
                                                 const uint32_t* cidS, const unsigned long long* key,
                                                 const unsigned* start, const SceneGrid* grid, Scalars* sc,
-                                                History H, int step, int* nbr, int* ncount)
+                                                History H, int step, int* nbr, int* ncount,
+                                                unsigned long long* occTape)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
@@ -862,6 +879,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
         }
         const long o = S.k(I_ORIG, p);
         unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
+        if (occTape) occTape[o] = occ;
         unsigned long long cur = 0;
         uint32_t* keys = H.ppKey + o * PP_SLOTS;
         double* vals = H.ppVal + o * PP_SLOTS * 9;
@@ -935,7 +953,8 @@ __global__ void __launch_bounds__(BLOCK) k_off_apply(const int* cnt, long n, con
 // not idle lanes.  Writes the contact force on the owner and its torque term.
 __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const Rec* rec,
                                                  const int2* pairs, History H, const Material* mats,
-                                                 bool first, bool offs, const Scalars* sc, double* pairF)
+                                                 bool first, bool offs, const Scalars* sc, double* pairF,
+                                                 int2* tapePair, double* tapePrev)
 {
     const long total = sc->total_pairs;
     for (long e = (long)blockIdx.x * BLOCK + threadIdx.x; e < total; e += (long)gridDim.x * BLOCK) {
@@ -951,8 +970,13 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const
         const long o = S.k(I_ORIG, p);
         Material m = mats[rec[p].scene];
         if (first) m.dt = 0.0;  // the first evaluation advances no history
+        double* hv = H.ppVal + (o * PP_SLOTS + slot) * 9;
+        if (tapePair) {
+            tapePair[e] = make_int2((int)p, pr.y);
+            for (int k = 0; k < 9; ++k) tapePrev[e * 9 + k] = hv[k];
+        }
         V3 cf, tq;
-        pp_law(I, J, delta, d2, m, H.ppVal + (o * PP_SLOTS + slot) * 9, cf, tq);
+        pp_law(I, J, delta, d2, m, hv, cf, tq);
         double* out = pairF + e * 6;
         out[0] = cf.x; out[1] = cf.y; out[2] = cf.z;
         out[3] = tq.x; out[4] = tq.y; out[5] = tq.z;
@@ -1026,6 +1050,7 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
         const History& H = A.H;
         unsigned pwOcc = H.pwOcc[o] & H.pwSeen[(long)((s + 1) & 1) * n + o];
         unsigned pwCur = 0;
+        if (A.tape.pwOcc) A.tape.pwOcc[o] = pwOcc;
         // walls of the particle's scene (history key: the scene's wall index + 1)
         const int scene = S.k(I_SCENE, p);
         const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
@@ -1041,7 +1066,16 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
             const int slot = acquire(pwOcc, pwCur, H.pwKey + o * PW_SLOTS, H.pwVal + o * PW_SLOTS * 9,
                                      PW_SLOTS, (uint32_t)(w + 1), bodyHigh, pwOver);
             if (slot < 0) continue;
-            pw_law(I, outward, depth, mw, H.pwVal + (o * PW_SLOTS + slot) * 9, wF, wT);
+            double* hv = H.pwVal + (o * PW_SLOTS + slot) * 9;
+            if (A.tape.wallCount) {
+                const unsigned e = atomicAdd(A.tape.wallCount, 1u);
+                if (A.tape.wall) {
+                    A.tape.wall[e] = make_int4((int)p, w, slot, 0);
+                    for (int k = 0; k < 9; ++k) A.tape.wallPrev[(long)e * 9 + k] = hv[k];
+                }
+            }
+            const V3 cf = pw_law(I, outward, depth, mw, hv, wF, wT);
+            if (A.tape.plateAcc && w == A.tape.plate && !A.first) A.tape.plateAcc[p] += cf.z;
         }
         // particle pairs, summed in candidate order from the per-contact results
         const int cnt = A.ncount[t];
@@ -1598,6 +1632,8 @@ int blocks_for(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
 }  // namespace
 
 // ============================================================ solver object
+struct DiffRun;                 // differentiable-run state (dem_diff.inl)
+void diff_release(DiffRun* r);
 // timing events: grid build | contact scan + list compaction | pair law |
 // owner sums (triangles, walls, field, acceleration) | body reductions
 constexpr int NEV = 6;
@@ -1655,6 +1691,8 @@ struct dem_solver {
     int2* pairs = nullptr;   // compacted (owner, entry) [MAX_LIST * n]
     double* pairF = nullptr; // per-contact force and torque term [MAX_LIST * n][6]
     int* byorig = nullptr;
+    Tape tape{};  // set by a differentiable run while it records a step
+    DiffRun* diff = nullptr;
     History H{};
     double* wallBuf = nullptr;
     Outputs O{};
@@ -1705,6 +1743,7 @@ struct dem_solver {
     ~dem_solver()
     {
         if (stream) hipStreamSynchronize(stream);
+        diff_release(diff);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
                         rec, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
@@ -2244,7 +2283,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     const int step = s->stepIndex++;
     const int nblk = blocks_for(n);
     hipLaunchKernelGGL(k_scan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->cidS, s->keyA,
-                       s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount);
+                       s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc);
     hipLaunchKernelGGL(k_off_blocks, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk);
     hipLaunchKernelGGL(k_off_scan, dim3(1), dim3(BLOCK), 0, st, s->listBlk, nblk, s->sc);
     hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk, s->nbr,
@@ -2252,7 +2291,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     TRY(launch_check("contact scan"));
     HIP_TRY(hipEventRecord(s->ev[2], st), "event");
     hipLaunchKernelGGL(k_pairs, dim3(PAIR_BLOCKS), dim3(BLOCK), 0, st, S, s->valA, s->rec, s->pairs, s->H,
-                       s->dMatPP, step == 0, s->offs, s->sc, s->pairF);
+                       s->dMatPP, step == 0, s->offs, s->sc, s->pairF, s->tape.pair, s->tape.pairPrev);
     TRY(launch_check("particle contact law"));
     HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
@@ -2264,7 +2303,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (step == 0) A.pt.dt = 0.0;
     A.g[0] = s->g[0]; A.g[1] = s->g[1]; A.g[2] = s->g[2];
     A.gravity = s->gravity; A.universal = s->universal; A.byorig = s->byorig;
-    A.step = step; A.offs = s->offs; A.nb = nb;
+    A.step = step; A.offs = s->offs; A.nb = nb; A.tape = s->tape;
     if (s->universal) hipLaunchKernelGGL(k_byorig, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->byorig);
     if (nb > 0) {
         const int fb = 1024;
@@ -2638,4 +2677,8 @@ int dem_has_restored_leapfrog_state(const dem_solver* s, int* out)
     return DEM_OK;
 }
 
+#include "dem_diff_checks.inl"
+
 }  // extern "C"
+
+#include "dem_diff.inl"

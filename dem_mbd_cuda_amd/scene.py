@@ -365,6 +365,37 @@ def sweep_scenes(count=256, seed=4, nmin=500, nmax=20000, rmin=1.0e-3, rmax=1.5e
     return out
 
 
+def settle_scenes(count=64, n=5000, seed=5, box=0.03, rmin=1.0e-3, rmax=1.5e-3, density=2500.0,
+                  dt=1.0e-6, steps=2000, overlap=0.01, mesh_scale=0.0, gravity=(0.0, 0.0, -9.81)):
+    """Settling beds for differentiation: `count` boxes of `box` x `box`
+    footprint, n spheres of radius U[rmin, rmax] laid in layers (largest
+    first) at spacing (1 - overlap) x diameter and partly relaxed, so the bed
+    starts with small overlaps and settles under gravity against the floor
+    (wall index 4 of box_walls); per scene mu ~ U[0.1, 0.9], restitution
+    ~ U[0.2, 0.9], E log-uniform in [1e6, 1e8] (kN = E * mean radius),
+    muT = muR = 0.1, beta = 0.5, no cohesion.  Row order is shuffled."""
+    rng = np.random.RandomState(seed)
+    out = []
+    for k in range(count):
+        mu = float(rng.uniform(0.1, 0.9))
+        e = float(rng.uniform(0.2, 0.9))
+        E = float(10.0 ** rng.uniform(6.0, 8.0))
+        radius = rng.uniform(rmin, rmax, n)
+        lo = np.zeros(3)
+        radius, pos = layered_lattice(radius, lo, (box, box), factor=1.0 - overlap)
+        relax_packing(radius, pos, lo, (box, box), max_iter=3, tol=overlap)
+        perm = rng.permutation(n)
+        radius, pos = radius[perm].copy(), pos[perm].copy()
+        top = float(np.max(pos[:, 2] + radius))
+        hi = np.array([box, box, top + 4.0 * rmax])
+        m = mech(epsS=e, mu=mu, epsN=e, cohesion=0.0, beta=0.5, muT=0.1, muR=0.1, kN=E * float(np.mean(radius)))
+        wo, wn = box_walls(lo, hi)
+        out.append(Scene(**sphere_particles(pos, radius, density), wall_origin=wo, wall_normal=wn,
+                         pp=m, pt=dict(m), pw=dict(m), dt=dt, gravity=tuple(gravity), gravity_enabled=1,
+                         mesh_scale=mesh_scale, steps=steps))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Readers for the reference's input formats.
 # ---------------------------------------------------------------------------

@@ -249,6 +249,39 @@ int dem_synchronize(const dem_solver* solver);
 int dem_debug_last_timings(const dem_solver* solver, double* ms_out,
                            int capacity, int* count);
 
+/* ---- differentiable runs (no reference counterpart) ---------------------
+ * Reverse-mode derivatives of scene observables with respect to each
+ * scene's contact laws, for a solver freshly initialised (dem_initialize or
+ * dem_initialize_batch) with spheres, walls and gravity only.  The run is the
+ * reference's pure-DEM loop (cuda_simulation_integrator.cpp:245-286): the
+ * first force evaluation, the half-step initialisation, then `steps` x
+ * (advance, forces, finish), without particle reordering.  Per scene it
+ * reports the mean centre height z after the last step and the plate load:
+ * the z force of wall `plate_wall` (an index within each scene's walls) on
+ * the scene's particles, averaged over the `steps` force evaluations after
+ * the first.  The forward run keeps a checkpoint of the particle state and
+ * contact history every `checkpoint_interval` steps on the device. */
+int dem_diff_forward(dem_solver* solver, int steps, int checkpoint_interval,
+                     int plate_wall, double* bed_height /* [scenes] */,
+                     double* plate_load /* [scenes] */);
+/* After dem_diff_forward: given d(objective)/d(bed_height) and
+ * d(objective)/d(plate_load) per scene, returns d(objective)/d(MECH) per
+ * scene as [scene][8] = {mu, epsN, epsS, kN} of the particle-particle law,
+ * then the same four of the particle-wall law.  Recomputes every checkpoint
+ * interval from its checkpoint and sweeps it backwards; may be called again
+ * with other seeds.  Leaves the solver at the end of the run. */
+int dem_diff_backward(dem_solver* solver, const double* d_bed_height,
+                      const double* d_plate_load, double* d_mech /* [scenes][8] */);
+
+/* Host evaluations of one particle-particle / particle-wall contact law and
+ * of its reverse-mode derivative (the arithmetic the kernels run), for
+ * derivative checks; the packed layouts are documented in
+ * dem_mbd_cuda_amd/csrc/dem_diff_checks.inl.  No device is needed. */
+int dem_check_pp_law(const double* in, double* out);
+int dem_check_pp_law_vjp(const double* in, const double* seeds, double* grads);
+int dem_check_pw_law(const double* in, double* out);
+int dem_check_pw_law_vjp(const double* in, const double* seeds, double* grads);
+
 #ifdef __cplusplus
 }
 #endif
