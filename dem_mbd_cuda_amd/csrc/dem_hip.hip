@@ -1654,6 +1654,7 @@ struct dem_solver {
     uint64_t intervalSteps = 0;
     size_t freeBefore = 0;
     bool restored = false;
+    bool legacySphere = false;  // one zero-offset component per owner, id = owner id
 
     // device memory
     double* dState[2] = {nullptr, nullptr};
@@ -1987,6 +1988,10 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         s->compIds[p] = C->id[k]; s->ownerIds[p] = P->id[p]; s->compR[p] = C->radius[k];
         s->compB[3 * p] = D(F_BX, p); s->compB[3 * p + 1] = D(F_BY, p); s->compB[3 * p + 2] = D(F_BZ, p);
     }
+    s->legacySphere = C->count == n;
+    for (long p = 0; p < n && s->legacySphere; ++p)
+        s->legacySphere = s->compIds[p] == P->id[p] && s->compR[p] == P->radius[p] && s->compB[3 * p] == 0.0 &&
+                          s->compB[3 * p + 1] == 0.0 && s->compB[3 * p + 2] == 0.0;
     for (long p = 0; p < n; ++p) {
         D(F_X, p) = P->pos[3 * p]; D(F_Y, p) = P->pos[3 * p + 1]; D(F_Z, p) = P->pos[3 * p + 2];
         D(F_VX, p) = P->vel[3 * p]; D(F_VY, p) = P->vel[3 * p + 1]; D(F_VZ, p) = P->vel[3 * p + 2];
@@ -2656,18 +2661,231 @@ int dem_debug_last_timings(const dem_solver* s, double* ms, int capacity, int* c
     return DEM_OK;
 }
 
+// CudaDemSolver::loadContactHistory / saveContactHistory
+// (cuda_backend_cuda.cu:545-770): the DMHIST2 layout, byte for byte (native
+// byte order, checked by the endian marker):
+//   "DMHIST2\0", u32 version 2, u32 0x01020304, u32 owners, u32 components,
+//   u32 slots[3] = {64, 16, 64}, u32 topology record size 40, u32 owner-state
+//   record size 188; per component (i32 id, i32 owner id, f64 radius, body
+//   x, y, z); per owner (i32 id, 23 f64: x y z vx vy vz qw qx qy qz wx wy wz
+//   vhx vhy vhz whx why whz hdqw hdqx hdqy hdqz); the particle, wall and body
+//   history keys (u64, component-major, 0 = empty) and their values (9 f64
+//   per slot).  DMHIST1 ("DMHIST1\0", version 1, components, slots[3], keys,
+//   values) is accepted for the legacy sphere topology and restores no
+//   leapfrog state.  Keys and values of slots that the occupancy masks mark
+//   free are written as 0, as the reference's expiry leaves them.
 int dem_load_contact_history(dem_solver* s, const char* path)
 {
-    (void)path;
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "cannot load contact history before CUDA initialization");
-    return set_error(DEM_E_UNSUPPORTED, "DMHIST2 checkpoint reading is not implemented yet");
+    if (!path) return set_error(DEM_E_INVALID, "null argument");
+    TRY(flush_finish(s));
+    HIP_TRY(hipStreamSynchronize(s->stream), "load contact history");
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return set_error(DEM_E_IO, "cannot open CUDA contact-history checkpoint: %s", path);
+    auto rd = [&](void* p, size_t bytes) {
+        in.read(reinterpret_cast<char*>(p), (std::streamsize)bytes);
+        return (bool)in;
+    };
+    const long n = s->n;
+    char magic[8];
+    uint32_t version = 0, comps = 0, slots[3] = {0, 0, 0};
+    if (!rd(magic, 8) || !rd(&version, 4))
+        return set_error(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path);
+    s->restored = false;
+    bool v2 = false;
+    std::vector<double> state;
+    if (std::memcmp(magic, "DMHIST1", 8) == 0 && version == 1) {
+        if (!rd(&comps, 4) || !rd(slots, 12))
+            return set_error(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path);
+        if (!s->legacySphere)
+            return set_error(DEM_E_INVALID, "legacy DMHIST1 checkpoint is forbidden for clump topology: %s", path);
+    } else if (std::memcmp(magic, "DMHIST2", 8) == 0 && version == 2) {
+        uint32_t endian = 0, owners = 0, topo = 0, rec = 0;
+        if (!rd(&endian, 4) || !rd(&owners, 4) || !rd(&comps, 4) || !rd(slots, 12) || !rd(&topo, 4) || !rd(&rec, 4))
+            return set_error(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path);
+        if (endian != 0x01020304u || owners != (uint32_t)n || topo != 40u || rec != 188u)
+            return set_error(DEM_E_INVALID, "incompatible DMHIST2 header: %s", path);
+        if (comps != (uint32_t)n) return set_error(DEM_E_INVALID, "DMHIST2 component count mismatch: %s", path);
+        for (uint32_t i = 0; i < comps; ++i) {
+            int32_t ids[2];
+            double geo[4];
+            if (!rd(ids, 8) || !rd(geo, 32))
+                return set_error(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path);
+            if (ids[0] != s->compIds[i] || ids[1] != s->ownerIds[i] || geo[0] != s->compR[i] ||
+                geo[1] != s->compB[3 * i] || geo[2] != s->compB[3 * i + 1] || geo[3] != s->compB[3 * i + 2])
+                return set_error(DEM_E_INVALID, "DMHIST2 component topology mismatch: %s", path);
+        }
+        state.resize((size_t)23 * n);
+        for (long i = 0; i < n; ++i) {
+            int32_t id;
+            if (!rd(&id, 4) || !rd(&state[(size_t)23 * i], 23 * 8))
+                return set_error(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path);
+            if (id != s->ownerIds[i]) return set_error(DEM_E_INVALID, "DMHIST2 owner identity mismatch: %s", path);
+        }
+        v2 = true;
+    } else {
+        return set_error(DEM_E_INVALID, "unsupported contact-history checkpoint: %s", path);
+    }
+    if (comps != (uint32_t)n || slots[0] != (uint32_t)PP_SLOTS || slots[1] != (uint32_t)PW_SLOTS ||
+        slots[2] != (uint32_t)PT_SLOTS)
+        return set_error(DEM_E_INVALID, "incompatible CUDA contact-history checkpoint: %s", path);
+    const size_t nc = (size_t)n;
+    std::vector<uint64_t> pk(nc * PP_SLOTS), wk(nc * PW_SLOTS), bk(nc * PT_SLOTS);
+    std::vector<double> pv(nc * PP_SLOTS * 9), wv(nc * PW_SLOTS * 9), bv(nc * PT_SLOTS * 9);
+    if (!rd(pk.data(), 8 * pk.size()) || !rd(wk.data(), 8 * wk.size()) || !rd(bk.data(), 8 * bk.size()) ||
+        !rd(pv.data(), 8 * pv.size()) || !rd(wv.data(), 8 * wv.size()) || !rd(bv.data(), 8 * bv.size()))
+        return set_error(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path);
+    char extra;
+    if (in.read(&extra, 1)) return set_error(DEM_E_INVALID, "oversized CUDA contact-history checkpoint: %s", path);
+    for (uint64_t k : pk)
+        if (k > 0xFFFFFFFFull) return set_error(DEM_E_INVALID, "incompatible CUDA contact-history checkpoint: %s", path);
+    for (uint64_t k : wk)
+        if (k > 0xFFFFFFFFull) return set_error(DEM_E_INVALID, "incompatible CUDA contact-history checkpoint: %s", path);
+    for (uint64_t k : bk)
+        if (k > 0xFFFFFFFFull) return set_error(DEM_E_INVALID, "incompatible CUDA contact-history checkpoint: %s", path);
+    hipStream_t st = s->stream;
+    if (v2) {
+        // owner state by input row -> current rows
+        std::vector<double> hd((size_t)NDF * n);
+        std::vector<int> orig(n);
+        HIP_TRY(hipMemcpy(hd.data(), s->dState[s->cur], sizeof(double) * hd.size(), hipMemcpyDeviceToHost),
+                "download particle state");
+        HIP_TRY(hipMemcpy(orig.data(), s->iState[s->cur] + (size_t)I_ORIG * n, sizeof(int) * n,
+                          hipMemcpyDeviceToHost), "download particle rows");
+        const int fields[23] = {F_X,   F_Y,   F_Z,   F_VX,  F_VY,  F_VZ,  F_QW,  F_QX,
+                                F_QY,  F_QZ,  F_WX,  F_WY,  F_WZ,  F_VHX, F_VHY, F_VHZ,
+                                F_WHX, F_WHY, F_WHZ, F_HQW, F_HQX, F_HQY, F_HQZ};
+        for (long p = 0; p < n; ++p) {
+            const double* q = &state[(size_t)23 * orig[p]];
+            for (int k = 0; k < 23; ++k) hd[(size_t)fields[k] * n + p] = q[k];
+        }
+        HIP_TRY(hipMemcpy(s->dState[s->cur], hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice),
+                "upload particle state");
+    }
+    // keys -> occupancy masks; every restored slot counts as seen on the step
+    // before the next one (the reference resets lastSeen to -1 and the step
+    // counter to 0, so nothing expires on the next evaluation)
+    auto masks = [&](const std::vector<uint64_t>& keys, int slots_, std::vector<uint32_t>& k32,
+                     std::vector<unsigned long long>& occ) {
+        k32.resize(keys.size());
+        occ.assign(nc, 0ull);
+        for (size_t i = 0; i < nc; ++i)
+            for (int b = 0; b < slots_; ++b) {
+                const uint64_t k = keys[i * slots_ + b];
+                k32[i * slots_ + b] = (uint32_t)k;
+                if (k) occ[i] |= 1ull << b;
+            }
+    };
+    std::vector<uint32_t> pk32, wk32, bk32;
+    std::vector<unsigned long long> pocc, wocc, bocc;
+    masks(pk, PP_SLOTS, pk32, pocc);
+    masks(wk, PW_SLOTS, wk32, wocc);
+    masks(bk, PT_SLOTS, bk32, bocc);
+    std::vector<unsigned long long> pseen(2 * nc, 0ull);
+    std::copy(pocc.begin(), pocc.end(), pseen.begin() + nc);
+    std::vector<unsigned> wocc32(nc), wseen(2 * nc, 0u);
+    for (size_t i = 0; i < nc; ++i) wocc32[i] = wseen[nc + i] = (unsigned)wocc[i];
+    HIP_TRY(hipMemcpy(s->H.ppOcc, pocc.data(), 8 * nc, hipMemcpyHostToDevice), "upload particle history");
+    HIP_TRY(hipMemcpy(s->H.ppSeen, pseen.data(), 16 * nc, hipMemcpyHostToDevice), "upload particle history");
+    HIP_TRY(hipMemcpy(s->H.ppKey, pk32.data(), 4 * pk32.size(), hipMemcpyHostToDevice), "upload particle history keys");
+    HIP_TRY(hipMemcpy(s->H.ppVal, pv.data(), 8 * pv.size(), hipMemcpyHostToDevice), "upload particle history values");
+    HIP_TRY(hipMemcpy(s->H.pwOcc, wocc32.data(), 4 * nc, hipMemcpyHostToDevice), "upload wall history");
+    HIP_TRY(hipMemcpy(s->H.pwSeen, wseen.data(), 8 * nc, hipMemcpyHostToDevice), "upload wall history");
+    HIP_TRY(hipMemcpy(s->H.pwKey, wk32.data(), 4 * wk32.size(), hipMemcpyHostToDevice), "upload wall history keys");
+    HIP_TRY(hipMemcpy(s->H.pwVal, wv.data(), 8 * wv.size(), hipMemcpyHostToDevice), "upload wall history values");
+    if (s->H.ptOcc) {
+        std::vector<unsigned long long> bseen(2 * nc, 0ull);
+        std::copy(bocc.begin(), bocc.end(), bseen.begin() + nc);
+        HIP_TRY(hipMemcpy(s->H.ptOcc, bocc.data(), 8 * nc, hipMemcpyHostToDevice), "upload body history");
+        HIP_TRY(hipMemcpy(s->H.ptSeen, bseen.data(), 16 * nc, hipMemcpyHostToDevice), "upload body history");
+        HIP_TRY(hipMemcpy(s->H.ptKey, bk32.data(), 4 * bk32.size(), hipMemcpyHostToDevice), "upload body history keys");
+        HIP_TRY(hipMemcpy(s->H.ptVal, bv.data(), 8 * bv.size(), hipMemcpyHostToDevice), "upload body history values");
+    }
+    (void)st;
+    s->stepIndex = 0;
+    s->restored = v2;
+    return DEM_OK;
 }
 
-int dem_save_contact_history(const dem_solver* s, const char* path)
+int dem_save_contact_history(const dem_solver* cs, const char* path)
 {
-    (void)path;
-    if (!s || !s->initialized) return set_error(DEM_E_INVALID, "cannot save contact history before CUDA initialization");
-    return set_error(DEM_E_UNSUPPORTED, "DMHIST2 checkpoint writing is not implemented yet");
+    if (!cs || !cs->initialized) return set_error(DEM_E_INVALID, "cannot save contact history before CUDA initialization");
+    if (!path) return set_error(DEM_E_INVALID, "null argument");
+    TRY(flush_finish(cs));
+    HIP_TRY(hipStreamSynchronize(cs->stream), "save contact history");
+    const long n = cs->n;
+    const size_t nc = (size_t)n;
+    std::vector<double> hd((size_t)NDF * n);
+    std::vector<int> orig(n);
+    HIP_TRY(hipMemcpy(hd.data(), cs->dState[cs->cur], sizeof(double) * hd.size(), hipMemcpyDeviceToHost),
+            "download particle state");
+    HIP_TRY(hipMemcpy(orig.data(), cs->iState[cs->cur] + (size_t)I_ORIG * n, sizeof(int) * n, hipMemcpyDeviceToHost),
+            "download particle rows");
+    std::vector<double> state((size_t)23 * n);
+    const int fields[23] = {F_X,   F_Y,   F_Z,   F_VX,  F_VY,  F_VZ,  F_QW,  F_QX,  F_QY,  F_QZ,  F_WX, F_WY,
+                            F_WZ,  F_VHX, F_VHY, F_VHZ, F_WHX, F_WHY, F_WHZ, F_HQW, F_HQX, F_HQY, F_HQZ};
+    for (long p = 0; p < n; ++p)
+        for (int k = 0; k < 23; ++k) state[(size_t)23 * orig[p] + k] = hd[(size_t)fields[k] * n + p];
+    // history tables with free slots cleared
+    auto table = [&](const void* occDev, bool wide, const uint32_t* keyDev, const double* valDev, int slots_,
+                     std::vector<uint64_t>& keys, std::vector<double>& vals) -> int {
+        keys.assign(nc * slots_, 0ull);
+        vals.assign(nc * slots_ * 9, 0.0);
+        if (!occDev) return DEM_OK;
+        std::vector<unsigned long long> occ(nc);
+        if (wide) {
+            HIP_TRY(hipMemcpy(occ.data(), occDev, 8 * nc, hipMemcpyDeviceToHost), "download history");
+        } else {
+            std::vector<unsigned> o32(nc);
+            HIP_TRY(hipMemcpy(o32.data(), occDev, 4 * nc, hipMemcpyDeviceToHost), "download history");
+            for (size_t i = 0; i < nc; ++i) occ[i] = o32[i];
+        }
+        std::vector<uint32_t> k32(nc * slots_);
+        std::vector<double> v(nc * slots_ * 9);
+        HIP_TRY(hipMemcpy(k32.data(), keyDev, 4 * k32.size(), hipMemcpyDeviceToHost), "download history keys");
+        HIP_TRY(hipMemcpy(v.data(), valDev, 8 * v.size(), hipMemcpyDeviceToHost), "download history values");
+        for (size_t i = 0; i < nc; ++i)
+            for (int b = 0; b < slots_; ++b)
+                if ((occ[i] >> b) & 1ull) {
+                    const size_t idx = i * slots_ + b;
+                    keys[idx] = k32[idx];
+                    for (int q = 0; q < 9; ++q) vals[idx * 9 + q] = v[idx * 9 + q];
+                }
+        return DEM_OK;
+    };
+    std::vector<uint64_t> pk, wk, bk;
+    std::vector<double> pv, wv, bv;
+    TRY(table(cs->H.ppOcc, true, cs->H.ppKey, cs->H.ppVal, PP_SLOTS, pk, pv));
+    TRY(table(cs->H.pwOcc, false, cs->H.pwKey, cs->H.pwVal, PW_SLOTS, wk, wv));
+    TRY(table(cs->H.ptOcc, true, cs->H.ptKey, cs->H.ptVal, PT_SLOTS, bk, bv));
+    std::ofstream out(path, std::ios::binary | std::ios::trunc);
+    if (!out) return set_error(DEM_E_IO, "cannot create CUDA contact-history checkpoint: %s", path);
+    auto wr = [&](const void* p, size_t bytes) { out.write(reinterpret_cast<const char*>(p), (std::streamsize)bytes); };
+    const char magic[8] = {'D', 'M', 'H', 'I', 'S', 'T', '2', '\0'};
+    const uint32_t head[9] = {2u, 0x01020304u, (uint32_t)n, (uint32_t)n, (uint32_t)PP_SLOTS, (uint32_t)PW_SLOTS,
+                              (uint32_t)PT_SLOTS, 40u, 188u};
+    wr(magic, 8);
+    wr(head, sizeof head);
+    for (long i = 0; i < n; ++i) {
+        const int32_t ids[2] = {cs->compIds[i], cs->ownerIds[i]};
+        const double geo[4] = {cs->compR[i], cs->compB[3 * i], cs->compB[3 * i + 1], cs->compB[3 * i + 2]};
+        wr(ids, 8);
+        wr(geo, 32);
+    }
+    for (long i = 0; i < n; ++i) {
+        const int32_t id = cs->ownerIds[i];
+        wr(&id, 4);
+        wr(&state[(size_t)23 * i], 23 * 8);
+    }
+    wr(pk.data(), 8 * pk.size());
+    wr(wk.data(), 8 * wk.size());
+    wr(bk.data(), 8 * bk.size());
+    wr(pv.data(), 8 * pv.size());
+    wr(wv.data(), 8 * wv.size());
+    wr(bv.data(), 8 * bv.size());
+    out.close();
+    if (!out) return set_error(DEM_E_IO, "failed to write CUDA contact-history checkpoint: %s", path);
+    return DEM_OK;
 }
 
 int dem_has_restored_leapfrog_state(const dem_solver* s, int* out)

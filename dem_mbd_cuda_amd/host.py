@@ -10,6 +10,7 @@ files, so the same loop drives tests, benchmarks and file writers.
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 
@@ -69,8 +70,20 @@ def _pose13(b):
     return np.concatenate([b["center"], b["orien"], b["vel"], b["angvel"]], axis=1)
 
 
-def run_pure(solver, scene, steps=None, output_interval=None, on_frame=None, start_step=0):
-    """IntegrateDem (reference cuda_simulation_integrator.cpp:245-286)."""
+def restored(solver):
+    """hasRestoredParticleLeapfrogState: a DMHIST2 checkpoint was loaded."""
+    f = getattr(solver, "restored_leapfrog", None)
+    return bool(f()) if f is not None else False
+
+
+def run_pure(solver, scene, steps=None, output_interval=None, on_frame=None, start_step=0,
+             history_in=None, history_out=None):
+    """IntegrateDem (reference cuda_simulation_integrator.cpp:245-286).
+    history_in: contact-history checkpoint loaded before the first force
+    evaluation when the file exists; history_out: written after the last
+    step (the reference's InputFile/ and OutputFile/contact_history.bt)."""
+    if history_in is not None and os.path.exists(history_in):
+        solver.load_history(history_in)
     dt = scene.dt
     half = 0.5 * dt
     steps = scene.steps if steps is None else steps
@@ -79,7 +92,7 @@ def run_pure(solver, scene, steps=None, output_interval=None, on_frame=None, sta
     solver.compute_forces(collect=True)
     if on_frame:
         on_frame(-1, 0.0, None)
-    if not getattr(solver, "restored", False):
+    if not restored(solver):
         solver.init_half_step(half)
     counter = 0
     for step in range(start_step, end):
@@ -93,6 +106,8 @@ def run_pure(solver, scene, steps=None, output_interval=None, on_frame=None, sta
             if on_frame:
                 on_frame(step, next_time, None)
             counter = 0
+    if history_out is not None:
+        solver.save_history(history_out)
 
 
 def body_state_from_scene(scene):
@@ -104,10 +119,14 @@ def body_state_from_scene(scene):
                 state=np.asarray(scene.body_state, int))
 
 
-def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, start_step=0):
+def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, start_step=0,
+                history_in=None, history_out=None):
     """IntegrateDemMultiBody (reference cuda_simulation_integrator.cpp:288-446),
     without the prescribed-motion schedule file.  on_frame(step, time, bodies,
-    loads, impulses) gets the body state and the loads of that frame."""
+    loads, impulses) gets the body state and the loads of that frame.
+    history_in / history_out as for run_pure."""
+    if history_in is not None and os.path.exists(history_in):
+        solver.load_history(history_in)
     dt = scene.dt
     half = 0.5 * dt
     steps = scene.steps if steps is None else steps
@@ -120,7 +139,7 @@ def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, 
     if on_frame:
         on_frame(-1, 0.0, dict(bodies={k: v.copy() for k, v in B.items()}, loads=loads.copy(),
                                impulses=impulses.copy()))
-    if not getattr(solver, "restored", False):
+    if not restored(solver):
         solver.init_half_step(half)
     acc = dyn_eqn_bodyset(B, loads)
     hv = np.zeros((nb, 3)); hw = np.zeros((nb, 3)); hq = np.zeros((nb, 4)); hdq = np.zeros((nb, 4))
@@ -182,4 +201,6 @@ def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, 
                                              loads=loads.copy(), impulses=impulses.copy()))
             impulses[:] = 0.0
             counter = 0
+    if history_out is not None:
+        solver.save_history(history_out)
     return B

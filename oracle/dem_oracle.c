@@ -167,6 +167,8 @@ struct ora_solver {
     double *microState, *microInstant, *microInterval;
     Mech pp, pt, pw;
     int stepIndex;
+    int restored;     /* restoredParticleLeapfrogState (backend:658-770) */
+    int legacySphere; /* one zero-offset component per owner (backend:172-177) */
     uint64_t intervalSteps;
     dem_step_stats cached;
     uint64_t *cachedRegimeInstant, *cachedRegimeInterval;
@@ -317,6 +319,11 @@ int ora_initialize(ora_solver* s, const dem_particles_view* P,
     }
     s->maxRadius = maxr;
     s->meshSize = (ctl->mesh_scale > 0.0 ? ctl->mesh_scale : 2.5) * maxr;
+    /* legacy sphere topology (backend:172-177) */
+    s->legacySphere = nc == n;
+    for (int i = 0; i < nc && s->legacySphere; ++i)
+        s->legacySphere = s->cowner[i] == i && s->cid[i] == P->id[i] && s->cr[i] == P->radius[i] &&
+                          s->cbx[i] == 0.0 && s->cby[i] == 0.0 && s->cbz[i] == 0.0;
     /* walls */
     s->nw = W ? W->count : 0;
     ALLOC(s->wox, s->nw); ALLOC(s->woy, s->nw); ALLOC(s->woz, s->nw);
@@ -1558,5 +1565,160 @@ int ora_debug_triangle_grid(const ora_solver* s, uint64_t* keys, int32_t* tris)
     if (!s || !s->initialized) return fail(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     if (keys && s->tentries) memcpy(keys, s->tkey, sizeof(uint64_t) * s->tentries);
     if (tris && s->tentries) memcpy(tris, s->ttri, sizeof(int32_t) * s->tentries);
+    return DEM_OK;
+}
+
+/* ---- contact-history checkpoint (CudaDemSolver::saveContactHistory /
+ * loadContactHistory, cuda_backend_cuda.cu:545-770).  Layout (native byte
+ * order, checked by the endian marker): "DMHIST2\0", u32 version 2, u32
+ * 0x01020304, u32 owners, u32 components, u32 slots[3] = {64, 16, 64}, u32
+ * topology record size 40, u32 owner-state record size 188; per component
+ * (i32 id, i32 owner id, f64 radius, body x, y, z); per owner (i32 id, 23 f64:
+ * x y z vx vy vz qw qx qy qz wx wy wz vhx vhy vhz whx why whz hdqw hdqx hdqy
+ * hdqz); then the particle, wall and body history keys (u64, component-major,
+ * slot-minor) and values (9 f64 per slot) in that order.  DMHIST1 ("DMHIST1\0",
+ * version 1, components, slots[3], keys, values) is accepted for the legacy
+ * sphere topology only and restores no leapfrog state. */
+static int write_all(FILE* f, const void* p, size_t bytes) { return fwrite(p, 1, bytes, f) == bytes; }
+static int read_all(FILE* f, void* p, size_t bytes) { return fread(p, 1, bytes, f) == bytes; }
+
+int ora_save_contact_history(const ora_solver* s, const char* path)
+{
+    if (!s || !s->initialized) return fail(DEM_E_INVALID, "cannot save contact history before CUDA initialization");
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(DEM_E_IO, "cannot create CUDA contact-history checkpoint: %s", path);
+    const char magic[8] = {'D', 'M', 'H', 'I', 'S', 'T', '2', '\0'};
+    const uint32_t head[10] = {2u, 0x01020304u, (uint32_t)s->n, (uint32_t)s->nc, PP_SLOTS, PW_SLOTS, PT_SLOTS,
+                               2u * 4u + 4u * 8u, 4u + 23u * 8u, 0u};
+    int ok = write_all(f, magic, 8) && write_all(f, head, 9 * sizeof(uint32_t));
+    for (int i = 0; i < s->nc && ok; ++i) {
+        const int32_t ids[2] = {s->cid[i], s->pid[s->cowner[i]]};
+        const double geo[4] = {s->cr[i], s->cbx[i], s->cby[i], s->cbz[i]};
+        ok = write_all(f, ids, sizeof ids) && write_all(f, geo, sizeof geo);
+    }
+    for (int i = 0; i < s->n && ok; ++i) {
+        const int32_t id = s->pid[i];
+        const double st[23] = {s->x[i],   s->y[i],   s->z[i],   s->vx[i],  s->vy[i],   s->vz[i],
+                               s->qw[i],  s->qx[i],  s->qy[i],  s->qz[i],  s->wx[i],   s->wy[i],
+                               s->wz[i],  s->vhx[i], s->vhy[i], s->vhz[i], s->whx[i],  s->why[i],
+                               s->whz[i], s->hdqw[i], s->hdqx[i], s->hdqy[i], s->hdqz[i]};
+        ok = write_all(f, &id, sizeof id) && write_all(f, st, sizeof st);
+    }
+    const size_t nc = (size_t)s->nc;
+    if (ok)
+        ok = write_all(f, s->hpk, 8 * nc * PP_SLOTS) && write_all(f, s->hwk, 8 * nc * PW_SLOTS) &&
+             write_all(f, s->hbk, 8 * nc * PT_SLOTS) && write_all(f, s->hpv, 72 * nc * PP_SLOTS) &&
+             write_all(f, s->hwv, 72 * nc * PW_SLOTS) && write_all(f, s->hbv, 72 * nc * PT_SLOTS);
+    if (fclose(f) != 0) ok = 0;
+    return ok ? DEM_OK : fail(DEM_E_IO, "failed to write CUDA contact-history checkpoint: %s", path);
+}
+
+int ora_load_contact_history(ora_solver* s, const char* path)
+{
+    if (!s || !s->initialized) return fail(DEM_E_INVALID, "cannot load contact history before CUDA initialization");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(DEM_E_IO, "cannot open CUDA contact-history checkpoint: %s", path);
+    int rc = DEM_OK;
+    char magic[8];
+    uint32_t version = 0, comps = 0, slots[3] = {0, 0, 0};
+    double* state = NULL;
+    uint64_t *pk = NULL, *wk = NULL, *bk = NULL;
+    double *pv = NULL, *wv = NULL, *bv = NULL;
+#define TRUNC() do { rc = fail(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path); goto done; } while (0)
+    if (!read_all(f, magic, 8) || !read_all(f, &version, 4)) TRUNC();
+    s->restored = 0;
+    int v2 = 0;
+    if (memcmp(magic, "DMHIST1", 8) == 0 && version == 1) {
+        if (!read_all(f, &comps, 4) || !read_all(f, slots, 12)) TRUNC();
+        if (!s->legacySphere) {
+            rc = fail(DEM_E_INVALID, "legacy DMHIST1 checkpoint is forbidden for clump topology: %s", path);
+            goto done;
+        }
+    } else if (memcmp(magic, "DMHIST2", 8) == 0 && version == 2) {
+        uint32_t endian = 0, owners = 0, topo = 0, rec = 0;
+        if (!read_all(f, &endian, 4) || !read_all(f, &owners, 4) || !read_all(f, &comps, 4) ||
+            !read_all(f, slots, 12) || !read_all(f, &topo, 4) || !read_all(f, &rec, 4))
+            TRUNC();
+        if (endian != 0x01020304u || owners != (uint32_t)s->n || topo != 40u || rec != 188u) {
+            rc = fail(DEM_E_INVALID, "incompatible DMHIST2 header: %s", path);
+            goto done;
+        }
+        if (comps != (uint32_t)s->nc) {
+            rc = fail(DEM_E_INVALID, "DMHIST2 component count mismatch: %s", path);
+            goto done;
+        }
+        for (uint32_t i = 0; i < comps; ++i) {
+            int32_t ids[2];
+            double geo[4];
+            if (!read_all(f, ids, 8) || !read_all(f, geo, 32)) TRUNC();
+            if (ids[0] != s->cid[i] || ids[1] != s->pid[s->cowner[i]] || geo[0] != s->cr[i] ||
+                geo[1] != s->cbx[i] || geo[2] != s->cby[i] || geo[3] != s->cbz[i]) {
+                rc = fail(DEM_E_INVALID, "DMHIST2 component topology mismatch: %s", path);
+                goto done;
+            }
+        }
+        state = calloc((size_t)s->n * 23 + 1, sizeof(double));
+        for (int i = 0; i < s->n; ++i) {
+            int32_t id;
+            if (!read_all(f, &id, 4) || !read_all(f, state + (size_t)i * 23, 23 * 8)) TRUNC();
+            if (id != s->pid[i]) {
+                rc = fail(DEM_E_INVALID, "DMHIST2 owner identity mismatch: %s", path);
+                goto done;
+            }
+        }
+        v2 = 1;
+    } else {
+        rc = fail(DEM_E_INVALID, "unsupported contact-history checkpoint: %s", path);
+        goto done;
+    }
+    if (comps != (uint32_t)s->nc || slots[0] != PP_SLOTS || slots[1] != PW_SLOTS || slots[2] != PT_SLOTS) {
+        rc = fail(DEM_E_INVALID, "incompatible CUDA contact-history checkpoint: %s", path);
+        goto done;
+    }
+    {
+        const size_t nc = (size_t)s->nc;
+        pk = malloc(8 * nc * PP_SLOTS + 8); wk = malloc(8 * nc * PW_SLOTS + 8); bk = malloc(8 * nc * PT_SLOTS + 8);
+        pv = malloc(72 * nc * PP_SLOTS + 8); wv = malloc(72 * nc * PW_SLOTS + 8); bv = malloc(72 * nc * PT_SLOTS + 8);
+        if (!read_all(f, pk, 8 * nc * PP_SLOTS) || !read_all(f, wk, 8 * nc * PW_SLOTS) ||
+            !read_all(f, bk, 8 * nc * PT_SLOTS) || !read_all(f, pv, 72 * nc * PP_SLOTS) ||
+            !read_all(f, wv, 72 * nc * PW_SLOTS) || !read_all(f, bv, 72 * nc * PT_SLOTS))
+            TRUNC();
+        char extra;
+        if (fread(&extra, 1, 1, f) == 1) {
+            rc = fail(DEM_E_INVALID, "oversized CUDA contact-history checkpoint: %s", path);
+            goto done;
+        }
+        if (v2) {
+            for (int i = 0; i < s->n; ++i) {
+                const double* q = state + (size_t)i * 23;
+                s->x[i] = q[0]; s->y[i] = q[1]; s->z[i] = q[2];
+                s->vx[i] = q[3]; s->vy[i] = q[4]; s->vz[i] = q[5];
+                s->qw[i] = q[6]; s->qx[i] = q[7]; s->qy[i] = q[8]; s->qz[i] = q[9];
+                s->wx[i] = q[10]; s->wy[i] = q[11]; s->wz[i] = q[12];
+                s->vhx[i] = q[13]; s->vhy[i] = q[14]; s->vhz[i] = q[15];
+                s->whx[i] = q[16]; s->why[i] = q[17]; s->whz[i] = q[18];
+                s->hdqw[i] = q[19]; s->hdqx[i] = q[20]; s->hdqy[i] = q[21]; s->hdqz[i] = q[22];
+            }
+            s->restored = 1;
+        }
+        memcpy(s->hpk, pk, 8 * nc * PP_SLOTS); memcpy(s->hwk, wk, 8 * nc * PW_SLOTS);
+        memcpy(s->hbk, bk, 8 * nc * PT_SLOTS); memcpy(s->hpv, pv, 72 * nc * PP_SLOTS);
+        memcpy(s->hwv, wv, 72 * nc * PW_SLOTS); memcpy(s->hbv, bv, 72 * nc * PT_SLOTS);
+        for (size_t i = 0; i < nc * PP_SLOTS; ++i) s->hpl[i] = -1;
+        for (size_t i = 0; i < nc * PW_SLOTS; ++i) s->hwl[i] = -1;
+        for (size_t i = 0; i < nc * PT_SLOTS; ++i) s->hbl[i] = -1;
+        s->stepIndex = 0;
+    }
+#undef TRUNC
+done:
+    fclose(f);
+    free(state); free(pk); free(wk); free(bk); free(pv); free(wv); free(bv);
+    return rc;
+}
+
+int ora_has_restored_leapfrog_state(const ora_solver* s, int* out)
+{
+    if (!s || !out) return fail(DEM_E_INVALID, "null argument");
+    *out = s->restored;
     return DEM_OK;
 }

@@ -56,6 +56,9 @@ int ora_debug_history(const ora_solver* s, uint64_t* pp_keys,
                       uint64_t* pt_keys, double* pt_values);
 int ora_debug_triangle_grid(const ora_solver* s, uint64_t* keys,
                             int32_t* triangles);
+int ora_save_contact_history(const ora_solver* s, const char* path);
+int ora_load_contact_history(ora_solver* s, const char* path);
+int ora_has_restored_leapfrog_state(const ora_solver* s, int* out);
 const char* ora_last_error(void);
 
 /* Geometry kernels exposed for the known-answer tests. */
