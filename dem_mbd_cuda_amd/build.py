@@ -2,6 +2,7 @@
 
   libdemhip.so    HIP kernels + C-ABI (include/dem_hip.h), gfx950
   libdemscene.so  seeded sphere-packing builder used by scenes/benchmarks
+  libdemio.so     the reference's per-frame output files (ph, F, FB, At)
 """
 from __future__ import annotations
 
@@ -41,8 +42,15 @@ def build_scene():
     return out
 
 
+def build_io():
+    src = os.path.join(HERE, "csrc", "dem_io.c")
+    out = os.path.join(HERE, "libdemio.so")
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-o", out, src])
+    return out
+
+
 def build_all():
-    return [build_demhip(), build_scene()]
+    return [build_demhip(), build_scene(), build_io()]
 
 
 if __name__ == "__main__":

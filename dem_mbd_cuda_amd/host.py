@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import math
 import os
+import re
 
 import numpy as np
 
@@ -110,6 +111,47 @@ def run_pure(solver, scene, steps=None, output_interval=None, on_frame=None, sta
         solver.save_history(history_out)
 
 
+def load_prescribed_motion(path, start_step, end_step):
+    """loadPrescribedMotion (reference cuda_simulation_integrator.cpp:180-221):
+    rows "step vx vy vz wx wy wz" ('#' comments, commas as separators; rows
+    whose first field is not an integer are skipped); steps strictly
+    increasing inside [start_step, end_step), the first one at start_step.
+    Returns [(step, linear[3], angular[3])] ([] for an empty path)."""
+    if not path:
+        return []
+    try:
+        f = open(path, encoding="utf-8", errors="replace")
+    except OSError:
+        raise RuntimeError(f"cannot open prescribed motion file: {path}")
+    keys = []
+    with f:
+        for number, line in enumerate(f, start=1):
+            line = line.split("#", 1)[0].replace(",", " ")
+            tokens = line.split()
+            if not tokens:
+                continue
+            m = re.match(r"[+-]?\d+", tokens[0])
+            if not m:
+                continue
+            rest = tokens[0][m.end():]
+            tokens = ([rest] if rest else []) + tokens[1:]
+            try:
+                vals = [float(t) for t in tokens[:6]]
+            except ValueError:
+                vals = []
+            if len(vals) < 6:
+                raise RuntimeError(f"invalid prescribed motion row {number}")
+            if len(tokens) > 6:
+                raise RuntimeError(f"extra prescribed motion column at row {number}")
+            step = int(m.group(0))
+            if (keys and step <= keys[-1][0]) or step < start_step or step >= end_step:
+                raise RuntimeError("prescribed motion steps must increase within the integration range")
+            keys.append((step, np.array(vals[:3]), np.array(vals[3:])))
+    if not keys or keys[0][0] != start_step:
+        raise RuntimeError("prescribed motion must begin at ODE_StartStep")
+    return keys
+
+
 def body_state_from_scene(scene):
     nb = len(scene.body_mass)
     pose = np.asarray(scene.body_pose, float).reshape(nb, 13)
@@ -120,11 +162,12 @@ def body_state_from_scene(scene):
 
 
 def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, start_step=0,
-                history_in=None, history_out=None):
-    """IntegrateDemMultiBody (reference cuda_simulation_integrator.cpp:288-446),
-    without the prescribed-motion schedule file.  on_frame(step, time, bodies,
-    loads, impulses) gets the body state and the loads of that frame.
-    history_in / history_out as for run_pure."""
+                history_in=None, history_out=None, prescribed=None):
+    """IntegrateDemMultiBody (reference cuda_simulation_integrator.cpp:288-446).
+    on_frame(step, time, extra): extra holds the body state, the loads and
+    the impulses of that frame.  history_in / history_out as for run_pure;
+    prescribed: a load_prescribed_motion schedule (applyPrescribedMotion,
+    :223-243, at the start of every step)."""
     if history_in is not None and os.path.exists(history_in):
         solver.load_history(history_in)
     dt = scene.dt
@@ -177,7 +220,19 @@ def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, 
 
     half_update(True)
     counter = 0
+    cursor = 0
     for step in range(start_step, end):
+        if prescribed:
+            while cursor + 1 < len(prescribed) and prescribed[cursor + 1][0] <= step:
+                cursor += 1
+            if nb != 1 or B["state"][0] != BODY_PRESCRIBED:
+                raise RuntimeError("prescribed motion schedule currently requires one prescribed body")
+            B["vel"][0] = prescribed[cursor][1]
+            B["angvel"][0] = prescribed[cursor][2]
+            hv[0] = B["vel"][0]
+            hw[0] = B["angvel"][0]
+            hq[0] = qnormalize(B["orien"][0] + gama(B["orien"][0], B["angvel"][0]) * half)
+            hdq[0] = gama(hq[0], hw[0])
         current = step * dt
         next_time = (step + 1) * dt
         solver.advance(dt)

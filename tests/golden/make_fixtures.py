@@ -15,6 +15,13 @@ Outputs (committed):
                       DATA1 cases (particle state + force files, body state)
   data1_zero_vel_ref_cpu.npz  the initial (frame 0) outputs of the same binary
                       on DATA1 with all particle velocities set to zero
+  ref_output_text.npz the text of the reference binary's frame files on DATA1
+                      (coupled): ph, F, FB, At of frames 0 and 5, the first 40
+                      lines of each (byte-level pin of the output writers)
+  cases/<case>/InputFile/...  the input files of the reference's validation
+                      cases (validation/cases), as the reference holds them
+  ref_case_outputs.npz every OutputFile/ file the reference binary writes for
+                      those cases (text), keyed "<case>/<path under OutputFile>"
 """
 from __future__ import annotations
 
@@ -37,9 +44,10 @@ from dem_mbd_cuda_amd import scene as S  # noqa: E402
 CASES = ["free_fall", "wall_contact", "two_particle", "triangle_contact", "body_states"]
 
 
-def run_reference_cpu(case_dir, work, zero_velocity=False, tag=""):
-    """Runs oracle/_ref/dem_mbd_cpu on a copy of case_dir with all outputs on
-    (optionally with every particle velocity and angular velocity zeroed)."""
+def run_reference_cpu(case_dir, work, zero_velocity=False, tag="", all_outputs=True):
+    """Runs oracle/_ref/dem_mbd_cpu on a copy of case_dir, with all outputs on
+    unless all_outputs is False (optionally with every particle velocity and
+    angular velocity zeroed)."""
     dst = os.path.join(work, os.path.basename(case_dir) + tag)
     shutil.copytree(case_dir, dst)
     if zero_velocity:
@@ -55,7 +63,7 @@ def run_reference_cpu(case_dir, work, zero_velocity=False, tag=""):
     out = []
     for ln in lines:
         key = ln.split("=")[0].strip()
-        if key.startswith("CONTROL_OUTPUT_") and key != "CONTROL_OUTPUT_SUMMARY":
+        if all_outputs and key.startswith("CONTROL_OUTPUT_") and key != "CONTROL_OUTPUT_SUMMARY":
             ln = f"{key} = 1"
         out.append(ln)
     open(par, "w").write("\n".join(out) + "\n")
@@ -98,6 +106,35 @@ def main():
                 if k.endswith("_0"):
                     zref[f"{tag}__{k}"] = v
     np.savez_compressed(os.path.join(HERE, "data1_zero_vel_ref_cpu.npz"), **zref)
+    text = {}
+    with tempfile.TemporaryDirectory() as work:
+        out = run_reference_cpu(os.path.join(REF, "validation", "ctest_data1_mbd"), work)
+        for sub, name in (("state_particles", "ph.%08d.bt"), ("force_particles", "F.%08d.bt"),
+                          ("force_bodys", "FB.%05d.bt"), ("state_bodys", "At.%08d.bt")):
+            for idx in (0, 5):
+                f = os.path.join(out, "OutputFile", sub, name % idx)
+                lines = open(f).read().splitlines(keepends=True)[:40]
+                text[name.split(".")[0] + f"_{idx}"] = np.array("".join(lines))
+    np.savez_compressed(os.path.join(HERE, "ref_output_text.npz"), **text)
+    outs = {}
+    with tempfile.TemporaryDirectory() as work:
+        for name in CASES:
+            src = os.path.join(REF, "validation", "cases", name)
+            dst = os.path.join(HERE, "cases", name)
+            if os.path.exists(dst):
+                shutil.rmtree(dst)
+            shutil.copytree(os.path.join(src, "InputFile"), os.path.join(dst, "InputFile"))
+            for root, dirs, files in os.walk(dst):
+                for f in files:
+                    os.chmod(os.path.join(root, f), 0o644)
+                os.chmod(root, 0o755)
+            out = run_reference_cpu(src, work, all_outputs=False)
+            base = os.path.join(out, "OutputFile")
+            for root, _, files in os.walk(base):
+                for f in sorted(files):
+                    rel = os.path.relpath(os.path.join(root, f), base)
+                    outs[f"{name}/{rel}"] = np.array(open(os.path.join(root, f)).read())
+    np.savez_compressed(os.path.join(HERE, "ref_case_outputs.npz"), **outs)
     print("fixtures written to", HERE)
 
 
