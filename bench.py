@@ -47,7 +47,9 @@ def run_steps(s, dt, steps):
 def measure_sweep(dm, scene, steps, warmup, seq_steps, count, seed):
     """Parameter sweep of `count` independent box scenes: one batched solver
     against the same scenes run one after another in single-scene solvers
-    (same GPU, same step loop).  Returns ms per step of each and the ratio."""
+    on the same GPU.  Both sides run their steps through dem_run_steps
+    (captured HIP graphs); the sequential side is also timed with eager
+    launches.  Returns ms per step of each and the ratios."""
     import time as _t
 
     scenes = scene.sweep_scenes(count=count, seed=seed)
@@ -57,29 +59,31 @@ def measure_sweep(dm, scene, steps, warmup, seq_steps, count, seed):
     s.initialize_batch(batch)
     s.compute_forces(collect=True)
     s.init_half_step(0.5 * dt)
-    run_steps(s, dt, warmup)
+    s.run_steps(warmup, dt)
     s.synchronize()
     t0 = _t.perf_counter()
-    run_steps(s, dt, steps)
+    s.run_steps(steps, dt)
     s.synchronize()
     batched = (_t.perf_counter() - t0) / steps
     contacts = s.stats()["particle_contacts"]
     s.close()
-    seq = 0.0
+    seq = {True: 0.0, False: 0.0}
     for sc in scenes:
-        q = dm.CudaDemSolver()
-        q.initialize(sc)
-        q.compute_forces(collect=True)
-        q.init_half_step(0.5 * dt)
-        run_steps(q, dt, 3)
-        q.synchronize()
-        t0 = _t.perf_counter()
-        run_steps(q, dt, seq_steps)
-        q.synchronize()
-        seq += (_t.perf_counter() - t0) / seq_steps
-        q.close()
+        for graph in (True, False):
+            q = dm.CudaDemSolver()
+            q.initialize(sc)
+            q.compute_forces(collect=True)
+            q.init_half_step(0.5 * dt)
+            q.run_steps(4, dt, graph=graph)
+            q.synchronize()
+            t0 = _t.perf_counter()
+            q.run_steps(seq_steps, dt, graph=graph)
+            q.synchronize()
+            seq[graph] += (_t.perf_counter() - t0) / seq_steps
+            q.close()
     return {"scenes": count, "particles": batch.n, "batched_ms_per_step": 1e3 * batched,
-            "sequential_ms_per_step": 1e3 * seq, "speedup": seq / batched,
+            "sequential_ms_per_step": 1e3 * seq[True], "speedup": seq[True] / batched,
+            "sequential_eager_ms_per_step": 1e3 * seq[False], "speedup_vs_eager": seq[False] / batched,
             "batched_particle_steps_per_s": batch.n / batched, "contacts": contacts,
             "steps": steps, "sequential_steps_per_scene": seq_steps,
             "workload": f"{count} boxes, N ~ U[500, 20000] spheres r ~ U[1, 1.5] mm, seeded RSA, "
@@ -131,7 +135,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=60,
+                    help="untimed steps; the first ones autotune the reorder interval and contact grid")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -167,17 +172,13 @@ def main():
     s.initialize(sc)
     s.compute_forces(collect=True)
     s.init_half_step(0.5 * dt)
-    for _ in range(args.warmup):
-        s.advance(dt)
-        s.compute_forces(collect=False)
-        s.finish(dt)
+    # steps run as dem_run_steps: (advance, computeForces, finish) replayed from
+    # captured HIP graphs; the warmup autotunes the reorder interval and grid
+    s.run_steps(args.warmup, dt, graph=True, autotune=True)
     s.synchronize()
     print(f"# warmup done", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
-    for _ in range(args.steps):
-        s.advance(dt)
-        s.compute_forces(collect=False)
-        s.finish(dt)
+    s.run_steps(args.steps, dt, graph=True)
     s.synchronize()
     t2 = time.perf_counter()
     elapsed = t2 - t1

@@ -146,6 +146,8 @@ class Library:
             bind("device_description", [], C.c_char_p)
         if hasattr(L, p + "synchronize"):
             bind("synchronize", [vp])
+        if hasattr(L, p + "run_steps"):
+            bind("run_steps", [vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)])
         if hasattr(L, p + "diff_forward"):
             bind("diff_forward", [vp, C.c_int, C.c_int, C.c_int, _dp, _dp])
             bind("diff_backward", [vp, _dp, _dp, _dp])
@@ -353,6 +355,21 @@ class Solver:
     def synchronize(self):
         if self.lib.has("synchronize"):
             self.lib.call("synchronize", self.h)
+
+    def run_steps(self, steps, dt, graph=True, autotune=False):
+        """`steps` x (advance, compute_forces without diagnostics, finish) on the
+        device (HIP graphs unless graph=False).  Libraries without the entry
+        point run the three calls per step."""
+        if not self.lib.has("run_steps"):
+            for _ in range(steps):
+                self.advance(dt)
+                self.compute_forces(collect=False)
+                self.finish(dt)
+            return steps
+        done = C.c_int(0)
+        flags = (0 if graph else 1) | (2 if autotune else 0)
+        self.lib.call("run_steps", self.h, int(steps), float(dt), flags, C.byref(done))
+        return done.value
 
     def diff_forward(self, steps, checkpoint_interval, plate_wall, scenes=1):
         """Differentiable run; returns per-scene (bed height, plate load)."""

@@ -73,7 +73,7 @@ constexpr int MAX_LIST = PP_SLOTS;  // contact-list capacity per component
 constexpr int SLOT_BITS = 6;        // list entry = partner row << SLOT_BITS | slot
 static_assert((1 << SLOT_BITS) == PP_SLOTS, "slot field width");
 constexpr long MAX_PARTICLES = (1L << (31 - SLOT_BITS)) - 1;
-constexpr int PAIR_BLOCKS = 8192;   // grid of the per-contact law (grid-stride)
+constexpr int PAIR_BLOCKS = 8192;   // default grid of the per-contact law (grid-stride)
 constexpr int BLOCK = 256;
 
 // Double fields of the particle SoA (one array of n doubles each).
@@ -1634,6 +1634,8 @@ int blocks_for(long n) { return (int)((n + BLOCK - 1) / BLOCK); }
 // ============================================================ solver object
 struct DiffRun;                 // differentiable-run state (dem_diff.inl)
 void diff_release(DiffRun* r);
+struct GraphCache;              // captured step sequences (dem_graph.inl)
+void graph_release(GraphCache* g);
 // timing events: grid build | contact scan + list compaction | pair law |
 // owner sums (triangles, walls, field, acceleration) | body reductions
 constexpr int NEV = 6;
@@ -1650,7 +1652,10 @@ struct dem_solver {
     double g[3] = {0, 0, 0};
     int gravity = 0, universal = 0;
     int stepIndex = 0;
-    int reorderInterval = 1;
+    int reorderInterval = 8;  // steps between physical reorders (DEM_REORDER_INTERVAL)
+    int pairBlocks = PAIR_BLOCKS;
+    bool capturing = false;   // inside a HIP graph capture: no event records
+    GraphCache* graphs = nullptr;
     uint64_t intervalSteps = 0;
     size_t freeBefore = 0;
     bool restored = false;
@@ -1745,6 +1750,7 @@ struct dem_solver {
     {
         if (stream) hipStreamSynchronize(stream);
         diff_release(diff);
+        graph_release(graphs);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
                         rec, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
@@ -2255,7 +2261,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             TRY(launch_check("transform triangles"));
         }
     }
-    HIP_TRY(hipEventRecord(s->ev[0], st), "event");
+    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[0], st), "event");
     hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
     Soa S = s->soa();
     hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dSp, s->offs, s->sc,
@@ -2284,7 +2290,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     hipLaunchKernelGGL(k_pack, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->offs, s->rec);
     TRY(launch_check("grid build"));
     if (s->nt > 0) TRY(build_triangle_grid(s));
-    HIP_TRY(hipEventRecord(s->ev[1], st), "event");
+    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
     const int step = s->stepIndex++;
     const int nblk = blocks_for(n);
     hipLaunchKernelGGL(k_scan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->cidS, s->keyA,
@@ -2294,11 +2300,11 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk, s->nbr,
                        s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
-    HIP_TRY(hipEventRecord(s->ev[2], st), "event");
-    hipLaunchKernelGGL(k_pairs, dim3(PAIR_BLOCKS), dim3(BLOCK), 0, st, S, s->valA, s->rec, s->pairs, s->H,
+    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[2], st), "event");
+    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, S, s->valA, s->rec, s->pairs, s->H,
                        s->dMatPP, step == 0, s->offs, s->sc, s->pairF, s->tape.pair, s->tape.pairPrev);
     TRY(launch_check("particle contact law"));
-    HIP_TRY(hipEventRecord(s->ev[3], st), "event");
+    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
     A.S = S; A.rec = s->rec; A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
     A.pairF = s->pairF; A.H = s->H;
@@ -2323,7 +2329,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     }
     hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     TRY(launch_check("contact law"));
-    HIP_TRY(hipEventRecord(s->ev[4], st), "event");
+    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[4], st), "event");
     const bool interval = A.pt.dt > 0.0;
     if (nb > 0) {
         hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, S, s->valA, s->H,
@@ -2350,7 +2356,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         ++s->intervalSteps;
         if (!interval) --s->intervalSteps;
     }
-    HIP_TRY(hipEventRecord(s->ev[5], st), "event");
+    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[5], st), "event");
     s->evReady = true;
     HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
     const bool needLoads = loads6 && nb > 0;
@@ -2900,3 +2906,4 @@ int dem_has_restored_leapfrog_state(const dem_solver* s, int* out)
 }  // extern "C"
 
 #include "dem_diff.inl"
+#include "dem_graph.inl"

@@ -249,6 +249,21 @@ int dem_synchronize(const dem_solver* solver);
 int dem_debug_last_timings(const dem_solver* solver, double* ms_out,
                            int capacity, int* count);
 
+/* ---- multi-step runs (no reference counterpart) --------------------------
+ * Runs `steps` iterations of the reference's pure-DEM loop body
+ * (advanceParticles, computeForces without diagnostics, finishParticleStep;
+ * cuda_simulation_integrator.cpp:263-269), identical in every result to
+ * calling those three entry points in turn.  Without bodies the steps are
+ * captured into HIP graphs and replayed (DEM_RUN_NO_GRAPH: launch eagerly);
+ * DEM_RUN_AUTOTUNE times the reorder interval and the per-contact grid on
+ * the run's first steps (once per solver) and keeps the fastest.  The last
+ * step's finish is left pending, as after dem_finish_particle_step.
+ * *steps_done (may be NULL) receives the steps run. */
+#define DEM_RUN_NO_GRAPH 1
+#define DEM_RUN_AUTOTUNE 2
+int dem_run_steps(dem_solver* solver, int steps, double dt, int flags,
+                  int* steps_done);
+
 /* ---- differentiable runs (no reference counterpart) ---------------------
  * Reverse-mode derivatives of scene observables with respect to each
  * scene's contact laws, for a solver freshly initialised (dem_initialize or
