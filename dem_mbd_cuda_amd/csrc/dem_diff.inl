@@ -488,6 +488,7 @@ int restore_checkpoint(dem_solver* s, const Checkpoint& c)
     }
     s->stepIndex = c.step;
     s->pendingFinish = false;
+    s->vlistValid = false;
     return DEM_OK;
 }
 
@@ -551,6 +552,9 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
     R.K = checkpoint_interval;
     R.plate = plate_wall;
     s->reorderInterval = 0;  // rows stay put: row = input index in every tape
+    s->verlet = false;       // (candidate-list rebuilds reorder rows)
+    s->countAlways = true;   // checkpoints size history copies by the high-water marks
+    s->vlistValid = false;
     const long n = s->n;
     const double dt = s->pp.dt;
     TRY(dalloc(R.dPP, steps + 1, "hipMalloc(run counts)"));

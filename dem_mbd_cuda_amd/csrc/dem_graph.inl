@@ -12,9 +12,11 @@
 // s0' = s0 (mod P).  The first force evaluation (dt = 0) and runs with bodies
 // (their triangle grid needs a host-side count) stay eager.
 //
-// The optional autotuner times the candidate reorder intervals and
-// per-contact grid sizes on the run's own steps (both leave every result
-// bit-identical) and keeps the fastest.
+// Steps with candidate lists read a displacement flag back every step and
+// run eagerly.  The optional autotuner times candidate lists against the full
+// scan with several reorder intervals, each with several per-contact grid
+// sizes, on the run's own steps (every choice leaves every result
+// bit-identical), and keeps the fastest.
 
 struct GraphCache {
     struct Entry {
@@ -78,6 +80,53 @@ int timed_steps(dem_solver* s, double dt, int steps, float* ms)
 
 }  // namespace
 
+namespace {
+
+// Runs `count` steps: graph replays when the solver is in full-scan mode
+// without bodies, eager launches otherwise.  Returns the steps run in *done.
+int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
+{
+    GraphCache& G = *s->graphs;
+    int k = 0;
+    auto step1 = [&]() {
+        TRY(eager_step(s, dt));
+        ++k;
+        return DEM_OK;
+    };
+    // the first evaluation after initialisation or a load runs eagerly; a
+    // captured sequence starts with a finish pending (its first kernel is the
+    // fused finish + advance), as every replay does
+    while (k < count && (s->stepIndex == 0 || !s->pendingFinish)) TRY(step1());
+    const bool graphs = allowGraphs && s->nb == 0 && !s->verlet;
+    if (graphs) {
+        const int P = graph_period(s);
+        while (count - k >= P) {
+            const int phase = s->stepIndex % P;
+            GraphCache::Entry* hit = nullptr;
+            for (auto& e : G.entries)
+                if (e.phase == phase && e.cur == s->cur && e.period == P && e.pairBlocks == s->pairBlocks &&
+                    e.reorder == s->reorderInterval)
+                    hit = &e;
+            if (hit) {
+                // the captured steps reorder an even number of times: the buffer is unchanged
+                HIP_TRY(hipGraphLaunch(hit->exec, s->stream), "launch step graph");
+                s->stepIndex += P;
+            } else {
+                GraphCache::Entry e{phase, s->cur, P, s->pairBlocks, s->reorderInterval, nullptr};
+                TRY(capture_steps(s, dt, P, &e.exec));
+                HIP_TRY(hipGraphLaunch(e.exec, s->stream), "launch step graph");
+                G.entries.push_back(e);
+            }
+            k += P;
+        }
+    }
+    while (k < count) TRY(step1());
+    *done += k;
+    return DEM_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int dem_run_steps(dem_solver* s, int steps, double dt, int flags, int* steps_done)
@@ -87,62 +136,51 @@ int dem_run_steps(dem_solver* s, int steps, double dt, int flags, int* steps_don
     if (steps_done) *steps_done = 0;
     if (!s->graphs) s->graphs = new GraphCache();
     GraphCache& G = *s->graphs;
+    const bool allowGraphs = (flags & DEM_RUN_NO_GRAPH) == 0;
     int done = 0;
-    auto step1 = [&]() {
-        TRY(eager_step(s, dt));
-        ++done;
-        return DEM_OK;
-    };
-    // the first evaluation after initialisation or a load runs eagerly, as do bodies
-    const bool graphs = (flags & DEM_RUN_NO_GRAPH) == 0 && s->nb == 0;
-    while (done < steps && (s->stepIndex == 0 || !s->pendingFinish)) TRY(step1());
-    if ((flags & DEM_RUN_AUTOTUNE) && !G.tuned && steps - done >= 48) {
-        const int intervals[4] = {1, 2, 4, 8};
+    if ((flags & DEM_RUN_AUTOTUNE) && !G.tuned && steps >= 96) {
+        // candidates: candidate lists (when available) or the full scan with a
+        // reorder interval, each with three contact-grid sizes; each timed on
+        // a few steps of the run itself, graphs included
+        struct Cfg { bool verlet; int reorder, grid; };
+        std::vector<Cfg> cfgs;
         const int grids[3] = {2048, 8192, 32768};
-        float best = 1e30f;
-        int bi = s->reorderInterval, bg = s->pairBlocks;
-        for (int a = 0; a < 4; ++a)
-            for (int b = 0; b < 3; ++b) {
-                s->reorderInterval = intervals[a];
-                s->pairBlocks = grids[b];
-                float ms = 0.0f;
-                TRY(timed_steps(s, dt, 2, &ms));
-                done += 3;
-                if (ms < best) { best = ms; bi = intervals[a]; bg = grids[b]; }
-            }
-        s->reorderInterval = bi;
-        s->pairBlocks = bg;
-        G.tuned = true;
-    }
-    if (!graphs) {
-        while (done < steps) TRY(step1());
-        if (steps_done) *steps_done = done;
-        return DEM_OK;
-    }
-    // a captured sequence starts with a finish pending (its first kernel is the
-    // fused finish + advance), as every replay does
-    while (done < steps && !s->pendingFinish) TRY(step1());
-    const int P = graph_period(s);
-    while (steps - done >= P) {
-        const int phase = s->stepIndex % P;
-        GraphCache::Entry* hit = nullptr;
-        for (auto& e : G.entries)
-            if (e.phase == phase && e.cur == s->cur && e.period == P && e.pairBlocks == s->pairBlocks &&
-                e.reorder == s->reorderInterval)
-                hit = &e;
-        if (hit) {
-            // the captured steps reorder an even number of times: the buffer is unchanged
-            HIP_TRY(hipGraphLaunch(hit->exec, s->stream), "launch step graph");
-            s->stepIndex += P;
-        } else {
-            GraphCache::Entry e{phase, s->cur, P, s->pairBlocks, s->reorderInterval, nullptr};
-            TRY(capture_steps(s, dt, P, &e.exec));
-            HIP_TRY(hipGraphLaunch(e.exec, s->stream), "launch step graph");
-            G.entries.push_back(e);
+        const bool canLists = s->refC != nullptr && s->nb == 0 && !s->verletBroken;
+        for (int g : grids) {
+            if (canLists) cfgs.push_back({true, s->reorderInterval, g});
+            for (int r : {1, 4, 8}) cfgs.push_back({false, r, g});
         }
-        done += P;
+        const int span = 4;
+        if (steps >= (int)cfgs.size() * (span + 4) + 8) {
+            float best = 1e30f;
+            Cfg pick{s->verlet, s->reorderInterval, s->pairBlocks};
+            for (const Cfg& c : cfgs) {
+                if (c.verlet != s->verlet) s->vlistValid = false;
+                s->verlet = c.verlet && canLists;
+                s->reorderInterval = c.reorder;
+                s->pairBlocks = c.grid;
+                TRY(run_span(s, dt, 4, allowGraphs, &done));  // captures graphs, settles the mode
+                TRY(flush_finish(s));
+                HIP_TRY(hipEventRecord(s->ev[0], s->stream), "event");
+                TRY(run_span(s, dt, span, allowGraphs, &done));
+                TRY(flush_finish(s));
+                HIP_TRY(hipEventRecord(s->ev[5], s->stream), "event");
+                HIP_TRY(hipEventSynchronize(s->ev[5]), "event");
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[5]), "event");
+                if (s->verlet == c.verlet && ms < best) {
+                    best = ms;
+                    pick = c;
+                }
+            }
+            if (pick.verlet != s->verlet) s->vlistValid = false;
+            s->verlet = pick.verlet && canLists && !s->verletBroken;
+            s->reorderInterval = pick.reorder;
+            s->pairBlocks = pick.grid;
+            G.tuned = true;
+        }
     }
-    while (done < steps) TRY(step1());
+    TRY(run_span(s, dt, steps - done, allowGraphs, &done));
     if (steps_done) *steps_done = done;
     return DEM_OK;
 }

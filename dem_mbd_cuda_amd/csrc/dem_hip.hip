@@ -86,7 +86,10 @@ enum DField {
     NDF
 };
 // Int fields.
-enum IField { I_PID, I_STATUS, I_CID, I_ORIG, I_CX, I_CY, I_CZ, I_VALID, I_SCENE, NIF };
+// I_ORIG: input row (output order); I_HIST: the particle's row in the contact
+// history tables (the input row, or the row of the last candidate-list
+// rebuild, when the history moves with the particles).
+enum IField { I_PID, I_STATUS, I_CID, I_ORIG, I_CX, I_CY, I_CZ, I_VALID, I_SCENE, I_HIST, NIF };
 
 struct Soa {
     double* d;  // [NDF][n]
@@ -129,6 +132,8 @@ struct Scalars {
     int tri_overflow;
     int pad;
     long long total_pairs;
+    int need_rebuild;   // some component moved more than skin/2 since the last rebuild
+    int cand_overflow;  // a rebuild listed more partners than CAND_MAX
 };
 
 struct History {
@@ -255,6 +260,8 @@ __global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc,
     if (threadIdx.x == 0) {
         sc->grid_overflow = 0;
         sc->list_overflow = 0;
+        sc->need_rebuild = 0;
+        sc->cand_overflow = 0;
         sc->pp_high = 0;
         sc->body_high = 0;
         sc->particle_contacts = 0;
@@ -270,11 +277,13 @@ __global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc,
 // one scene; a block that straddles scenes adds its lanes to the scenes'
 // accumulators with atomics.  k_grid_setup folds both.
 __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, bool offs, Scalars* sc,
-                                                 int* part, int* partScene, int* sceneAcc)
+                                                 int* part, int* partScene, int* sceneAcc, const double* refC,
+                                                 double halfSkin2)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
     int scene = -1;
+    int moved = 0;
     if (p < S.n) {
         scene = S.k(I_SCENE, p);
         const double inv = sp[scene].inv;
@@ -285,6 +294,10 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
             S.f(F_OFX, p) = o.x; S.f(F_OFY, p) = o.y; S.f(F_OFZ, p) = o.z;
             cx = cx + o.x; cy = cy + o.y; cz = cz + o.z;
             S.f(F_CPX, p) = cx; S.f(F_CPY, p) = cy; S.f(F_CPZ, p) = cz;
+        }
+        if (refC) {
+            const double ex = cx - refC[p], ey = cy - refC[S.n + p], ez = cz - refC[2 * S.n + p];
+            moved = ex * ex + ey * ey + ez * ez > halfSkin2 ? 1 : 0;
         }
         const double fx = floor(cx * inv), fy = floor(cy * inv), fz = floor(cz * inv);
         const double minCell = -(double)(1 << 20), maxCell = (double)((1 << 20) - 1);
@@ -300,6 +313,7 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
         }
         S.k(I_VALID, p) = ok ? 1 : 0;
     }
+    if (refC) atomic_max_wave(&sc->need_rebuild, moved);
     __shared__ int red[BLOCK / 64][8];
     const int wave = threadIdx.x >> 6;
     {
@@ -687,7 +701,7 @@ __global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Rec* R)
 // Contact centre, radius and component id in sorted order (what the
 // candidate scan reads per candidate).
 __global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool offs, double4* posr,
-                                                uint32_t* cidS)
+                                                uint32_t* cidS, int4* meta)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= S.n) return;
@@ -698,6 +712,22 @@ __global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool o
     v.w = S.f(F_CR, p);
     posr[t] = v;
     cidS[t] = (uint32_t)S.k(I_CID, p);
+    // cell and input row for the candidate-list test (one 16-byte load per
+    // partner); a component outside the hash range gets a cell no neighbour
+    // is adjacent to
+    if (meta)
+        meta[t] = S.k(I_VALID, p) ? make_int4(S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p), S.k(I_ORIG, p))
+                                  : make_int4(INT_MIN / 2, INT_MIN / 2, INT_MIN / 2, S.k(I_ORIG, p));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  This maps the hardware block index to a logical one so that every XCD
+// works on one contiguous eighth of the range (neighbouring particles, and
+// their neighbours' data, stay in one L2).
+__device__ __forceinline__ long xcd_block(long b, long nb)
+{
+    const long q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
+    return x * q + (x < r ? x : r) + i;
 }
 
 // History slot acquisition (acquireHistorySlot, contact:190-215) on an
@@ -770,6 +800,7 @@ struct LawArgs {
     bool offs;
     int nb;
     Tape tape;
+    bool counters;  // per-step statistics (contacts, high-water marks) wanted
 };
 
 // History lookup with the first 16 keys of the owner held in registers (kept
@@ -817,6 +848,44 @@ __device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long
     return b;
 }
 
+// History expiry at the start of the owner's step (expireHistory,
+// contact:217-229) and one slot per contact in list order
+// (acquireHistorySlot, contact:190-215).  nbr[c][t] holds the sorted index of
+// the c-th contact on entry and (partner row << SLOT_BITS | slot) on return,
+// contacts without a slot (overflow) dropped; returns the entries kept.
+__device__ __forceinline__ int assign_slots(const Soa& S, long t, long p, int found, const int* sorted,
+                                            const uint32_t* cidS, const History& H, int step, int* nbr,
+                                            unsigned long long* occTape, int& high, int& over)
+{
+    const long n = S.n;
+    const long o = S.k(I_HIST, p);
+    unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
+    if (occTape) occTape[o] = occ;
+    unsigned long long cur = 0;
+    uint32_t* keys = H.ppKey + o * PP_SLOTS;
+    double* vals = H.ppVal + o * PP_SLOTS * 9;
+    uint32_t k16[16];
+    {
+        const uint4* kv = reinterpret_cast<const uint4*>(keys);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint4 v = kv[i];
+            k16[4 * i] = v.x; k16[4 * i + 1] = v.y; k16[4 * i + 2] = v.z; k16[4 * i + 3] = v.w;
+        }
+    }
+    int stored = 0;
+    for (int c = 0; c < found; ++c) {
+        const long k = nbr[(long)c * n + t];
+        const int slot = acquire_pp(occ, cur, k16, keys, vals, cidS[k] + 1u, high, over);
+        if (slot < 0) continue;  // slot overflow: counted, no force
+        nbr[(long)stored * n + t] = (sorted[k] << SLOT_BITS) | slot;
+        ++stored;
+    }
+    H.ppOcc[o] = occ;
+    H.ppSeen[(long)(step & 1) * n + o] = cur;
+    return stored;
+}
+
 // 27-cell candidate scan (particleContact's overlap test, contact:579-591, in
 // the reference's cell and entry order: the nine (dx, dy) columns, each one
 // contiguous run of the sorted array over dz = -1..1) followed by the
@@ -830,7 +899,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
                                                 const uint32_t* cidS, const unsigned long long* key,
                                                 const unsigned* start, const SceneGrid* grid, Scalars* sc,
                                                 History H, int step, int* nbr, int* ncount,
-                                                unsigned long long* occTape)
+                                                unsigned long long* occTape, bool counters)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
@@ -877,34 +946,15 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
             atomicAdd(&sc->list_overflow, 1);
             found = MAX_LIST;
         }
-        const long o = S.k(I_ORIG, p);
-        unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
-        if (occTape) occTape[o] = occ;
-        unsigned long long cur = 0;
-        uint32_t* keys = H.ppKey + o * PP_SLOTS;
-        double* vals = H.ppVal + o * PP_SLOTS * 9;
-        uint32_t k16[16];
-        {
-            const uint4* kv = reinterpret_cast<const uint4*>(keys);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint4 v = kv[i];
-                k16[4 * i] = v.x; k16[4 * i + 1] = v.y; k16[4 * i + 2] = v.z; k16[4 * i + 3] = v.w;
-            }
-        }
-        for (int c = 0; c < found; ++c) {
-            const long k = nbr[(long)c * n + t];
-            const int slot = acquire_pp(occ, cur, k16, keys, vals, cidS[k] + 1u, high, over);
-            if (slot < 0) continue;  // slot overflow: counted, no force
-            nbr[(long)stored * n + t] = (sorted[k] << SLOT_BITS) | slot;
-            ++stored;
-        }
-        H.ppOcc[o] = occ;
-        H.ppSeen[(long)(step & 1) * n + o] = cur;
+        stored = assign_slots(S, t, p, found, sorted, cidS, H, step, nbr, occTape, high, over);
         ncount[t] = stored;
     }
-    atomic_add_wave(&sc->particle_contacts, stored);
-    atomic_max_wave(&sc->pp_high, high);
+    // per-step statistics only when they are read; overflow counts are
+    // cumulative and (almost always) zero
+    if (counters) {
+        atomic_add_wave(&sc->particle_contacts, stored);
+        atomic_max_wave(&sc->pp_high, high);
+    }
     atomic_add_wave(&sc->pp_overflow, over);
 }
 
@@ -967,7 +1017,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const
         const Body J = rec_body(rec, q);
         const V3 delta = sub(J.pos, I.pos);
         const double d2 = dot(delta, delta);
-        const long o = S.k(I_ORIG, p);
+        const long o = S.k(I_HIST, p);
         Material m = mats[rec[p].scene];
         if (first) m.dt = 0.0;  // the first evaluation advances no history
         double* hv = H.ppVal + (o * PP_SLOTS + slot) * 9;
@@ -982,6 +1032,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const
         out[3] = tq.x; out[4] = tq.y; out[5] = tq.z;
     }
 }
+
+#include "dem_verlet.inl"
 
 __device__ __forceinline__ Tri load_tri(const TriSet& T, int t)
 {
@@ -1022,13 +1074,13 @@ __device__ __forceinline__ long upper_bound_u64(const unsigned long long* k, lon
 // acceleration (particle:71-101).
 __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
 {
-    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     const Soa& S = A.S;
     const long n = S.n;
     int bodyHigh = 0, pwOver = 0;
     if (t < n) {
         const long p = A.sorted[t];
-        const long o = S.k(I_ORIG, p);
+        const long o = S.k(I_HIST, p);  // history row
         const int s = A.step;
         Body I = rec_body(A.rec, p);
         if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
@@ -1122,7 +1174,7 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
         S.f(F_AWY, p) = (ty - (ix - iz) * wx * wz) / iy;
         S.f(F_AWZ, p) = (tz - (iy - ix) * wy * wx) / iz;
     }
-    atomic_max_wave(&A.sc->body_high, bodyHigh);
+    if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
 }
 
@@ -1139,6 +1191,7 @@ __global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
     if (t < n) {
         const long p = A.sorted[t];
         const long o = S.k(I_ORIG, p);
+        const long hr = S.k(I_HIST, p);  // history row
         const int s = A.step;
         const History& H = A.H;
         Body I = rec_body(A.rec, p);
@@ -1146,7 +1199,7 @@ __global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
         V3 bF = zero3(), bT = zero3();
         // triangles of the own cell (contact:1148-1208)
         {
-            unsigned long long ptOcc = H.ptOcc[o] & H.ptSeen[(long)((s + 1) & 1) * n + o];
+            unsigned long long ptOcc = H.ptOcc[hr] & H.ptSeen[(long)((s + 1) & 1) * n + hr];
             unsigned long long ptCur = 0;
             const TriSet& T = A.T;
             if (T.nt > 0 && T.entries > 0 && S.k(I_VALID, p)) {
@@ -1175,11 +1228,11 @@ __global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
                         const V3 mN = unit(wN);
                         const V3 mC = scl(wC, 1.0 / total);
                         const double mD = wD / total;
-                        const int slot = acquire(ptOcc, ptCur, H.ptKey + o * PT_SLOTS,
-                                                 H.ptVal + o * PT_SLOTS * 9, PT_SLOTS,
+                        const int slot = acquire(ptOcc, ptCur, H.ptKey + hr * PT_SLOTS,
+                                                 H.ptVal + hr * PT_SLOTS * 9, PT_SLOTS,
                                                  (uint32_t)(body + 1), bodyHigh, ptOver);
                         if (slot >= 0) {
-                            double* sv = H.ptVal + (o * PT_SLOTS + slot) * 9;
+                            double* sv = H.ptVal + (hr * PT_SLOTS + slot) * 9;
                             const Pose P = load_pose(T, body);
                             bool slide = false, twisty = false;
                             manifold_law(I, P, A.pt, mN, mC, mD, sv, &slide, &twisty);
@@ -1232,8 +1285,8 @@ __global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
                     A.O.fzPart[(long)body * n + t] = 0.0;
                 }
             }
-            H.ptOcc[o] = ptOcc;
-            H.ptSeen[(long)(s & 1) * n + o] = ptCur;
+            H.ptOcc[hr] = ptOcc;
+            H.ptSeen[(long)(s & 1) * n + hr] = ptCur;
         }
         double* ob = A.O.body;
         ob[0 * n + p] = bF.x; ob[1 * n + p] = bF.y; ob[2 * n + p] = bF.z;
@@ -1279,7 +1332,7 @@ __global__ void __launch_bounds__(BLOCK) k_observe(Soa S, const int* sorted, His
         wD += distance * scv;
     }
     if (total <= 0.0) return;
-    const long o = S.k(I_ORIG, p);
+    const long o = S.k(I_HIST, p);  // history row
     const unsigned long long seen = H.ptSeen[(long)(step & 1) * n + o];
     int slot = -1;
     for (int b = 0; b < PT_SLOTS; ++b) {
@@ -1654,7 +1707,21 @@ struct dem_solver {
     int stepIndex = 0;
     int reorderInterval = 8;  // steps between physical reorders (DEM_REORDER_INTERVAL)
     int pairBlocks = PAIR_BLOCKS;
+    // candidate lists with a skin (pure DEM): rebuild only when needed
+    bool verlet = false;       // active
+    bool vlistValid = false;   // lists built and not yet invalidated
+    double skin = 0.0;
+    double* refC = nullptr;    // [3][n] centres at the last rebuild
+    int* cand = nullptr;       // [CAND_MAX][n]
+    int* ncand = nullptr;      // [n]
+    uint32_t* nkey = nullptr;  // [MAX_LIST][n] order keys of the current contacts
+    int4* meta = nullptr;      // [n] cell and input row in cell order
+    int* hFlag = nullptr;      // pinned: displacement / overflow flags
+    History H2{};              // second history buffer (candidate-list rebuilds move rows)
+    long rebuilds = 0;
+    bool verletBroken = false;  // a rebuild overflowed CAND_MAX: lists stay off
     bool capturing = false;   // inside a HIP graph capture: no event records
+    bool countAlways = false; // per-step counters on every step (differentiable runs read them)
     GraphCache* graphs = nullptr;
     uint64_t intervalSteps = 0;
     size_t freeBefore = 0;
@@ -1752,10 +1819,11 @@ struct dem_solver {
         diff_release(diff);
         graph_release(graphs);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
-                        rec, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
+                        rec, refC, cand, ncand, nkey, meta, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
-                        H.ptVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart, O.fzPart,
+                        H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
+                        H2.pwVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart, O.fzPart,
                         O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, tLocal, tWorld, tAabb,
                         pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
                         regimeInstant, regimeInterval, microInstant, microInterval, bodyResult,
@@ -1765,6 +1833,7 @@ struct dem_solver {
         if (hPose) hipHostFree(hPose);
         if (hLoads) hipHostFree(hLoads);
         if (hScalars) hipHostFree(hScalars);
+        if (hFlag) hipHostFree(hFlag);
         for (auto& e : ev)
             if (e) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
@@ -2009,6 +2078,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         Iv(I_PID, p) = P->id[p];
         Iv(I_STATUS, p) = P->status[p];
         Iv(I_ORIG, p) = (int)p;
+        Iv(I_HIST, p) = (int)p;
         Iv(I_SCENE, p) = sceneOf[p];
     }
     for (int b = 0; b < 2; ++b) {
@@ -2052,6 +2122,27 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     TRY(dalloc(s->listBlk, blocks_for(n), "hipMalloc(contact list offsets)"));
     TRY(dalloc(s->pairs, (size_t)MAX_LIST * n, "hipMalloc(compacted contacts)"));
     TRY(dalloc(s->pairF, (size_t)6 * MAX_LIST * n, "hipMalloc(contact forces)"));
+    {
+        // candidate lists (pure DEM): skin = DEM_SKIN (default 0.2) x the smallest
+        // scene's largest radius; the +-2-cell rebuild scan must cover r_i + r_j + skin
+        const char* e = getenv("DEM_SKIN");
+        const double frac = e ? atof(e) : 0.2;
+        double rmin = maxrScene[0];
+        for (int k = 1; k < nsc; ++k) rmin = std::min(rmin, maxrScene[k]);
+        s->skin = frac * rmin;
+        bool ok = frac > 0.0 && (!B || B->count == 0);
+        for (int k = 0; k < nsc && ok; ++k) ok = 2.0 / s->hsp[k].inv >= 2.0 * maxrScene[k] + s->skin;
+        if (ok) {
+            TRY(dalloc(s->refC, (size_t)3 * n, "hipMalloc(candidate lists)"));
+            TRY(dalloc(s->cand, (size_t)CAND_MAX * n, "hipMalloc(candidate lists)"));
+            TRY(dalloc(s->ncand, n, "hipMalloc(candidate lists)"));
+            TRY(dalloc(s->nkey, (size_t)MAX_LIST * n, "hipMalloc(candidate lists)"));
+            TRY(dalloc(s->meta, n, "hipMalloc(candidate lists)"));
+            HIP_TRY(hipHostMalloc((void**)&s->hFlag, sizeof(int) * 2, hipHostMallocDefault), "hipHostMalloc");
+            s->hFlag[0] = s->hFlag[1] = 0;
+            s->verlet = true;
+        }
+    }
     TRY(dalloc(s->byorig, n, "hipMalloc(row index)"));
     // history
     TRY(dalloc(s->H.ppOcc, n, "hipMalloc(particle history)"));
@@ -2068,6 +2159,16 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipMemset(s->H.pwSeen, 0, sizeof(unsigned) * 2 * n), "clear history");
     HIP_TRY(hipMemset(s->H.ppKey, 0, sizeof(uint32_t) * PP_SLOTS * n), "clear history");
     HIP_TRY(hipMemset(s->H.pwKey, 0, sizeof(uint32_t) * PW_SLOTS * n), "clear history");
+    if (s->verlet) {
+        TRY(dalloc(s->H2.ppOcc, n, "hipMalloc(particle history)"));
+        TRY(dalloc(s->H2.ppSeen, 2 * n, "hipMalloc(particle history)"));
+        TRY(dalloc(s->H2.ppKey, (size_t)PP_SLOTS * n, "hipMalloc(particle history keys)"));
+        TRY(dalloc(s->H2.ppVal, (size_t)PP_SLOTS * 9 * n, "hipMalloc(particle history values)"));
+        TRY(dalloc(s->H2.pwOcc, n, "hipMalloc(wall history)"));
+        TRY(dalloc(s->H2.pwSeen, 2 * n, "hipMalloc(wall history)"));
+        TRY(dalloc(s->H2.pwKey, (size_t)PW_SLOTS * n, "hipMalloc(wall history keys)"));
+        TRY(dalloc(s->H2.pwVal, (size_t)PW_SLOTS * 9 * n, "hipMalloc(wall history values)"));
+    }
     // walls
     s->nw = W ? W->count : 0;
     TRY(dalloc(s->wallBuf, (size_t)6 * std::max(s->nw, 1), "hipMalloc(walls)"));
@@ -2186,6 +2287,7 @@ int dem_initialize_batch(dem_solver* s, const dem_particles_view* P, const dem_c
 namespace {
 
 int flush_finish(const dem_solver* cs);
+int history_rows(const dem_solver* s, std::vector<int>& hrow);
 
 TriSet triset(const dem_solver* s)
 {
@@ -2264,37 +2366,84 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[0], st), "event");
     hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
     Soa S = s->soa();
+    // candidate lists: only pure-DEM steps (bodies need the per-step cell order)
+    const bool verlet = s->verlet && nb == 0;
+    const bool lists = verlet && s->vlistValid;
     hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dSp, s->offs, s->sc,
-                       s->boxPart, s->partScene, s->sceneAcc);
-    hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,
-                       s->partScene, s->sceneAcc, s->dGrid, s->nscenes, s->cellCap, s->gridDone);
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dGrid, s->keyA, s->valA);
-    TRY(launch_check("grid keys"));
-    TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits,
-                   s->rsDigits));
-    hipLaunchKernelGGL(k_fixup, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->keyA, s->valA, n,
-                       S.i + (long)I_ORIG * n);
-    const bool reorder = s->reorderInterval > 0 && (s->stepIndex % s->reorderInterval) == 0;
-    if (reorder) {
-        Soa D{s->dState[1 - s->cur], s->iState[1 - s->cur], n};
-        hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valA);
-        hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->valA, n);
-        s->cur = 1 - s->cur;
-        S = s->soa();
+                       s->boxPart, s->partScene, s->sceneAcc, lists ? s->refC : nullptr,
+                       0.25 * s->skin * s->skin);
+    bool rebuild = true;
+    if (lists) {
+        if (s->capturing) return set_error(DEM_E_INVALID, "candidate-list steps cannot be captured");
+        HIP_TRY(hipMemcpyAsync(s->hFlag, &s->sc->need_rebuild, sizeof(int), hipMemcpyDeviceToHost, st),
+                "read displacement flag");
+        HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
+        rebuild = s->hFlag[0] != 0;
     }
-    const int cellBlocks = (int)std::min<unsigned long long>((s->cellCap + BLOCK - 1) / BLOCK, 65535ull);
-    hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, n, s->sc, s->cellStart,
-                       s->cellCap);
-    hipLaunchKernelGGL(k_posr, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->offs, s->posr,
-                       s->cidS);
+    if (rebuild) {
+        hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,
+                           s->partScene, s->sceneAcc, s->dGrid, s->nscenes, s->cellCap, s->gridDone);
+        hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dGrid, s->keyA, s->valA);
+        TRY(launch_check("grid keys"));
+        TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits,
+                       s->rsDigits));
+        hipLaunchKernelGGL(k_fixup, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->keyA, s->valA, n,
+                           S.i + (long)I_ORIG * n);
+        // with candidate lists the rows are put in cell order at every rebuild
+        const bool reorder = verlet || (s->reorderInterval > 0 && (s->stepIndex % s->reorderInterval) == 0);
+        if (reorder) {
+            Soa D{s->dState[1 - s->cur], s->iState[1 - s->cur], n};
+            hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valA);
+            hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->valA, n);
+            s->cur = 1 - s->cur;
+            S = s->soa();
+            if (verlet && s->H2.ppOcc) {
+                hipLaunchKernelGGL(k_hist_move, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->H, s->H2);
+                std::swap(s->H, s->H2);
+            }
+        }
+        const int cellBlocks = (int)std::min<unsigned long long>((s->cellCap + BLOCK - 1) / BLOCK, 65535ull);
+        hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, n, s->sc, s->cellStart,
+                           s->cellCap);
+    }
+    hipLaunchKernelGGL(k_posr, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->offs, s->posr, s->cidS,
+                       verlet ? s->meta : nullptr);
     hipLaunchKernelGGL(k_pack, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->offs, s->rec);
     TRY(launch_check("grid build"));
+    bool useLists = verlet;
+    if (verlet && rebuild) {
+        hipLaunchKernelGGL(k_vbuild, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->posr, s->keyA, s->cellStart,
+                           s->dGrid, s->sc, s->skin, s->cand, s->ncand, s->refC);
+        TRY(launch_check("candidate lists"));
+        if (s->capturing) return set_error(DEM_E_INVALID, "candidate-list steps cannot be captured");
+        HIP_TRY(hipMemcpyAsync(s->hFlag + 1, &s->sc->cand_overflow, sizeof(int), hipMemcpyDeviceToHost, st),
+                "read candidate overflow");
+        HIP_TRY(hipStreamSynchronize(st), "read candidate overflow");
+        ++s->rebuilds;
+        if (s->hFlag[1] != 0) {
+            // more partners than a list holds: fall back to the full scan for good
+            s->verlet = false;
+            s->verletBroken = true;
+            s->vlistValid = false;
+            useLists = false;
+        } else {
+            s->vlistValid = true;
+        }
+    }
     if (s->nt > 0) TRY(build_triangle_grid(s));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
     const int step = s->stepIndex++;
     const int nblk = blocks_for(n);
-    hipLaunchKernelGGL(k_scan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->cidS, s->keyA,
-                       s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc);
+    // the reference reads its per-step counters only on diagnostics steps
+    const bool counters = collect || s->countAlways;
+    if (useLists)
+        hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta, s->cidS,
+                           s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->nkey, s->ncount, s->tape.ppOcc,
+                           counters);
+    else
+        hipLaunchKernelGGL(k_scan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->cidS, s->keyA,
+                           s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
+                       counters);
     hipLaunchKernelGGL(k_off_blocks, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk);
     hipLaunchKernelGGL(k_off_scan, dim3(1), dim3(BLOCK), 0, st, s->listBlk, nblk, s->sc);
     hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk, s->nbr,
@@ -2314,7 +2463,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (step == 0) A.pt.dt = 0.0;
     A.g[0] = s->g[0]; A.g[1] = s->g[1]; A.g[2] = s->g[2];
     A.gravity = s->gravity; A.universal = s->universal; A.byorig = s->byorig;
-    A.step = step; A.offs = s->offs; A.nb = nb; A.tape = s->tape;
+    A.step = step; A.offs = s->offs; A.nb = nb; A.tape = s->tape; A.counters = counters;
     if (s->universal) hipLaunchKernelGGL(k_byorig, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->byorig);
     if (nb > 0) {
         const int fb = 1024;
@@ -2429,6 +2578,21 @@ namespace {
 // finishParticleStep is deferred so it can run fused with the next
 // advanceParticles; any other call that reads or writes particle state
 // completes it first.
+// History-table row of every input row (the tables follow the particles
+// when candidate-list rebuilds reorder them).
+int history_rows(const dem_solver* s, std::vector<int>& hrow)
+{
+    const long n = s->n;
+    std::vector<int> orig(n), hist(n);
+    HIP_TRY(hipMemcpy(orig.data(), s->iState[s->cur] + (size_t)I_ORIG * n, sizeof(int) * n, hipMemcpyDeviceToHost),
+            "download particle rows");
+    HIP_TRY(hipMemcpy(hist.data(), s->iState[s->cur] + (size_t)I_HIST * n, sizeof(int) * n, hipMemcpyDeviceToHost),
+            "download history rows");
+    hrow.assign(n, 0);
+    for (long p = 0; p < n; ++p) hrow[orig[p]] = hist[p];
+    return DEM_OK;
+}
+
 int flush_finish(const dem_solver* cs)
 {
     dem_solver* s = const_cast<dem_solver*>(cs);
@@ -2583,10 +2747,23 @@ int dem_debug_grid(const dem_solver* s, uint64_t* keys, int32_t* ids)
     TRY(dalloc(di, n, "hipMalloc"));
     hipLaunchKernelGGL(k_debug_grid, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), s->valA, dk, di);
     HIP_TRY(hipStreamSynchronize(s->stream), "debug grid");
-    if (keys) HIP_TRY(hipMemcpy(keys, dk, sizeof(uint64_t) * n, hipMemcpyDeviceToHost), "download");
-    if (ids) HIP_TRY(hipMemcpy(ids, di, sizeof(int) * n, hipMemcpyDeviceToHost), "download");
+    std::vector<uint64_t> hk(n);
+    std::vector<int32_t> hi(n);
+    HIP_TRY(hipMemcpy(hk.data(), dk, sizeof(uint64_t) * n, hipMemcpyDeviceToHost), "download");
+    HIP_TRY(hipMemcpy(hi.data(), di, sizeof(int) * n, hipMemcpyDeviceToHost), "download");
     hipFree(dk);
     hipFree(di);
+    // (cell key, input row) order of the current cells: the device order is
+    // already that after a sort, and the last rebuild's with candidate lists
+    std::vector<long> idx(n);
+    std::iota(idx.begin(), idx.end(), 0L);
+    std::sort(idx.begin(), idx.end(), [&](long a, long b) {
+        return hk[a] != hk[b] ? hk[a] < hk[b] : hi[a] < hi[b];
+    });
+    for (long i = 0; i < n; ++i) {
+        if (keys) keys[i] = hk[idx[i]];
+        if (ids) ids[i] = hi[idx[i]];
+    }
     return DEM_OK;
 }
 
@@ -2596,6 +2773,8 @@ int dem_debug_history(const dem_solver* s, uint64_t* pk, double* pv, uint64_t* w
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     HIP_TRY(hipStreamSynchronize(s->stream), "debug history");
     const long n = s->n;
+    std::vector<int> hrow;
+    TRY(history_rows(s, hrow));
     struct Tab { const void* occ; bool wide; const uint32_t* key; const double* val; int slots; uint64_t* ok; double* ov; };
     Tab tabs[] = {{s->H.ppOcc, true, s->H.ppKey, s->H.ppVal, PP_SLOTS, pk, pv},
                   {s->H.pwOcc, false, s->H.pwKey, s->H.pwVal, PW_SLOTS, wk, wv},
@@ -2622,14 +2801,16 @@ int dem_debug_history(const dem_solver* s, uint64_t* pk, double* pv, uint64_t* w
             val.resize((size_t)T.slots * 9 * n);
             HIP_TRY(hipMemcpy(val.data(), T.val, sizeof(double) * val.size(), hipMemcpyDeviceToHost), "download");
         }
-        for (long i = 0; i < n; ++i)
+        for (long i = 0; i < n; ++i) {
+            const long h = hrow[i];
             for (int b = 0; b < T.slots; ++b) {
-                const bool on = (occ[i] >> b) & 1ull;
-                const size_t idx = (size_t)i * T.slots + b;
-                if (T.ok) T.ok[idx] = on ? key[idx] : 0;
+                const bool on = (occ[h] >> b) & 1ull;
+                const size_t idx = (size_t)i * T.slots + b, src = (size_t)h * T.slots + b;
+                if (T.ok) T.ok[idx] = on ? key[src] : 0;
                 if (T.ov)
-                    for (int q = 0; q < 9; ++q) T.ov[idx * 9 + q] = on ? val[idx * 9 + q] : 0.0;
+                    for (int q = 0; q < 9; ++q) T.ov[idx * 9 + q] = on ? val[src * 9 + q] : 0.0;
             }
+        }
     }
     return DEM_OK;
 }
@@ -2771,17 +2952,32 @@ int dem_load_contact_history(dem_solver* s, const char* path)
     // keys -> occupancy masks; every restored slot counts as seen on the step
     // before the next one (the reference resets lastSeen to -1 and the step
     // counter to 0, so nothing expires on the next evaluation)
+    std::vector<int> hrow;
+    TRY(history_rows(s, hrow));
+    // file rows are input rows; the tables are indexed by history row
     auto masks = [&](const std::vector<uint64_t>& keys, int slots_, std::vector<uint32_t>& k32,
                      std::vector<unsigned long long>& occ) {
         k32.resize(keys.size());
         occ.assign(nc, 0ull);
-        for (size_t i = 0; i < nc; ++i)
+        for (size_t i = 0; i < nc; ++i) {
+            const size_t h = (size_t)hrow[i];
             for (int b = 0; b < slots_; ++b) {
                 const uint64_t k = keys[i * slots_ + b];
-                k32[i * slots_ + b] = (uint32_t)k;
-                if (k) occ[i] |= 1ull << b;
+                k32[h * slots_ + b] = (uint32_t)k;
+                if (k) occ[h] |= 1ull << b;
             }
+        }
     };
+    auto rows9 = [&](std::vector<double>& v, int slots_) {
+        std::vector<double> r(v.size());
+        for (size_t i = 0; i < nc; ++i)
+            std::copy(v.begin() + i * slots_ * 9, v.begin() + (i + 1) * slots_ * 9,
+                      r.begin() + (size_t)hrow[i] * slots_ * 9);
+        v.swap(r);
+    };
+    rows9(pv, PP_SLOTS);
+    rows9(wv, PW_SLOTS);
+    rows9(bv, PT_SLOTS);
     std::vector<uint32_t> pk32, wk32, bk32;
     std::vector<unsigned long long> pocc, wocc, bocc;
     masks(pk, PP_SLOTS, pk32, pocc);
@@ -2810,6 +3006,7 @@ int dem_load_contact_history(dem_solver* s, const char* path)
     (void)st;
     s->stepIndex = 0;
     s->restored = v2;
+    s->vlistValid = false;
     return DEM_OK;
 }
 
@@ -2832,6 +3029,8 @@ int dem_save_contact_history(const dem_solver* cs, const char* path)
                             F_WZ,  F_VHX, F_VHY, F_VHZ, F_WHX, F_WHY, F_WHZ, F_HQW, F_HQX, F_HQY, F_HQZ};
     for (long p = 0; p < n; ++p)
         for (int k = 0; k < 23; ++k) state[(size_t)23 * orig[p] + k] = hd[(size_t)fields[k] * n + p];
+    std::vector<int> hrow;
+    TRY(history_rows(cs, hrow));
     // history tables with free slots cleared
     auto table = [&](const void* occDev, bool wide, const uint32_t* keyDev, const double* valDev, int slots_,
                      std::vector<uint64_t>& keys, std::vector<double>& vals) -> int {
@@ -2850,13 +3049,15 @@ int dem_save_contact_history(const dem_solver* cs, const char* path)
         std::vector<double> v(nc * slots_ * 9);
         HIP_TRY(hipMemcpy(k32.data(), keyDev, 4 * k32.size(), hipMemcpyDeviceToHost), "download history keys");
         HIP_TRY(hipMemcpy(v.data(), valDev, 8 * v.size(), hipMemcpyDeviceToHost), "download history values");
-        for (size_t i = 0; i < nc; ++i)
+        for (size_t i = 0; i < nc; ++i) {
+            const size_t h = (size_t)hrow[i];
             for (int b = 0; b < slots_; ++b)
-                if ((occ[i] >> b) & 1ull) {
-                    const size_t idx = i * slots_ + b;
-                    keys[idx] = k32[idx];
-                    for (int q = 0; q < 9; ++q) vals[idx * 9 + q] = v[idx * 9 + q];
+                if ((occ[h] >> b) & 1ull) {
+                    const size_t idx = i * slots_ + b, src = h * slots_ + b;
+                    keys[idx] = k32[src];
+                    for (int q = 0; q < 9; ++q) vals[idx * 9 + q] = v[src * 9 + q];
                 }
+        }
         return DEM_OK;
     };
     std::vector<uint64_t> pk, wk, bk;

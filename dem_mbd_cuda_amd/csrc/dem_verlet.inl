@@ -1,0 +1,159 @@
+// dem_verlet.inl -- candidate lists with a skin for pure-DEM steps.
+// Included by dem_hip.hip after the contact-scan kernels.
+//
+// The reference re-sorts the grid and scans 27 cells for every component on
+// every step (contact:1121-1143).  Here the grid is rebuilt only when some
+// component centre has moved more than skin/2 since the last rebuild.  At a
+// rebuild every component lists the partners within r_i + r_j + skin
+// (scanning +-2 cells, which covers that radius for cells of at least
+// 2 r_max); in between, every listed partner is re-tested with the exact
+// overlap test of the full scan, restricted to the 27 cells around the
+// component's current cell as the reference is, and the contacts are put in
+// the reference's candidate order: cell offset (dx, dy, dz) lexicographic,
+// then input row within a cell.  Any pair that overlaps now was within
+// r_i + r_j + skin at the rebuild (each centre moved at most skin/2), so the
+// contact set, its order and hence every slot and force are the full scan's.
+
+constexpr int CAND_MAX = 96;  // listed partners per component
+
+// Rebuild: partners within r_i + r_j + skin, rows in cell order (the
+// arrays were just reordered, so sorted index = row), and the centres the
+// displacement test compares against.
+__global__ void __launch_bounds__(BLOCK) k_vbuild(Soa S, const double4* posr, const unsigned long long* key,
+                                                  const unsigned* start, const SceneGrid* grid, Scalars* sc,
+                                                  double skin, int* cand, int* ncand, double* refC)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long n = S.n;
+    if (t >= n) return;
+    const double4 me = posr[t];
+    refC[t] = me.x;
+    refC[n + t] = me.y;
+    refC[2 * n + t] = me.z;
+    int count = 0;
+    if (S.k(I_VALID, t)) {
+        const SceneGrid& g = grid[S.k(I_SCENE, t)];
+        const long X = (long)g.dims[0], Y = (long)g.dims[1], Z = (long)g.dims[2];
+        const long cx = S.k(I_CX, t) - g.lo[0], cy = S.k(I_CY, t) - g.lo[1], cz = S.k(I_CZ, t) - g.lo[2];
+        const long zlo = cz > 1 ? cz - 2 : 0, zhi = cz + 2 < Z ? cz + 2 : Z - 1;
+        const bool dense = sc->dense;
+        for (long dx = -2; dx <= 2; ++dx) {
+            const long xx = cx + dx;
+            if (xx < 0 || xx >= X) continue;
+            for (long dy = -2; dy <= 2; ++dy) {
+                const long yy = cy + dy;
+                if (yy < 0 || yy >= Y) continue;
+                const unsigned long long col = g.base + ((unsigned long long)xx * Y + yy) * Z;
+                long b, e;
+                if (dense) {
+                    b = start[col + zlo];
+                    e = start[col + zhi + 1];
+                } else {
+                    b = lower_bound_u64(key, n, col + zlo);
+                    e = lower_bound_u64(key, n, col + zhi + 1);
+                }
+                for (long k = b; k < e; ++k) {
+                    if (k == t) continue;
+                    const double4 c = posr[k];
+                    const double ex = c.x - me.x, ey = c.y - me.y, ez = c.z - me.z;
+                    const double reach = me.w + c.w + skin;
+                    if (ex * ex + ey * ey + ez * ez >= reach * reach) continue;
+                    if (count < CAND_MAX) cand[(long)count * n + t] = (int)k;
+                    ++count;
+                }
+            }
+        }
+    }
+    if (count > CAND_MAX) atomicAdd(&sc->cand_overflow, 1);
+    ncand[t] = min(count, CAND_MAX);
+}
+
+// Step between rebuilds: exact contact test over the listed partners, the
+// reference's candidate order, then history slots (assign_slots).
+__global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const double4* posr, const int4* meta,
+                                                 const uint32_t* cidS, const int* cand, const int* ncand,
+                                                 Scalars* sc, History H, int step, int* nbr, uint32_t* nkey,
+                                                 int* ncount, unsigned long long* occTape, bool counters)
+{
+    const long t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const long n = S.n;
+    int found = 0, stored = 0, high = 0, over = 0;
+    if (t < n) {
+        if (S.k(I_VALID, t)) {
+            const double4 me = posr[t];
+            const int4 mine = meta[t];
+            const int m = ncand[t];
+            for (int c = 0; c < m; ++c) {
+                const long k = cand[(long)c * n + t];
+                const double4 o = posr[k];
+                const double dx_ = o.x - me.x, dy_ = o.y - me.y, dz_ = o.z - me.z;
+                const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+                const double rs = me.w + o.w;
+                if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
+                const int4 other = meta[k];
+                const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
+                if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
+                const uint32_t kk = ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) |
+                                    (uint32_t)other.w;
+                if (found >= MAX_LIST) {
+                    ++found;
+                    continue;
+                }
+                // insertion into the ordered list (nearly sorted: lists keep
+                // their order from step to step)
+                int pos = found;
+                while (pos > 0 && nkey[(long)(pos - 1) * n + t] > kk) {
+                    nkey[(long)pos * n + t] = nkey[(long)(pos - 1) * n + t];
+                    nbr[(long)pos * n + t] = nbr[(long)(pos - 1) * n + t];
+                    --pos;
+                }
+                nkey[(long)pos * n + t] = kk;
+                nbr[(long)pos * n + t] = (int)k;
+                ++found;
+            }
+        }
+        if (found > MAX_LIST) {
+            atomicAdd(&sc->list_overflow, 1);
+            found = MAX_LIST;
+        }
+        stored = assign_slots(S, t, sorted[t], found, sorted, cidS, H, step, nbr, occTape, high, over);
+        ncount[t] = stored;
+    }
+    if (counters) {
+        atomic_add_wave(&sc->particle_contacts, stored);
+        atomic_max_wave(&sc->pp_high, high);
+    }
+    atomic_add_wave(&sc->pp_overflow, over);
+}
+
+// History rows follow the particles: after a rebuild reordered the particle
+// arrays, row t's history moves from its old row (I_HIST) to row t, so the
+// per-contact history traffic of the following steps is as local as the
+// particle data.  Only the slots below the highest occupied one are copied
+// (the others are never read before being reset).
+__global__ void __launch_bounds__(BLOCK) k_hist_move(Soa S, History src, History dst)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const long n = S.n;
+    if (t >= n) return;
+    const long h = S.k(I_HIST, t);
+    const unsigned long long occ = src.ppOcc[h];
+    dst.ppOcc[t] = occ;
+    dst.ppSeen[t] = src.ppSeen[h];
+    dst.ppSeen[n + t] = src.ppSeen[n + h];
+    const int top = occ ? 64 - __clzll((long long)occ) : 0;
+    for (int b = 0; b < top; ++b) dst.ppKey[t * PP_SLOTS + b] = src.ppKey[h * PP_SLOTS + b];
+    const double* sv = src.ppVal + h * PP_SLOTS * 9;
+    double* dv = dst.ppVal + t * PP_SLOTS * 9;
+    for (int k = 0; k < top * 9; ++k) dv[k] = sv[k];
+    const unsigned wocc = src.pwOcc[h];
+    dst.pwOcc[t] = wocc;
+    dst.pwSeen[t] = src.pwSeen[h];
+    dst.pwSeen[n + t] = src.pwSeen[n + h];
+    const int wtop = wocc ? 32 - __clz((int)wocc) : 0;
+    for (int b = 0; b < wtop; ++b) dst.pwKey[t * PW_SLOTS + b] = src.pwKey[h * PW_SLOTS + b];
+    const double* sw = src.pwVal + h * PW_SLOTS * 9;
+    double* dw = dst.pwVal + t * PW_SLOTS * 9;
+    for (int k = 0; k < wtop * 9; ++k) dw[k] = sw[k];
+    S.k(I_HIST, t) = (int)t;
+}
