@@ -540,7 +540,7 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     if (steps < 1 || checkpoint_interval < 1) return set_error(DEM_E_INVALID, "steps and checkpoint interval must be positive");
     if (s->stepIndex != 0) return set_error(DEM_E_INVALID, "a differentiable run starts from a freshly initialized solver");
-    if (s->nb > 0 || s->offs || s->universal)
+    if (s->nb > 0 || s->offs || s->universal || s->clump)
         return set_error(DEM_E_UNSUPPORTED, "differentiable runs cover spheres, walls and gravity (no bodies, clump offsets or universal gravitation)");
     for (const SceneParams& q : s->hsp)
         if (plate_wall < 0 || plate_wall >= q.nwall)

@@ -146,9 +146,11 @@ int dem_run_steps(dem_solver* s, int steps, double dt, int flags, int* steps_don
         std::vector<Cfg> cfgs;
         const int grids[3] = {2048, 8192, 32768};
         const bool canLists = s->refC != nullptr && s->nb == 0 && !s->verletBroken;
+        // clumps keep their rows: reorder interval 0 only
+        const std::vector<int> reorders = s->clump ? std::vector<int>{0} : std::vector<int>{1, 4, 8};
         for (int g : grids) {
             if (canLists) cfgs.push_back({true, s->reorderInterval, g});
-            for (int r : {1, 4, 8}) cfgs.push_back({false, r, g});
+            for (int r : reorders) cfgs.push_back({false, r, g});
         }
         const int span = 4;
         if (steps >= (int)cfgs.size() * (span + 4) + 8) {

@@ -271,6 +271,22 @@ __global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc,
     }
 }
 
+// Owner state copied to each of its component rows (clumps), so that the
+// contact pipeline runs over components with the per-row code of single
+// spheres: the component centre and velocities follow from the owner's
+// pose and the body-frame offset (updateComponentsKernel, particle:116-147).
+__global__ void __launch_bounds__(BLOCK) k_expand(Soa S, Soa R, const int* owner)
+{
+    const long c = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= R.n) return;
+    const long o = owner[c];
+    constexpr int fields[] = {F_X,   F_Y,   F_Z,   F_VX,  F_VY,  F_VZ,  F_WX,  F_WY,  F_WZ,  F_QW,
+                              F_QX,  F_QY,  F_QZ,  F_VHX, F_VHY, F_VHZ, F_WHX, F_WHY, F_WHZ, F_MASS};
+#pragma unroll
+    for (int f : fields) R.f(f, c) = S.f(f, o);
+    R.k(I_STATUS, c) = S.k(I_STATUS, o);
+}
+
 // component centre + cell + range check + bbox (grid:51-71), cell size of
 // the particle's scene.  The cell bounding box of each scene is reduced per
 // block (wave shuffles, then LDS) into part[block][6] when the block lies in
@@ -700,12 +716,19 @@ __global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Rec* R)
 
 // Contact centre, radius and component id in sorted order (what the
 // candidate scan reads per candidate).
+// With clumps it also records each row's sorted position and the owner of
+// the t-th sorted row (the scan skips partners of the same owner).
 __global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool offs, double4* posr,
-                                                uint32_t* cidS, int4* meta)
+                                                uint32_t* cidS, int4* meta, const int* owner, int* ownS,
+                                                int* rank)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= S.n) return;
     const long p = sorted[t];
+    if (ownS) {
+        ownS[t] = owner[p];
+        rank[p] = (int)t;
+    }
     double4 v;
     if (offs) { v.x = S.f(F_CPX, p); v.y = S.f(F_CPY, p); v.z = S.f(F_CPZ, p); }
     else { v.x = S.f(F_X, p); v.y = S.f(F_Y, p); v.z = S.f(F_Z, p); }
@@ -777,7 +800,10 @@ struct Tape {
 };
 
 struct LawArgs {
-    Soa S;
+    Soa S;                       // owners
+    Soa R;                       // contact rows: the owners, or their components (clumps)
+    const int* rank;             // [rows] sorted position of each contact row (clumps)
+    const int* ownerStart;       // [owners + 1] first component of each owner (clumps)
     const Rec* rec;
     const int* sorted;
     const int* ncount;
@@ -895,17 +921,21 @@ __device__ __forceinline__ int assign_slots(const Soa& S, long t, long p, int fo
 // overlapping candidates, so the slot lookups (dependent loads) run once per
 // contact rather than once per candidate iteration of the wave.  Leaves the
 // owner's contact list as entries (partner row << SLOT_BITS | slot), k-major.
+// CLUMP: rows are clump components; partners of the row's own owner are
+// skipped (computeContactsKernel compares owners, contact:1138).
+template <bool CLUMP>
 __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const double4* posr,
                                                 const uint32_t* cidS, const unsigned long long* key,
                                                 const unsigned* start, const SceneGrid* grid, Scalars* sc,
                                                 History H, int step, int* nbr, int* ncount,
-                                                unsigned long long* occTape, bool counters)
+                                                unsigned long long* occTape, bool counters, const int* ownS)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
     int found = 0, stored = 0, high = 0, over = 0;
     if (t < n) {
         const long p = sorted[t];
+        const int mine = CLUMP ? ownS[t] : 0;
         if (S.k(I_VALID, p)) {
             const double4 me = posr[t];
             const SceneGrid& g = grid[S.k(I_SCENE, p)];
@@ -930,7 +960,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
                         e = lower_bound_u64(key, n, col + zhi + 1);
                     }
                     for (long k = b; k < e; ++k) {
-                        if (k == t) continue;
+                        if (CLUMP ? ownS[k] == mine : k == t) continue;
                         const double4 c = posr[k];
                         const double dx_ = c.x - me.x, dy_ = c.y - me.y, dz_ = c.z - me.z;
                         const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
@@ -1070,8 +1100,206 @@ __device__ __forceinline__ long upper_bound_u64(const unsigned long long* k, lon
     return first;
 }
 
-// computeContactsKernel (contact:1065-1221) for one owner, plus the
-// acceleration (particle:71-101).
+// Field force on owner row p: gravity, then the O(N^2) universal
+// gravitation over the owners in input-row order (contact:1083-1106).
+__device__ __forceinline__ V3 field_force(const LawArgs& A, long p, double mass)
+{
+    const Soa& S = A.S;
+    V3 fF = zero3();
+    if (A.gravity) fF = mk(mass * A.g[0], mass * A.g[1], mass * A.g[2]);
+    if (A.universal) {
+        const V3 pi = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p));
+        for (long oj = 0; oj < S.n; ++oj) {
+            const long j = A.byorig[oj];
+            if (j == p) continue;
+            const V3 d = sub(pi, mk(S.f(F_X, j), S.f(F_Y, j), S.f(F_Z, j)));
+            const double d2 = dot(d, d);
+            if (d2 > 1.0e-28)
+                fF = add(fF, scl(d, kGravityConstant * mass * S.f(F_MASS, j) / (d2 * sqrt(d2))));
+        }
+    }
+    return fF;
+}
+
+// Walls of one contact row r (history row hr) in wall order (wallContact,
+// contact:975-1063), after the row's wall-history expiry (expireHistory,
+// contact:217-229: slots not seen last step are dropped); adds to wF / wT.
+__device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long r, long hr, int scene, V3& wF,
+                                          V3& wT, int& bodyHigh, int& pwOver)
+{
+    const History& H = A.H;
+    const long nr = A.R.n;
+    const int s = A.step;
+    unsigned pwOcc = H.pwOcc[hr] & H.pwSeen[(long)((s + 1) & 1) * nr + hr];
+    unsigned pwCur = 0;
+    if (A.tape.pwOcc) A.tape.pwOcc[hr] = pwOcc;
+    // walls of the row's scene (history key: the scene's wall index + 1)
+    const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
+    Material mw = A.matPW[scene];
+    if (A.first) mw.dt = 0.0;
+    for (int w = 0; w < nw; ++w) {
+        const int g = w0 + w;
+        const V3 outward = mk(A.W.nrm[3 * g], A.W.nrm[3 * g + 1], A.W.nrm[3 * g + 2]);
+        const V3 origin = mk(A.W.o[3 * g], A.W.o[3 * g + 1], A.W.o[3 * g + 2]);
+        const double distance = dot(sub(I.pos, origin), outward);
+        const double depth = I.r - distance;
+        if (depth <= 0.0) continue;
+        const int slot = acquire(pwOcc, pwCur, H.pwKey + hr * PW_SLOTS, H.pwVal + hr * PW_SLOTS * 9,
+                                 PW_SLOTS, (uint32_t)(w + 1), bodyHigh, pwOver);
+        if (slot < 0) continue;
+        double* hv = H.pwVal + (hr * PW_SLOTS + slot) * 9;
+        if (A.tape.wallCount) {
+            const unsigned e = atomicAdd(A.tape.wallCount, 1u);
+            if (A.tape.wall) {
+                A.tape.wall[e] = make_int4((int)r, w, slot, 0);
+                for (int k = 0; k < 9; ++k) A.tape.wallPrev[(long)e * 9 + k] = hv[k];
+            }
+        }
+        const V3 cf = pw_law(I, outward, depth, mw, hv, wF, wT);
+        if (A.tape.plateAcc && w == A.tape.plate && !A.first) A.tape.plateAcc[r] += cf.z;
+    }
+    H.pwOcc[hr] = pwOcc;
+    H.pwSeen[(long)(s & 1) * nr + hr] = pwCur;
+}
+
+// Particle-triangle part of computeContactsKernel (contact:1148-1208) for
+// contact row r at sorted position t (history row hr, owner input row o):
+// per body, the area-weighted manifold over the triangles of the row's own
+// cell advances one history slot, then every triangle contributes its
+// read-only contact.  Adds the row's force and torque to bF / bT and the
+// owner's per-body outputs at o; writes the row's body load and Fz at t
+// (reduced over the rows later) and its regime state at r.
+__device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r, long t, long hr, long o,
+                                         V3& bF, V3& bT, int& bodyHigh, int& ptOver, int& bcount)
+{
+    const Soa& R = A.R;
+    const long nr = R.n, no = A.S.n;
+    const int s = A.step;
+    const History& H = A.H;
+    unsigned long long ptOcc = H.ptOcc[hr] & H.ptSeen[(long)((s + 1) & 1) * nr + hr];
+    unsigned long long ptCur = 0;
+    const TriSet& T = A.T;
+    if (T.nt > 0 && T.entries > 0 && R.k(I_VALID, r)) {
+        const unsigned long long key = pack_cell(R.k(I_CX, r), R.k(I_CY, r), R.k(I_CZ, r));
+        const long b0 = lower_bound_u64(T.tkey, T.entries, key);
+        const long e0 = upper_bound_u64(T.tkey, T.entries, key);
+        for (int body = 0; body < A.nb; ++body) {
+            double total = 0.0, wD = 0.0;
+            V3 wN = zero3(), wC = zero3();
+            for (long k = b0; k < e0; ++k) {
+                const int tr = T.ttri[k];
+                if (T.body[tr] != body) continue;
+                const Tri tri = load_tri(T, tr);
+                const double scv = tri_area_scale(I.pos, I.r, tri);
+                if (scv <= 0.0) continue;
+                const double distance = dot(tri.n, sub(I.pos, tri.a));
+                const V3 cc = sub(I.pos, scl(tri.n, distance));
+                total += scv;
+                wN = add(wN, scl(tri.n, -scv));
+                wC = add(wC, scl(cc, scv));
+                wD += distance * scv;
+            }
+            double bl[6] = {0, 0, 0, 0, 0, 0};
+            double fz = 0.0;
+            if (total > 0.0) {
+                const V3 mN = unit(wN);
+                const V3 mC = scl(wC, 1.0 / total);
+                const double mD = wD / total;
+                const int slot = acquire(ptOcc, ptCur, H.ptKey + hr * PT_SLOTS, H.ptVal + hr * PT_SLOTS * 9,
+                                         PT_SLOTS, (uint32_t)(body + 1), bodyHigh, ptOver);
+                if (slot >= 0) {
+                    double* sv = H.ptVal + (hr * PT_SLOTS + slot) * 9;
+                    const Pose P = load_pose(T, body);
+                    bool slide = false, twisty = false;
+                    manifold_law(I, P, A.pt, mN, mC, mD, sv, &slide, &twisty);
+                    A.O.regime[(long)body * nr + r] = 1 | (slide ? 2 : 0) | (twisty ? 4 : 0);
+                    Material ro = A.pt;
+                    ro.dt = 0.0;
+                    for (long k = b0; k < e0; ++k) {
+                        const int tr = T.ttri[k];
+                        if (T.body[tr] != body) continue;
+                        const Tri tri = load_tri(T, tr);
+                        const PtOut q = pt_law(I, tri, P, ro, sv);
+                        if (!q.hit) continue;
+                        bF = add(bF, q.pf);
+                        bT = add(bT, q.ptq);
+                        const V3 btq = add(add(q.bnt, q.bft), add(q.btw, q.brl));
+                        bl[0] += q.bnf.x + q.bfr.x;
+                        bl[1] += q.bnf.y + q.bfr.y;
+                        bl[2] += q.bnf.z + q.bfr.z;
+                        bl[3] += btq.x;
+                        bl[4] += btq.y;
+                        bl[5] += btq.z;
+                        fz += q.bnf.z + q.bfr.z;
+                        ++bcount;
+                        atomicMax(&A.sc->body_max_depth_bits, (unsigned long long)__double_as_longlong(q.depth));
+                        if (A.O.bodyParticle) {
+                            double* d = A.O.bodyParticle + ((long)body * no + o) * 6;
+                            d[0] += q.pf.x; d[1] += q.pf.y; d[2] += q.pf.z;
+                            d[3] += q.ptq.x; d[4] += q.ptq.y; d[5] += q.ptq.z;
+                        }
+                        if (A.O.bodyDetail) {
+                            double* d = A.O.bodyDetail + ((long)body * no + o) * 19;
+                            const V3 gr[6] = {q.bnf, q.bfr, q.bnt, q.bft, q.btw, q.brl};
+                            for (int g = 0; g < 6; ++g) {
+                                d[3 * g] += gr[g].x;
+                                d[3 * g + 1] += gr[g].y;
+                                d[3 * g + 2] += gr[g].z;
+                            }
+                            if (d[18] < q.depth) d[18] = q.depth;
+                        }
+                    }
+                }
+            }
+            for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * nr + t] = bl[k];
+            A.O.fzPart[(long)body * nr + t] = fz;
+        }
+    } else {
+        for (int body = 0; body < A.nb; ++body) {
+            for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * nr + t] = 0.0;
+            A.O.fzPart[(long)body * nr + t] = 0.0;
+        }
+    }
+    H.ptOcc[hr] = ptOcc;
+    H.ptSeen[(long)(s & 1) * nr + hr] = ptCur;
+}
+
+// Force outputs of owner row p (contact:1211-1220) and its acceleration
+// (accelerationKernel, particle:71-101).
+__device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3 cF, V3 cT, V3 fF, V3 wF, V3 wT,
+                                          V3 bF, V3 bT)
+{
+    const Soa& S = A.S;
+    const long n = S.n;
+    double* oc = A.O.contact;
+    oc[0 * n + p] = cF.x; oc[1 * n + p] = cF.y; oc[2 * n + p] = cF.z;
+    oc[3 * n + p] = cT.x; oc[4 * n + p] = cT.y; oc[5 * n + p] = cT.z;
+    double* ow = A.O.wall;
+    ow[0 * n + p] = wF.x; ow[1 * n + p] = wF.y; ow[2 * n + p] = wF.z;
+    ow[3 * n + p] = wT.x; ow[4 * n + p] = wT.y; ow[5 * n + p] = wT.z;
+    double* of = A.O.field;
+    of[0 * n + p] = fF.x; of[1 * n + p] = fF.y; of[2 * n + p] = fF.z;
+    const V3 tF = add(add(cF, fF), add(wF, bF));
+    const V3 tT = add(cT, add(wT, bT));
+    S.f(F_AX, p) = tF.x / m;
+    S.f(F_AY, p) = tF.y / m;
+    S.f(F_AZ, p) = tF.z / m;
+    const double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+    const double tx = (1.0 - 2.0 * (qy * qy + qz * qz)) * tT.x + 2.0 * (qx * qy + qw * qz) * tT.y +
+                      2.0 * (qx * qz - qw * qy) * tT.z;
+    const double ty = 2.0 * (qx * qy - qw * qz) * tT.x + (1.0 - 2.0 * (qx * qx + qz * qz)) * tT.y +
+                      2.0 * (qy * qz + qw * qx) * tT.z;
+    const double tz = 2.0 * (qx * qz + qw * qy) * tT.x + 2.0 * (qy * qz - qw * qx) * tT.y +
+                      (1.0 - 2.0 * (qx * qx + qy * qy)) * tT.z;
+    const double wx = S.f(F_WX, p), wy = S.f(F_WY, p), wz = S.f(F_WZ, p);
+    const double ix = S.f(F_IX, p), iy = S.f(F_IY, p), iz = S.f(F_IZ, p);
+    S.f(F_AWX, p) = (tx - (iz - iy) * wz * wy) / ix;
+    S.f(F_AWY, p) = (ty - (ix - iz) * wx * wz) / iy;
+    S.f(F_AWZ, p) = (tz - (iy - ix) * wy * wx) / iz;
+}
+
+// computeContactsKernel (contact:1065-1221) for one single-sphere owner,
+// plus the acceleration (particle:71-101).
 __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
 {
     const long t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -1080,55 +1308,11 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
     int bodyHigh = 0, pwOver = 0;
     if (t < n) {
         const long p = A.sorted[t];
-        const long o = S.k(I_HIST, p);  // history row
-        const int s = A.step;
         Body I = rec_body(A.rec, p);
         if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
-        V3 cF = zero3(), cT = zero3(), fF = zero3(), wF = zero3(), wT = zero3(), bF = zero3(),
-           bT = zero3();
-        if (A.gravity) fF = mk(I.mass * A.g[0], I.mass * A.g[1], I.mass * A.g[2]);
-        if (A.universal) {
-            const V3 pi = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p));
-            for (long oj = 0; oj < n; ++oj) {
-                const long j = A.byorig[oj];
-                if (j == p) continue;
-                const V3 d = sub(pi, mk(S.f(F_X, j), S.f(F_Y, j), S.f(F_Z, j)));
-                const double d2 = dot(d, d);
-                if (d2 > 1.0e-28)
-                    fF = add(fF, scl(d, kGravityConstant * I.mass * S.f(F_MASS, j) / (d2 * sqrt(d2))));
-            }
-        }
-        // wall expiry (expireHistory, contact:217-229): drop slots not seen last step
-        const History& H = A.H;
-        unsigned pwOcc = H.pwOcc[o] & H.pwSeen[(long)((s + 1) & 1) * n + o];
-        unsigned pwCur = 0;
-        if (A.tape.pwOcc) A.tape.pwOcc[o] = pwOcc;
-        // walls of the particle's scene (history key: the scene's wall index + 1)
-        const int scene = S.k(I_SCENE, p);
-        const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
-        Material mw = A.matPW[scene];
-        if (A.first) mw.dt = 0.0;
-        for (int w = 0; w < nw; ++w) {
-            const int g = w0 + w;
-            const V3 outward = mk(A.W.nrm[3 * g], A.W.nrm[3 * g + 1], A.W.nrm[3 * g + 2]);
-            const V3 origin = mk(A.W.o[3 * g], A.W.o[3 * g + 1], A.W.o[3 * g + 2]);
-            const double distance = dot(sub(I.pos, origin), outward);
-            const double depth = I.r - distance;
-            if (depth <= 0.0) continue;
-            const int slot = acquire(pwOcc, pwCur, H.pwKey + o * PW_SLOTS, H.pwVal + o * PW_SLOTS * 9,
-                                     PW_SLOTS, (uint32_t)(w + 1), bodyHigh, pwOver);
-            if (slot < 0) continue;
-            double* hv = H.pwVal + (o * PW_SLOTS + slot) * 9;
-            if (A.tape.wallCount) {
-                const unsigned e = atomicAdd(A.tape.wallCount, 1u);
-                if (A.tape.wall) {
-                    A.tape.wall[e] = make_int4((int)p, w, slot, 0);
-                    for (int k = 0; k < 9; ++k) A.tape.wallPrev[(long)e * 9 + k] = hv[k];
-                }
-            }
-            const V3 cf = pw_law(I, outward, depth, mw, hv, wF, wT);
-            if (A.tape.plateAcc && w == A.tape.plate && !A.first) A.tape.plateAcc[p] += cf.z;
-        }
+        V3 cF = zero3(), cT = zero3(), wF = zero3(), wT = zero3(), bF = zero3(), bT = zero3();
+        const V3 fF = field_force(A, p, I.mass);
+        wall_pass(A, I, p, S.k(I_HIST, p), S.k(I_SCENE, p), wF, wT, bodyHigh, pwOver);
         // particle pairs, summed in candidate order from the per-contact results
         const int cnt = A.ncount[t];
         const long base = A.off[t];
@@ -1143,45 +1327,14 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
             bF = mk(ob_[0 * n + p], ob_[1 * n + p], ob_[2 * n + p]);
             bT = mk(ob_[3 * n + p], ob_[4 * n + p], ob_[5 * n + p]);
         }
-        H.pwOcc[o] = pwOcc;
-        H.pwSeen[(long)(s & 1) * n + o] = pwCur;
-        // outputs (contact:1211-1220)
-        double* oc = A.O.contact;
-        oc[0 * n + p] = cF.x; oc[1 * n + p] = cF.y; oc[2 * n + p] = cF.z;
-        oc[3 * n + p] = cT.x; oc[4 * n + p] = cT.y; oc[5 * n + p] = cT.z;
-        double* ow = A.O.wall;
-        ow[0 * n + p] = wF.x; ow[1 * n + p] = wF.y; ow[2 * n + p] = wF.z;
-        ow[3 * n + p] = wT.x; ow[4 * n + p] = wT.y; ow[5 * n + p] = wT.z;
-        double* of = A.O.field;
-        of[0 * n + p] = fF.x; of[1 * n + p] = fF.y; of[2 * n + p] = fF.z;
-        const V3 tF = add(add(cF, fF), add(wF, bF));
-        const V3 tT = add(cT, add(wT, bT));
-        // accelerationKernel (particle:71-101)
-        const double m = I.mass;
-        S.f(F_AX, p) = tF.x / m;
-        S.f(F_AY, p) = tF.y / m;
-        S.f(F_AZ, p) = tF.z / m;
-        const double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
-        const double tx = (1.0 - 2.0 * (qy * qy + qz * qz)) * tT.x + 2.0 * (qx * qy + qw * qz) * tT.y +
-                          2.0 * (qx * qz - qw * qy) * tT.z;
-        const double ty = 2.0 * (qx * qy - qw * qz) * tT.x + (1.0 - 2.0 * (qx * qx + qz * qz)) * tT.y +
-                          2.0 * (qy * qz + qw * qx) * tT.z;
-        const double tz = 2.0 * (qx * qz + qw * qy) * tT.x + 2.0 * (qy * qz - qw * qx) * tT.y +
-                          (1.0 - 2.0 * (qx * qx + qy * qy)) * tT.z;
-        const double wx = S.f(F_WX, p), wy = S.f(F_WY, p), wz = S.f(F_WZ, p);
-        const double ix = S.f(F_IX, p), iy = S.f(F_IY, p), iz = S.f(F_IZ, p);
-        S.f(F_AWX, p) = (tx - (iz - iy) * wz * wy) / ix;
-        S.f(F_AWY, p) = (ty - (ix - iz) * wx * wz) / iy;
-        S.f(F_AWZ, p) = (tz - (iy - ix) * wy * wx) / iz;
+        owner_out(A, p, I.mass, cF, cT, fF, wF, wT, bF, bT);
     }
     if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
 }
 
-// Particle-triangle part of computeContactsKernel (contact:1148-1208): per
-// body, the area-weighted manifold over the triangles of the component's own
-// cell advances one history slot, then every triangle contributes its
-// read-only contact.  Writes the body accumulator (bF, bT) read by k_law.
+// Triangle contacts of every single-sphere owner (tri_pass).  Writes the
+// body accumulator (bF, bT) read by k_law.
 __global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
@@ -1190,110 +1343,62 @@ __global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
     int bodyHigh = 0, ptOver = 0, bcount = 0;
     if (t < n) {
         const long p = A.sorted[t];
-        const long o = S.k(I_ORIG, p);
-        const long hr = S.k(I_HIST, p);  // history row
-        const int s = A.step;
-        const History& H = A.H;
         Body I = rec_body(A.rec, p);
         if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
         V3 bF = zero3(), bT = zero3();
-        // triangles of the own cell (contact:1148-1208)
-        {
-            unsigned long long ptOcc = H.ptOcc[hr] & H.ptSeen[(long)((s + 1) & 1) * n + hr];
-            unsigned long long ptCur = 0;
-            const TriSet& T = A.T;
-            if (T.nt > 0 && T.entries > 0 && S.k(I_VALID, p)) {
-                const unsigned long long key = pack_cell(S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p));
-                const long b0 = lower_bound_u64(T.tkey, T.entries, key);
-                const long e0 = upper_bound_u64(T.tkey, T.entries, key);
-                for (int body = 0; body < A.nb; ++body) {
-                    double total = 0.0, wD = 0.0;
-                    V3 wN = zero3(), wC = zero3();
-                    for (long k = b0; k < e0; ++k) {
-                        const int tr = T.ttri[k];
-                        if (T.body[tr] != body) continue;
-                        const Tri tri = load_tri(T, tr);
-                        const double scv = tri_area_scale(I.pos, I.r, tri);
-                        if (scv <= 0.0) continue;
-                        const double distance = dot(tri.n, sub(I.pos, tri.a));
-                        const V3 cc = sub(I.pos, scl(tri.n, distance));
-                        total += scv;
-                        wN = add(wN, scl(tri.n, -scv));
-                        wC = add(wC, scl(cc, scv));
-                        wD += distance * scv;
-                    }
-                    double bl[6] = {0, 0, 0, 0, 0, 0};
-                    double fz = 0.0;
-                    if (total > 0.0) {
-                        const V3 mN = unit(wN);
-                        const V3 mC = scl(wC, 1.0 / total);
-                        const double mD = wD / total;
-                        const int slot = acquire(ptOcc, ptCur, H.ptKey + hr * PT_SLOTS,
-                                                 H.ptVal + hr * PT_SLOTS * 9, PT_SLOTS,
-                                                 (uint32_t)(body + 1), bodyHigh, ptOver);
-                        if (slot >= 0) {
-                            double* sv = H.ptVal + (hr * PT_SLOTS + slot) * 9;
-                            const Pose P = load_pose(T, body);
-                            bool slide = false, twisty = false;
-                            manifold_law(I, P, A.pt, mN, mC, mD, sv, &slide, &twisty);
-                            A.O.regime[(long)body * n + p] = 1 | (slide ? 2 : 0) | (twisty ? 4 : 0);
-                            Material ro = A.pt;
-                            ro.dt = 0.0;
-                            for (long k = b0; k < e0; ++k) {
-                                const int tr = T.ttri[k];
-                                if (T.body[tr] != body) continue;
-                                const Tri tri = load_tri(T, tr);
-                                const PtOut r = pt_law(I, tri, P, ro, sv);
-                                if (!r.hit) continue;
-                                bF = add(bF, r.pf);
-                                bT = add(bT, r.ptq);
-                                const V3 btq = add(add(r.bnt, r.bft), add(r.btw, r.brl));
-                                bl[0] += r.bnf.x + r.bfr.x;
-                                bl[1] += r.bnf.y + r.bfr.y;
-                                bl[2] += r.bnf.z + r.bfr.z;
-                                bl[3] += btq.x;
-                                bl[4] += btq.y;
-                                bl[5] += btq.z;
-                                fz += r.bnf.z + r.bfr.z;
-                                ++bcount;
-                                atomicMax(&A.sc->body_max_depth_bits,
-                                          (unsigned long long)__double_as_longlong(r.depth));
-                                if (A.O.bodyParticle) {
-                                    double* d = A.O.bodyParticle + ((long)body * n + o) * 6;
-                                    d[0] += r.pf.x; d[1] += r.pf.y; d[2] += r.pf.z;
-                                    d[3] += r.ptq.x; d[4] += r.ptq.y; d[5] += r.ptq.z;
-                                }
-                                if (A.O.bodyDetail) {
-                                    double* d = A.O.bodyDetail + ((long)body * n + o) * 19;
-                                    const V3 gr[6] = {r.bnf, r.bfr, r.bnt, r.bft, r.btw, r.brl};
-                                    for (int g = 0; g < 6; ++g) {
-                                        d[3 * g] += gr[g].x;
-                                        d[3 * g + 1] += gr[g].y;
-                                        d[3 * g + 2] += gr[g].z;
-                                    }
-                                    if (d[18] < r.depth) d[18] = r.depth;
-                                }
-                            }
-                        }
-                    }
-                    for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * n + t] = bl[k];
-                    A.O.fzPart[(long)body * n + t] = fz;
-                }
-            } else {
-                for (int body = 0; body < A.nb; ++body) {
-                    for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * n + t] = 0.0;
-                    A.O.fzPart[(long)body * n + t] = 0.0;
-                }
-            }
-            H.ptOcc[hr] = ptOcc;
-            H.ptSeen[(long)(s & 1) * n + hr] = ptCur;
-        }
+        tri_pass(A, I, p, t, S.k(I_HIST, p), S.k(I_ORIG, p), bF, bT, bodyHigh, ptOver, bcount);
         double* ob = A.O.body;
         ob[0 * n + p] = bF.x; ob[1 * n + p] = bF.y; ob[2 * n + p] = bF.z;
         ob[3 * n + p] = bT.x; ob[4 * n + p] = bT.y; ob[5 * n + p] = bT.z;
     }
     atomic_add_wave(&A.sc->body_contacts, bcount);
     atomic_max_wave(&A.sc->body_high, bodyHigh);
+    atomic_add_wave(&A.sc->pt_overflow, ptOver);
+}
+
+// computeContactsKernel (contact:1065-1221) for owners made of several
+// sphere components (clumps): one thread per owner visits its components in
+// (owner, id) order, each with its walls, its particle contacts (the
+// per-contact results of k_pairs, in candidate order) and its triangles;
+// each kind of contact is summed in one accumulator across the components,
+// as the reference's per-owner loop does.  Then the acceleration
+// (particle:71-101).  Owners keep their input rows (no reordering).
+__global__ void __launch_bounds__(BLOCK) k_law_clump(LawArgs A)
+{
+    const long i = (long)blockIdx.x * BLOCK + threadIdx.x;
+    const Soa& S = A.S;
+    const Soa& R = A.R;
+    int bodyHigh = 0, pwOver = 0, ptOver = 0, bcount = 0;
+    if (i < S.n) {
+        const double mass = S.f(F_MASS, i);
+        V3 cF = zero3(), cT = zero3(), wF = zero3(), wT = zero3(), bF = zero3(), bT = zero3();
+        const V3 fF = field_force(A, i, mass);
+        for (long c = A.ownerStart[i]; c < A.ownerStart[i + 1]; ++c) {
+            const long t = A.rank[c];
+            Body I = rec_body(A.rec, c);
+            I.off = mk(R.f(F_OFX, c), R.f(F_OFY, c), R.f(F_OFZ, c));
+            const long hr = R.k(I_HIST, c);
+            wall_pass(A, I, c, hr, R.k(I_SCENE, c), wF, wT, bodyHigh, pwOver);
+            const int cnt = A.ncount[t];
+            const long base = A.off[t];
+            for (int k = 0; k < cnt; ++k) {
+                const double* pf = A.pairF + (base + k) * 6;
+                cF = add(cF, mk(pf[0], pf[1], pf[2]));
+                cT = add(cT, mk(pf[3], pf[4], pf[5]));
+            }
+            if (A.nb > 0) tri_pass(A, I, c, t, hr, i, bF, bT, bodyHigh, ptOver, bcount);
+        }
+        if (A.nb > 0) {
+            double* ob = A.O.body;
+            const long n = S.n;
+            ob[0 * n + i] = bF.x; ob[1 * n + i] = bF.y; ob[2 * n + i] = bF.z;
+            ob[3 * n + i] = bT.x; ob[4 * n + i] = bT.y; ob[5 * n + i] = bT.z;
+        }
+        owner_out(A, i, mass, cF, cT, fF, wF, wT, bF, bT);
+    }
+    atomic_add_wave(&A.sc->body_contacts, bcount);
+    if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
+    atomic_add_wave(&A.sc->pw_overflow, pwOver);
     atomic_add_wave(&A.sc->pt_overflow, ptOver);
 }
 
@@ -1727,6 +1832,16 @@ struct dem_solver {
     size_t freeBefore = 0;
     bool restored = false;
     bool legacySphere = false;  // one zero-offset component per owner, id = owner id
+    // clumps (an owner with several sphere components): the contact pipeline
+    // runs over component rows [nc] that carry copies of their owner's state
+    bool clump = false;
+    long nc = 0;                 // contact rows: components (== n without clumps)
+    double* cState = nullptr;    // [NDF][nc] component rows
+    int* ciState = nullptr;      // [NIF][nc]
+    int* cOwner = nullptr;       // [nc] owner row of each component
+    int* ownerStart = nullptr;   // [n + 1] first component of each owner
+    int* cRank = nullptr;        // [nc] sorted position of each component
+    int* ownS = nullptr;         // [nc] owner of the t-th sorted component
 
     // device memory
     double* dState[2] = {nullptr, nullptr};
@@ -1808,10 +1923,13 @@ struct dem_solver {
     hipEvent_t ev[NEV] = {};
     bool evReady = false;
     // metadata for checkpoints
-    std::vector<int> compIds, ownerIds;
+    std::vector<int> compIds, ownerIds;  // per component: id, owner id
+    std::vector<int> pids;               // per owner: id
     std::vector<double> compR, compB;
 
     Soa soa() const { return Soa{dState[cur], iState[cur], n}; }
+    // the rows the grid, candidate scan and pair law run over
+    Soa rows() const { return clump ? Soa{cState, ciState, nc} : soa(); }
 
     ~dem_solver()
     {
@@ -1827,7 +1945,7 @@ struct dem_solver {
                         O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, tLocal, tWorld, tAabb,
                         pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
                         regimeInstant, regimeInterval, microInstant, microInterval, bodyResult,
-                        redPartial, dl};
+                        redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS};
         for (void* p : ptrs)
             if (p) hipFree(p);
         if (hPose) hipHostFree(hPose);
@@ -2007,11 +2125,15 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     for (long i = 0; i < n; ++i)
         if (perOwner[i] < 1 || perOwner[i] > 8)
             return set_error(DEM_E_INVALID, "each CUDA owner requires 1 to 8 components");
-    if (C->count != n)
-        return set_error(DEM_E_UNSUPPORTED,
-                         "multi-sphere clumps (more than one component per owner) are not supported by this build");
-    if (n > (long)INT_MAX / PP_SLOTS)
+    // several components on some owner: the contact pipeline runs over component rows
+    const long nc = C->count;
+    const bool clump = nc != n;
+    if (clump && nsc > 1) return set_error(DEM_E_UNSUPPORTED, "batched scenes cannot carry clumps");
+    if (nc > MAX_PARTICLES)
+        return set_error(DEM_E_CAPACITY, "component count exceeds the contact-list row range (2^25 - 1)");
+    if (nc > (long)INT_MAX / PP_SLOTS)
         return set_error(DEM_E_CAPACITY, "particle count exceeds CUDA contact-history index capacity");
+    const long nr = nc;  // contact rows
     if (!dem_backend_available()) return set_error(DEM_E_DEVICE, "no CUDA device is available");
 
     HIP_TRY(hipSetDevice(0), "hipSetDevice");
@@ -2022,6 +2144,8 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
 
     s->n = n;
+    s->nc = nc;
+    s->clump = clump;
     s->nscenes = nsc;
     const double scale = ctl->mesh_scale > 0.0 ? ctl->mesh_scale : 2.5;
     s->maxRadius = maxr;
@@ -2036,7 +2160,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         SceneParams& q = s->hsp[k];
         q.inv = 1.0 / (scale * maxrScene[k]);
         q.row0 = spec.row[k];
-        q.row1 = spec.row[k + 1];
+        q.row1 = clump ? nc : spec.row[k + 1];  // grid rows of the scene
         q.wall0 = spec.wall[k];
         q.nwall = spec.wall[k + 1] - spec.wall[k];
         hpp[k] = make_material(spec.pp[k], ctl->dt);
@@ -2048,22 +2172,33 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     s->keepBodyParticle = ctl->keep_body_particle != 0;
     s->keepBodyDetail = ctl->keep_body_detail != 0;
 
-    // host SoA in input row order; component k of owner row o sits at row o
+    // components in (owner, id) order; without clumps component c belongs to
+    // owner row c and its fields live in the owner's row
+    s->compIds.assign(nc, 0); s->ownerIds.assign(nc, 0); s->compR.assign(nc, 0.0); s->compB.assign(3 * nc, 0.0);
+    s->pids.assign(P->id, P->id + n);
+    std::vector<int> cown(nc), ostart(n + 1, 0);
+    for (long c = 0; c < nc; ++c) {
+        const int k = order[c];
+        const int ow = C->owner[k];
+        cown[c] = ow;
+        ++ostart[ow + 1];
+        s->compIds[c] = C->id[k]; s->ownerIds[c] = P->id[ow]; s->compR[c] = C->radius[k];
+        for (int a = 0; a < 3; ++a) s->compB[3 * c + a] = C->pos_body[3 * k + a];
+        if (s->compB[3 * c] != 0.0 || s->compB[3 * c + 1] != 0.0 || s->compB[3 * c + 2] != 0.0) s->offs = true;
+    }
+    for (long i = 0; i < n; ++i) ostart[i + 1] += ostart[i];
+    // host SoA in input row order
     std::vector<double> hd((size_t)NDF * n, 0.0);
     std::vector<int> hi((size_t)NIF * n, 0);
     auto D = [&](int f, long p) -> double& { return hd[(size_t)f * n + p]; };
     auto Iv = [&](int f, long p) -> int& { return hi[(size_t)f * n + p]; };
-    s->compIds.assign(n, 0); s->ownerIds.assign(n, 0); s->compR.assign(n, 0.0); s->compB.assign(3 * n, 0.0);
-    for (int k : order) {
-        const long p = C->owner[k];
-        D(F_CR, p) = C->radius[k];
-        D(F_BX, p) = C->pos_body[3 * k]; D(F_BY, p) = C->pos_body[3 * k + 1]; D(F_BZ, p) = C->pos_body[3 * k + 2];
-        Iv(I_CID, p) = C->id[k];
-        if (D(F_BX, p) != 0.0 || D(F_BY, p) != 0.0 || D(F_BZ, p) != 0.0) s->offs = true;
-        s->compIds[p] = C->id[k]; s->ownerIds[p] = P->id[p]; s->compR[p] = C->radius[k];
-        s->compB[3 * p] = D(F_BX, p); s->compB[3 * p + 1] = D(F_BY, p); s->compB[3 * p + 2] = D(F_BZ, p);
-    }
-    s->legacySphere = C->count == n;
+    if (!clump)
+        for (long p = 0; p < n; ++p) {
+            D(F_CR, p) = s->compR[p];
+            D(F_BX, p) = s->compB[3 * p]; D(F_BY, p) = s->compB[3 * p + 1]; D(F_BZ, p) = s->compB[3 * p + 2];
+            Iv(I_CID, p) = s->compIds[p];
+        }
+    s->legacySphere = !clump;
     for (long p = 0; p < n && s->legacySphere; ++p)
         s->legacySphere = s->compIds[p] == P->id[p] && s->compR[p] == P->radius[p] && s->compB[3 * p] == 0.0 &&
                           s->compB[3 * p + 1] == 0.0 && s->compB[3 * p + 2] == 0.0;
@@ -2090,22 +2225,49 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipMemcpy(s->iState[0], hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice),
             "upload particle ints");
     s->cur = 0;
+    if (clump) {
+        // component rows: geometry and ids here, owner state copied each step
+        std::vector<double> cd((size_t)NDF * nc, 0.0);
+        std::vector<int> ci((size_t)NIF * nc, 0);
+        for (long c = 0; c < nc; ++c) {
+            cd[(size_t)F_CR * nc + c] = s->compR[c];
+            cd[(size_t)F_BX * nc + c] = s->compB[3 * c];
+            cd[(size_t)F_BY * nc + c] = s->compB[3 * c + 1];
+            cd[(size_t)F_BZ * nc + c] = s->compB[3 * c + 2];
+            ci[(size_t)I_CID * nc + c] = s->compIds[c];
+            ci[(size_t)I_PID * nc + c] = s->ownerIds[c];
+            ci[(size_t)I_ORIG * nc + c] = (int)c;  // grid ties ordered by component index
+            ci[(size_t)I_HIST * nc + c] = (int)c;
+            ci[(size_t)I_SCENE * nc + c] = 0;
+        }
+        TRY(dalloc(s->cState, cd.size(), "hipMalloc(component rows)"));
+        TRY(dalloc(s->ciState, ci.size(), "hipMalloc(component rows)"));
+        TRY(dalloc(s->cOwner, nc, "hipMalloc(component owners)"));
+        TRY(dalloc(s->ownerStart, n + 1, "hipMalloc(component owners)"));
+        TRY(dalloc(s->cRank, nc, "hipMalloc(component ranks)"));
+        TRY(dalloc(s->ownS, nc, "hipMalloc(component owners)"));
+        HIP_TRY(hipMemcpy(s->cState, cd.data(), sizeof(double) * cd.size(), hipMemcpyHostToDevice), "upload components");
+        HIP_TRY(hipMemcpy(s->ciState, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice), "upload components");
+        HIP_TRY(hipMemcpy(s->cOwner, cown.data(), sizeof(int) * nc, hipMemcpyHostToDevice), "upload components");
+        HIP_TRY(hipMemcpy(s->ownerStart, ostart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice),
+                "upload components");
+    }
     TRY(dalloc(s->sc, 1, "hipMalloc(step scalars)"));
     HIP_TRY(hipMemset(s->sc, 0, sizeof(Scalars)), "clear step scalars");
-    TRY(dalloc(s->keyA, n, "hipMalloc(grid keys)"));
-    TRY(dalloc(s->keyB, n, "hipMalloc(grid keys)"));
-    TRY(dalloc(s->valA, n, "hipMalloc(grid ids)"));
-    TRY(dalloc(s->valB, n, "hipMalloc(grid ids)"));
-    s->rsBlocks = (int)((n + RS_TILE - 1) / RS_TILE);
+    TRY(dalloc(s->keyA, nr, "hipMalloc(grid keys)"));
+    TRY(dalloc(s->keyB, nr, "hipMalloc(grid keys)"));
+    TRY(dalloc(s->valA, nr, "hipMalloc(grid ids)"));
+    TRY(dalloc(s->valB, nr, "hipMalloc(grid ids)"));
+    s->rsBlocks = (int)((nr + RS_TILE - 1) / RS_TILE);
     TRY(dalloc(s->rsCounts, (size_t)256 * s->rsBlocks, "hipMalloc(sort counts)"));
     TRY(dalloc(s->rsDigits, 512, "hipMalloc(sort digit totals)"));
-    s->cellCap = (unsigned long long)std::max<long>(8 * n + 64, 1 << 20);
+    s->cellCap = (unsigned long long)std::max<long>(8 * nr + 64, 1 << 20);
     TRY(dalloc(s->cellStart, s->cellCap, "hipMalloc(cell table)"));
-    TRY(dalloc(s->posr, n, "hipMalloc(packed positions)"));
-    TRY(dalloc(s->cidS, n, "hipMalloc(sorted component ids)"));
-    TRY(dalloc(s->rec, n, "hipMalloc(contact records)"));
-    TRY(dalloc(s->boxPart, (size_t)6 * blocks_for(n), "hipMalloc(grid box partials)"));
-    TRY(dalloc(s->partScene, blocks_for(n), "hipMalloc(grid box partials)"));
+    TRY(dalloc(s->posr, nr, "hipMalloc(packed positions)"));
+    TRY(dalloc(s->cidS, nr, "hipMalloc(sorted component ids)"));
+    TRY(dalloc(s->rec, nr, "hipMalloc(contact records)"));
+    TRY(dalloc(s->boxPart, (size_t)6 * blocks_for(nr), "hipMalloc(grid box partials)"));
+    TRY(dalloc(s->partScene, blocks_for(nr), "hipMalloc(grid box partials)"));
     TRY(dalloc(s->sceneAcc, (size_t)6 * nsc, "hipMalloc(scene boxes)"));
     TRY(dalloc(s->gridDone, 1, "hipMalloc(grid counter)"));
     HIP_TRY(hipMemset(s->gridDone, 0, sizeof(unsigned)), "clear grid counter");
@@ -2116,12 +2278,12 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipMemcpy(s->dSp, s->hsp.data(), sizeof(SceneParams) * nsc, hipMemcpyHostToDevice), "upload scenes");
     HIP_TRY(hipMemcpy(s->dMatPP, hpp.data(), sizeof(Material) * nsc, hipMemcpyHostToDevice), "upload scenes");
     HIP_TRY(hipMemcpy(s->dMatPW, hpw.data(), sizeof(Material) * nsc, hipMemcpyHostToDevice), "upload scenes");
-    TRY(dalloc(s->nbr, (size_t)MAX_LIST * n, "hipMalloc(contact lists)"));
-    TRY(dalloc(s->ncount, n, "hipMalloc(contact counts)"));
-    TRY(dalloc(s->listOff, n + 1, "hipMalloc(contact list offsets)"));
-    TRY(dalloc(s->listBlk, blocks_for(n), "hipMalloc(contact list offsets)"));
-    TRY(dalloc(s->pairs, (size_t)MAX_LIST * n, "hipMalloc(compacted contacts)"));
-    TRY(dalloc(s->pairF, (size_t)6 * MAX_LIST * n, "hipMalloc(contact forces)"));
+    TRY(dalloc(s->nbr, (size_t)MAX_LIST * nr, "hipMalloc(contact lists)"));
+    TRY(dalloc(s->ncount, nr, "hipMalloc(contact counts)"));
+    TRY(dalloc(s->listOff, nr + 1, "hipMalloc(contact list offsets)"));
+    TRY(dalloc(s->listBlk, blocks_for(nr), "hipMalloc(contact list offsets)"));
+    TRY(dalloc(s->pairs, (size_t)MAX_LIST * nr, "hipMalloc(compacted contacts)"));
+    TRY(dalloc(s->pairF, (size_t)6 * MAX_LIST * nr, "hipMalloc(contact forces)"));
     {
         // candidate lists (pure DEM): skin = DEM_SKIN (default 0.2) x the smallest
         // scene's largest radius; the +-2-cell rebuild scan must cover r_i + r_j + skin
@@ -2130,7 +2292,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         double rmin = maxrScene[0];
         for (int k = 1; k < nsc; ++k) rmin = std::min(rmin, maxrScene[k]);
         s->skin = frac * rmin;
-        bool ok = frac > 0.0 && (!B || B->count == 0);
+        bool ok = frac > 0.0 && (!B || B->count == 0) && !clump;
         for (int k = 0; k < nsc && ok; ++k) ok = 2.0 / s->hsp[k].inv >= 2.0 * maxrScene[k] + s->skin;
         if (ok) {
             TRY(dalloc(s->refC, (size_t)3 * n, "hipMalloc(candidate lists)"));
@@ -2145,29 +2307,29 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     }
     TRY(dalloc(s->byorig, n, "hipMalloc(row index)"));
     // history
-    TRY(dalloc(s->H.ppOcc, n, "hipMalloc(particle history)"));
-    TRY(dalloc(s->H.ppSeen, 2 * n, "hipMalloc(particle history)"));
-    TRY(dalloc(s->H.ppKey, (size_t)PP_SLOTS * n, "hipMalloc(particle history keys)"));
-    TRY(dalloc(s->H.ppVal, (size_t)PP_SLOTS * 9 * n, "hipMalloc(particle history values)"));
-    TRY(dalloc(s->H.pwOcc, n, "hipMalloc(wall history)"));
-    TRY(dalloc(s->H.pwSeen, 2 * n, "hipMalloc(wall history)"));
-    TRY(dalloc(s->H.pwKey, (size_t)PW_SLOTS * n, "hipMalloc(wall history keys)"));
-    TRY(dalloc(s->H.pwVal, (size_t)PW_SLOTS * 9 * n, "hipMalloc(wall history values)"));
-    HIP_TRY(hipMemset(s->H.ppOcc, 0, sizeof(unsigned long long) * n), "clear history");
-    HIP_TRY(hipMemset(s->H.ppSeen, 0, sizeof(unsigned long long) * 2 * n), "clear history");
-    HIP_TRY(hipMemset(s->H.pwOcc, 0, sizeof(unsigned) * n), "clear history");
-    HIP_TRY(hipMemset(s->H.pwSeen, 0, sizeof(unsigned) * 2 * n), "clear history");
-    HIP_TRY(hipMemset(s->H.ppKey, 0, sizeof(uint32_t) * PP_SLOTS * n), "clear history");
-    HIP_TRY(hipMemset(s->H.pwKey, 0, sizeof(uint32_t) * PW_SLOTS * n), "clear history");
+    TRY(dalloc(s->H.ppOcc, nr, "hipMalloc(particle history)"));
+    TRY(dalloc(s->H.ppSeen, 2 * nr, "hipMalloc(particle history)"));
+    TRY(dalloc(s->H.ppKey, (size_t)PP_SLOTS * nr, "hipMalloc(particle history keys)"));
+    TRY(dalloc(s->H.ppVal, (size_t)PP_SLOTS * 9 * nr, "hipMalloc(particle history values)"));
+    TRY(dalloc(s->H.pwOcc, nr, "hipMalloc(wall history)"));
+    TRY(dalloc(s->H.pwSeen, 2 * nr, "hipMalloc(wall history)"));
+    TRY(dalloc(s->H.pwKey, (size_t)PW_SLOTS * nr, "hipMalloc(wall history keys)"));
+    TRY(dalloc(s->H.pwVal, (size_t)PW_SLOTS * 9 * nr, "hipMalloc(wall history values)"));
+    HIP_TRY(hipMemset(s->H.ppOcc, 0, sizeof(unsigned long long) * nr), "clear history");
+    HIP_TRY(hipMemset(s->H.ppSeen, 0, sizeof(unsigned long long) * 2 * nr), "clear history");
+    HIP_TRY(hipMemset(s->H.pwOcc, 0, sizeof(unsigned) * nr), "clear history");
+    HIP_TRY(hipMemset(s->H.pwSeen, 0, sizeof(unsigned) * 2 * nr), "clear history");
+    HIP_TRY(hipMemset(s->H.ppKey, 0, sizeof(uint32_t) * PP_SLOTS * nr), "clear history");
+    HIP_TRY(hipMemset(s->H.pwKey, 0, sizeof(uint32_t) * PW_SLOTS * nr), "clear history");
     if (s->verlet) {
-        TRY(dalloc(s->H2.ppOcc, n, "hipMalloc(particle history)"));
-        TRY(dalloc(s->H2.ppSeen, 2 * n, "hipMalloc(particle history)"));
-        TRY(dalloc(s->H2.ppKey, (size_t)PP_SLOTS * n, "hipMalloc(particle history keys)"));
-        TRY(dalloc(s->H2.ppVal, (size_t)PP_SLOTS * 9 * n, "hipMalloc(particle history values)"));
-        TRY(dalloc(s->H2.pwOcc, n, "hipMalloc(wall history)"));
-        TRY(dalloc(s->H2.pwSeen, 2 * n, "hipMalloc(wall history)"));
-        TRY(dalloc(s->H2.pwKey, (size_t)PW_SLOTS * n, "hipMalloc(wall history keys)"));
-        TRY(dalloc(s->H2.pwVal, (size_t)PW_SLOTS * 9 * n, "hipMalloc(wall history values)"));
+        TRY(dalloc(s->H2.ppOcc, nr, "hipMalloc(particle history)"));
+        TRY(dalloc(s->H2.ppSeen, 2 * nr, "hipMalloc(particle history)"));
+        TRY(dalloc(s->H2.ppKey, (size_t)PP_SLOTS * nr, "hipMalloc(particle history keys)"));
+        TRY(dalloc(s->H2.ppVal, (size_t)PP_SLOTS * 9 * nr, "hipMalloc(particle history values)"));
+        TRY(dalloc(s->H2.pwOcc, nr, "hipMalloc(wall history)"));
+        TRY(dalloc(s->H2.pwSeen, 2 * nr, "hipMalloc(wall history)"));
+        TRY(dalloc(s->H2.pwKey, (size_t)PW_SLOTS * nr, "hipMalloc(wall history keys)"));
+        TRY(dalloc(s->H2.pwVal, (size_t)PW_SLOTS * 9 * nr, "hipMalloc(wall history values)"));
     }
     // walls
     s->nw = W ? W->count : 0;
@@ -2196,13 +2358,13 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     s->nt = (int)ntl;
     if (s->nb > 0) {
         const int nb = s->nb;
-        TRY(dalloc(s->H.ptOcc, n, "hipMalloc(body history)"));
-        TRY(dalloc(s->H.ptSeen, 2 * n, "hipMalloc(body history)"));
-        TRY(dalloc(s->H.ptKey, (size_t)PT_SLOTS * n, "hipMalloc(body history keys)"));
-        TRY(dalloc(s->H.ptVal, (size_t)PT_SLOTS * 9 * n, "hipMalloc(body history values)"));
-        HIP_TRY(hipMemset(s->H.ptOcc, 0, sizeof(unsigned long long) * n), "clear history");
-        HIP_TRY(hipMemset(s->H.ptSeen, 0, sizeof(unsigned long long) * 2 * n), "clear history");
-        HIP_TRY(hipMemset(s->H.ptKey, 0, sizeof(uint32_t) * PT_SLOTS * n), "clear history");
+        TRY(dalloc(s->H.ptOcc, nr, "hipMalloc(body history)"));
+        TRY(dalloc(s->H.ptSeen, 2 * nr, "hipMalloc(body history)"));
+        TRY(dalloc(s->H.ptKey, (size_t)PT_SLOTS * nr, "hipMalloc(body history keys)"));
+        TRY(dalloc(s->H.ptVal, (size_t)PT_SLOTS * 9 * nr, "hipMalloc(body history values)"));
+        HIP_TRY(hipMemset(s->H.ptOcc, 0, sizeof(unsigned long long) * nr), "clear history");
+        HIP_TRY(hipMemset(s->H.ptSeen, 0, sizeof(unsigned long long) * 2 * nr), "clear history");
+        HIP_TRY(hipMemset(s->H.ptKey, 0, sizeof(uint32_t) * PT_SLOTS * nr), "clear history");
         TRY(dalloc(s->pose, (size_t)13 * nb, "hipMalloc(body poses)"));
         HIP_TRY(hipMemcpy(s->pose, B->pose13, sizeof(double) * 13 * nb, hipMemcpyHostToDevice), "upload body poses");
         std::vector<int> tb(s->nt);
@@ -2229,11 +2391,11 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         HIP_TRY(hipMemset(s->regimeInterval, 0, sizeof(unsigned long long) * 7 * nb), "clear regime");
         HIP_TRY(hipMemset(s->microInterval, 0, sizeof(double) * 22 * nb), "clear microstate");
         TRY(dalloc(s->bodyResult, (size_t)7 * nb, "hipMalloc(body resultant)"));
-        TRY(dalloc(s->O.bodyPart, (size_t)6 * nb * n, "hipMalloc(body load partials)"));
-        TRY(dalloc(s->O.fzPart, (size_t)nb * n, "hipMalloc(body load partials)"));
-        TRY(dalloc(s->O.regime, (size_t)nb * n, "hipMalloc(body regime state)"));
-        TRY(dalloc(s->O.micro, (size_t)22 * nb * n, "hipMalloc(body microstate)"));
-        s->redBlocks = (int)((n + RED_TILE - 1) / RED_TILE);
+        TRY(dalloc(s->O.bodyPart, (size_t)6 * nb * nr, "hipMalloc(body load partials)"));
+        TRY(dalloc(s->O.fzPart, (size_t)nb * nr, "hipMalloc(body load partials)"));
+        TRY(dalloc(s->O.regime, (size_t)nb * nr, "hipMalloc(body regime state)"));
+        TRY(dalloc(s->O.micro, (size_t)22 * nb * nr, "hipMalloc(body microstate)"));
+        s->redBlocks = (int)((nr + RED_TILE - 1) / RED_TILE);
         TRY(dalloc(s->redPartial, (size_t)22 * nb * s->redBlocks, "hipMalloc(reduction partials)"));
         if (s->keepBodyParticle) TRY(dalloc(s->O.bodyParticle, (size_t)nb * n * 6, "hipMalloc(body-particle forces)"));
         if (s->keepBodyDetail) TRY(dalloc(s->O.bodyDetail, (size_t)nb * n * 19, "hipMalloc(body contact details)"));
@@ -2256,6 +2418,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipDeviceSynchronize(), "initialize");
     const char* env = getenv("DEM_REORDER_INTERVAL");
     if (env) s->reorderInterval = std::max(0, atoi(env));
+    if (clump) s->reorderInterval = 0;  // owners keep their rows (k_law_clump indexes them directly)
     s->initialized = true;
     return DEM_OK;
 }
@@ -2365,11 +2528,14 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     }
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[0], st), "event");
     hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
-    Soa S = s->soa();
+    Soa S = s->soa();   // owners
+    Soa R = s->rows();  // contact rows (the owners, or their components with clumps)
+    const long nr = R.n;
     // candidate lists: only pure-DEM steps (bodies need the per-step cell order)
     const bool verlet = s->verlet && nb == 0;
     const bool lists = verlet && s->vlistValid;
-    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dSp, s->offs, s->sc,
+    if (s->clump) hipLaunchKernelGGL(k_expand, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, R, s->cOwner);
+    hipLaunchKernelGGL(k_cells, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dSp, s->offs, s->sc,
                        s->boxPart, s->partScene, s->sceneAcc, lists ? s->refC : nullptr,
                        0.25 * s->skin * s->skin);
     bool rebuild = true;
@@ -2383,32 +2549,34 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (rebuild) {
         hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,
                            s->partScene, s->sceneAcc, s->dGrid, s->nscenes, s->cellCap, s->gridDone);
-        hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dGrid, s->keyA, s->valA);
+        hipLaunchKernelGGL(k_keys, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dGrid, s->keyA, s->valA);
         TRY(launch_check("grid keys"));
-        TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits,
+        TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, nr, s->rsCounts, s->rsBlocks, &s->sc->nbits,
                        s->rsDigits));
-        hipLaunchKernelGGL(k_fixup, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->keyA, s->valA, n,
-                           S.i + (long)I_ORIG * n);
+        hipLaunchKernelGGL(k_fixup, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, s->keyA, s->valA, nr,
+                           R.i + (long)I_ORIG * nr);
         // with candidate lists the rows are put in cell order at every rebuild
-        const bool reorder = verlet || (s->reorderInterval > 0 && (s->stepIndex % s->reorderInterval) == 0);
+        const bool reorder = !s->clump && (verlet || (s->reorderInterval > 0 &&
+                                                      (s->stepIndex % s->reorderInterval) == 0));
         if (reorder) {
             Soa D{s->dState[1 - s->cur], s->iState[1 - s->cur], n};
             hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valA);
             hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->valA, n);
             s->cur = 1 - s->cur;
             S = s->soa();
+            R = S;
             if (verlet && s->H2.ppOcc) {
                 hipLaunchKernelGGL(k_hist_move, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->H, s->H2);
                 std::swap(s->H, s->H2);
             }
         }
         const int cellBlocks = (int)std::min<unsigned long long>((s->cellCap + BLOCK - 1) / BLOCK, 65535ull);
-        hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, n, s->sc, s->cellStart,
+        hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, nr, s->sc, s->cellStart,
                            s->cellCap);
     }
-    hipLaunchKernelGGL(k_posr, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->offs, s->posr, s->cidS,
-                       verlet ? s->meta : nullptr);
-    hipLaunchKernelGGL(k_pack, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->offs, s->rec);
+    hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr, s->cidS,
+                       verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank);
+    hipLaunchKernelGGL(k_pack, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->offs, s->rec);
     TRY(launch_check("grid build"));
     bool useLists = verlet;
     if (verlet && rebuild) {
@@ -2433,29 +2601,33 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (s->nt > 0) TRY(build_triangle_grid(s));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
     const int step = s->stepIndex++;
-    const int nblk = blocks_for(n);
+    const int nblk = blocks_for(nr);
     // the reference reads its per-step counters only on diagnostics steps
     const bool counters = collect || s->countAlways;
     if (useLists)
         hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta, s->cidS,
                            s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->nkey, s->ncount, s->tape.ppOcc,
                            counters);
-    else
-        hipLaunchKernelGGL(k_scan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->cidS, s->keyA,
+    else if (s->clump)
+        hipLaunchKernelGGL(k_scan<true>, dim3(nblk), dim3(BLOCK), 0, st, R, s->valA, s->posr, s->cidS, s->keyA,
                            s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
-                       counters);
-    hipLaunchKernelGGL(k_off_blocks, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk);
+                           counters, (const int*)s->ownS);
+    else
+        hipLaunchKernelGGL(k_scan<false>, dim3(nblk), dim3(BLOCK), 0, st, R, s->valA, s->posr, s->cidS, s->keyA,
+                           s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
+                           counters, (const int*)nullptr);
+    hipLaunchKernelGGL(k_off_blocks, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, nr, s->listBlk);
     hipLaunchKernelGGL(k_off_scan, dim3(1), dim3(BLOCK), 0, st, s->listBlk, nblk, s->sc);
-    hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, n, s->listBlk, s->nbr,
+    hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, nr, s->listBlk, s->nbr,
                        s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[2], st), "event");
-    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, S, s->valA, s->rec, s->pairs, s->H,
+    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, s->valA, s->rec, s->pairs, s->H,
                        s->dMatPP, step == 0, s->offs, s->sc, s->pairF, s->tape.pair, s->tape.pairPrev);
     TRY(launch_check("particle contact law"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
-    A.S = S; A.rec = s->rec; A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
+    A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.rec = s->rec; A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
     A.pairF = s->pairF; A.H = s->H;
     A.W = Walls{s->nw, s->wallBuf, s->wallBuf + 3 * s->nw};
     A.T = triset(s); A.O = s->O; A.sc = s->sc;
@@ -2467,34 +2639,38 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (s->universal) hipLaunchKernelGGL(k_byorig, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->byorig);
     if (nb > 0) {
         const int fb = 1024;
-        hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(BLOCK), 0, st, (unsigned*)s->O.regime, (long)nb * n);
+        hipLaunchKernelGGL(k_fill_u32, dim3(fb), dim3(BLOCK), 0, st, (unsigned*)s->O.regime, (long)nb * nr);
         if (s->O.bodyParticle) hipLaunchKernelGGL(k_fill, dim3(fb), dim3(BLOCK), 0, st, s->O.bodyParticle, (long)nb * n * 6, 0.0);
         if (s->O.bodyDetail) hipLaunchKernelGGL(k_fill, dim3(fb), dim3(BLOCK), 0, st, s->O.bodyDetail, (long)nb * n * 19, 0.0);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(BLOCK), 0, st, s->regimeInstant, (long)7 * nb);
     }
-    if (nb > 0) {
-        hipLaunchKernelGGL(k_law_pt, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
-        TRY(launch_check("particle-triangle contact law"));
+    if (s->clump) {
+        hipLaunchKernelGGL(k_law_clump, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+    } else {
+        if (nb > 0) {
+            hipLaunchKernelGGL(k_law_pt, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+            TRY(launch_check("particle-triangle contact law"));
+        }
+        hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     }
-    hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     TRY(launch_check("contact law"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[4], st), "event");
     const bool interval = A.pt.dt > 0.0;
     if (nb > 0) {
-        hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, S, s->valA, s->H,
+        hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)nb * nr)), dim3(BLOCK), 0, st, R, s->valA, s->H,
                            A.T, s->O, A.pt, step, s->offs);
-        hipLaunchKernelGGL(k_regime, dim3(blocks_for((long)nb * n)), dim3(BLOCK), 0, st, s->O.regime, n, nb,
+        hipLaunchKernelGGL(k_regime, dim3(blocks_for((long)nb * nr)), dim3(BLOCK), 0, st, s->O.regime, nr, nb,
                            s->regimeInstant, s->regimeInterval, interval ? 1 : 0);
         // body loads [nb*6] then fz [nb] (rows laid out contiguously)
-        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 6 * nb), dim3(BLOCK), 0, st, s->O.bodyPart, n,
+        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 6 * nb), dim3(BLOCK), 0, st, s->O.bodyPart, nr,
                            6 * nb, s->redPartial, s->redBlocks);
         hipLaunchKernelGGL(k_reduce_final, dim3((6 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 6 * nb, s->redBlocks,
                            s->bodyResult, 0);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, nb), dim3(BLOCK), 0, st, s->O.fzPart, n, nb,
+        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, nb), dim3(BLOCK), 0, st, s->O.fzPart, nr, nb,
                            s->redPartial, s->redBlocks);
         hipLaunchKernelGGL(k_reduce_final, dim3((nb + 255) / 256), dim3(256), 0, st, s->redPartial, nb, s->redBlocks,
                            s->bodyResult + 6 * nb, 0);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 22 * nb), dim3(BLOCK), 0, st, s->O.micro, n,
+        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 22 * nb), dim3(BLOCK), 0, st, s->O.micro, nr,
                            22 * nb, s->redPartial, s->redBlocks);
         hipLaunchKernelGGL(k_reduce_final, dim3((22 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 22 * nb, s->redBlocks,
                            s->microInstant, 0);
@@ -2582,6 +2758,12 @@ namespace {
 // when candidate-list rebuilds reorder them).
 int history_rows(const dem_solver* s, std::vector<int>& hrow)
 {
+    if (s->clump) {
+        // component rows never move: the row is the component index
+        hrow.resize(s->nc);
+        std::iota(hrow.begin(), hrow.end(), 0);
+        return DEM_OK;
+    }
     const long n = s->n;
     std::vector<int> orig(n), hist(n);
     HIP_TRY(hipMemcpy(orig.data(), s->iState[s->cur] + (size_t)I_ORIG * n, sizeof(int) * n, hipMemcpyDeviceToHost),
@@ -2717,7 +2899,7 @@ int dem_get_step_stats(const dem_solver* s, dem_step_stats* stats, uint64_t* ri,
 int dem_particle_grid_entry_count(const dem_solver* s, size_t* out)
 {
     if (!s || !out) return set_error(DEM_E_INVALID, "null argument");
-    *out = (size_t)s->n;
+    *out = (size_t)s->nc;
     return DEM_OK;
 }
 
@@ -2740,12 +2922,12 @@ int dem_device_memory_bytes(const dem_solver* s, size_t* out)
 int dem_debug_grid(const dem_solver* s, uint64_t* keys, int32_t* ids)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
-    const long n = s->n;
+    const long n = s->nc;
     unsigned long long* dk = nullptr;
     int* di = nullptr;
     TRY(dalloc(dk, n, "hipMalloc"));
     TRY(dalloc(di, n, "hipMalloc"));
-    hipLaunchKernelGGL(k_debug_grid, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), s->valA, dk, di);
+    hipLaunchKernelGGL(k_debug_grid, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->rows(), s->valA, dk, di);
     HIP_TRY(hipStreamSynchronize(s->stream), "debug grid");
     std::vector<uint64_t> hk(n);
     std::vector<int32_t> hi(n);
@@ -2772,7 +2954,7 @@ int dem_debug_history(const dem_solver* s, uint64_t* pk, double* pv, uint64_t* w
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     HIP_TRY(hipStreamSynchronize(s->stream), "debug history");
-    const long n = s->n;
+    const long n = s->nc;
     std::vector<int> hrow;
     TRY(history_rows(s, hrow));
     struct Tab { const void* occ; bool wide; const uint32_t* key; const double* val; int slots; uint64_t* ok; double* ov; };
@@ -2892,7 +3074,7 @@ int dem_load_contact_history(dem_solver* s, const char* path)
             return set_error(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path);
         if (endian != 0x01020304u || owners != (uint32_t)n || topo != 40u || rec != 188u)
             return set_error(DEM_E_INVALID, "incompatible DMHIST2 header: %s", path);
-        if (comps != (uint32_t)n) return set_error(DEM_E_INVALID, "DMHIST2 component count mismatch: %s", path);
+        if (comps != (uint32_t)s->nc) return set_error(DEM_E_INVALID, "DMHIST2 component count mismatch: %s", path);
         for (uint32_t i = 0; i < comps; ++i) {
             int32_t ids[2];
             double geo[4];
@@ -2907,16 +3089,16 @@ int dem_load_contact_history(dem_solver* s, const char* path)
             int32_t id;
             if (!rd(&id, 4) || !rd(&state[(size_t)23 * i], 23 * 8))
                 return set_error(DEM_E_IO, "truncated CUDA contact-history checkpoint: %s", path);
-            if (id != s->ownerIds[i]) return set_error(DEM_E_INVALID, "DMHIST2 owner identity mismatch: %s", path);
+            if (id != s->pids[i]) return set_error(DEM_E_INVALID, "DMHIST2 owner identity mismatch: %s", path);
         }
         v2 = true;
     } else {
         return set_error(DEM_E_INVALID, "unsupported contact-history checkpoint: %s", path);
     }
-    if (comps != (uint32_t)n || slots[0] != (uint32_t)PP_SLOTS || slots[1] != (uint32_t)PW_SLOTS ||
+    if (comps != (uint32_t)s->nc || slots[0] != (uint32_t)PP_SLOTS || slots[1] != (uint32_t)PW_SLOTS ||
         slots[2] != (uint32_t)PT_SLOTS)
         return set_error(DEM_E_INVALID, "incompatible CUDA contact-history checkpoint: %s", path);
-    const size_t nc = (size_t)n;
+    const size_t nc = (size_t)s->nc;
     std::vector<uint64_t> pk(nc * PP_SLOTS), wk(nc * PW_SLOTS), bk(nc * PT_SLOTS);
     std::vector<double> pv(nc * PP_SLOTS * 9), wv(nc * PW_SLOTS * 9), bv(nc * PT_SLOTS * 9);
     if (!rd(pk.data(), 8 * pk.size()) || !rd(wk.data(), 8 * wk.size()) || !rd(bk.data(), 8 * bk.size()) ||
@@ -3017,7 +3199,7 @@ int dem_save_contact_history(const dem_solver* cs, const char* path)
     TRY(flush_finish(cs));
     HIP_TRY(hipStreamSynchronize(cs->stream), "save contact history");
     const long n = cs->n;
-    const size_t nc = (size_t)n;
+    const size_t nc = (size_t)cs->nc;
     std::vector<double> hd((size_t)NDF * n);
     std::vector<int> orig(n);
     HIP_TRY(hipMemcpy(hd.data(), cs->dState[cs->cur], sizeof(double) * hd.size(), hipMemcpyDeviceToHost),
@@ -3069,18 +3251,18 @@ int dem_save_contact_history(const dem_solver* cs, const char* path)
     if (!out) return set_error(DEM_E_IO, "cannot create CUDA contact-history checkpoint: %s", path);
     auto wr = [&](const void* p, size_t bytes) { out.write(reinterpret_cast<const char*>(p), (std::streamsize)bytes); };
     const char magic[8] = {'D', 'M', 'H', 'I', 'S', 'T', '2', '\0'};
-    const uint32_t head[9] = {2u, 0x01020304u, (uint32_t)n, (uint32_t)n, (uint32_t)PP_SLOTS, (uint32_t)PW_SLOTS,
+    const uint32_t head[9] = {2u, 0x01020304u, (uint32_t)n, (uint32_t)nc, (uint32_t)PP_SLOTS, (uint32_t)PW_SLOTS,
                               (uint32_t)PT_SLOTS, 40u, 188u};
     wr(magic, 8);
     wr(head, sizeof head);
-    for (long i = 0; i < n; ++i) {
+    for (size_t i = 0; i < nc; ++i) {
         const int32_t ids[2] = {cs->compIds[i], cs->ownerIds[i]};
         const double geo[4] = {cs->compR[i], cs->compB[3 * i], cs->compB[3 * i + 1], cs->compB[3 * i + 2]};
         wr(ids, 8);
         wr(geo, 32);
     }
     for (long i = 0; i < n; ++i) {
-        const int32_t id = cs->ownerIds[i];
+        const int32_t id = cs->pids[i];
         wr(&id, 4);
         wr(&state[(size_t)23 * i], 23 * 8);
     }

@@ -127,6 +127,42 @@ def sphere_particles(pos, radius, density=None, mass=None, ids=None, status=None
                 quat=np.tile([1.0, 0.0, 0.0, 0.0], (n, 1)))
 
 
+def clumpify(scene, counts, seed=0, shrink=0.7, spread=None):
+    """Replaces every sphere owner of `scene` by a clump of counts[i] (1..8)
+    sphere components: radius shrink * r, centres at spread * r (default
+    1 - shrink: the clump stays inside the original sphere) from the owner
+    centre along seeded random body-frame directions.  Owner mass and
+    inertia are kept; component ids are 8 * owner id + k.  Returns a new
+    Scene."""
+    spread = 1.0 - shrink if spread is None else spread
+    counts = np.broadcast_to(np.asarray(counts, int), (scene.n,))
+    if np.any(counts < 1) or np.any(counts > 8):
+        raise ValueError("1 to 8 components per owner")
+    rng = np.random.RandomState(seed)
+    cid, owner, radius, posb = [], [], [], []
+    for i in range(scene.n):
+        r = float(scene.radius[i])
+        d = rng.normal(size=(counts[i], 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        for k in range(counts[i]):
+            cid.append(8 * int(scene.pid[i]) + k)
+            owner.append(i)
+            radius.append(shrink * r)
+            posb.append(spread * r * d[k])
+    out = scene.copy()
+    out.comp_id = np.array(cid, np.int32)
+    out.comp_owner = np.array(owner, np.int32)
+    out.comp_radius = np.array(radius)
+    out.comp_pos_body = np.array(posb).reshape(-1, 3)
+    return out
+
+
+def random_quaternions(n, seed):
+    """Seeded uniformly distributed unit quaternions (w, x, y, z)."""
+    q = np.random.RandomState(seed).normal(size=(n, 4))
+    return q / np.linalg.norm(q, axis=1)[:, None]
+
+
 def box_walls(lo, hi):
     """Six planar walls of an axis-aligned box, normals pointing inward."""
     lo = np.asarray(lo, float)
@@ -446,6 +482,100 @@ def read_particles_bt(path):
                 vel=data[:, 7:10].copy(), quat=data[:, 10:14].copy(), angvel=data[:, 14:17].copy())
 
 
+def _sidecar_records(path, what):
+    """Non-blank lines of a sidecar with their 1-based line numbers."""
+    if not os.path.isfile(path):
+        raise RuntimeError(f"Cannot open {what}: {os.path.abspath(path)}")
+    out = []
+    with open(path, encoding="utf-8", errors="replace") as f:
+        for number, line in enumerate(f, start=1):
+            if line.strip(" \t\r\n"):
+                out.append((number, line.split()))
+    return out
+
+
+def _int_token(tok):
+    if not re.fullmatch(r"[+-]?\d+", tok):
+        raise ValueError(tok)
+    v = int(tok)
+    if v < -2 ** 31 or v >= 2 ** 31:
+        raise ValueError(tok)
+    return v
+
+
+def read_principal_inertia(path, pid):
+    """PARTICLE::LoadPrincipalInertia (reference particle.cpp:108-163): one
+    "owner_id Ixx Iyy Izz" record per owner (blank lines skipped), every
+    owner exactly once, finite positive moments.  Returns [n][3] by owner row."""
+    rows = {int(v): i for i, v in enumerate(pid)}
+    inertia = np.zeros((len(pid), 3))
+    seen = np.zeros(len(pid), bool)
+    for number, tok in _sidecar_records(path, "particle principal-inertia sidecar"):
+        try:
+            if len(tok) != 4:
+                raise ValueError
+            owner = _int_token(tok[0])
+            vals = [float(t) for t in tok[1:]]
+        except ValueError:
+            raise RuntimeError(f"Invalid principal-inertia record at {os.path.abspath(path)}:{number}; "
+                               "expected owner_id Ixx Iyy Izz")
+        if owner not in rows:
+            raise RuntimeError(f"Unknown owner ID in principal-inertia sidecar: {owner}")
+        r = rows[owner]
+        if seen[r]:
+            raise RuntimeError(f"Duplicate owner ID in principal-inertia sidecar: {owner}")
+        if not all(np.isfinite(v) and v > 0.0 for v in vals):
+            raise RuntimeError(f"Principal inertia must be finite and strictly positive for owner {owner}")
+        inertia[r] = vals
+        seen[r] = True
+    if seen.sum() != len(pid):
+        raise RuntimeError("Principal-inertia sidecar must contain exactly one record per owner")
+    return inertia
+
+
+def read_clump_components(path, pid):
+    """PARTICLE::LoadClumpComponents (reference particle.cpp:165-278):
+    "component_id owner_id radius body_x body_y body_z" per component, in
+    file order; unique non-negative component ids, finite geometry with a
+    positive radius, 1 to 8 components per owner.  Returns comp_id,
+    comp_owner (owner row), comp_radius, comp_pos_body."""
+    rows = {int(v): i for i, v in enumerate(pid)}
+    recs = _sidecar_records(path, "clump-component sidecar")
+    if not recs:
+        raise RuntimeError("Clump-component sidecar must not be empty")
+    cid, owner, radius, posb = [], [], [], []
+    ids = set()
+    counts = np.zeros(len(pid), int)
+    for number, tok in recs:
+        try:
+            if len(tok) != 6:
+                raise ValueError
+            c, o = _int_token(tok[0]), _int_token(tok[1])
+            r, x, y, z = (float(t) for t in tok[2:])
+        except ValueError:
+            raise RuntimeError(f"Invalid clump-component record at {os.path.abspath(path)}:{number}; "
+                               "expected component_id owner_id radius body_x body_y body_z")
+        if o not in rows:
+            raise RuntimeError(f"Unknown owner ID in clump-component sidecar: {o}")
+        if c in ids:
+            raise RuntimeError(f"Duplicate component ID in clump-component sidecar: {c}")
+        ids.add(c)
+        if c < 0:
+            raise RuntimeError("Clump component IDs must lie in [0, INT32_MAX]")
+        if not (np.isfinite(r) and r > 0.0 and np.isfinite(x) and np.isfinite(y) and np.isfinite(z)):
+            raise RuntimeError("Clump-component geometry must be finite with positive radius")
+        cid.append(c); owner.append(rows[o]); radius.append(r); posb.append((x, y, z))
+        counts[rows[o]] += 1
+    for i, k in enumerate(counts):
+        if k == 0:
+            raise RuntimeError("Every particle owner must have at least one clump component; "
+                               f"missing owner {int(pid[i])}")
+        if k > 8:
+            raise RuntimeError(f"Each particle owner may have at most 8 clump components; owner {int(pid[i])}")
+    return dict(comp_id=np.array(cid, np.int32), comp_owner=np.array(owner, np.int32),
+                comp_radius=np.array(radius, float), comp_pos_body=np.array(posb, float).reshape(-1, 3))
+
+
 def read_walls_bt(path):
     tok = open(path).read().split()
     n = int(tok[0])
@@ -513,6 +643,14 @@ def read_case(case_dir):
 
     parts = read_particles_bt(resolve(par["CONTROL_PARTICLES_FILE"]))
     kw = dict(parts)
+    # sidecars (reference simulation.cpp:109-110): explicit principal inertia
+    # and clump components; absent or empty keys keep the solid-sphere defaults
+    if par.get("CONTROL_PARTICLE_INERTIA_FILE", "").strip():
+        kw["inertia"] = read_principal_inertia(resolve(par["CONTROL_PARTICLE_INERTIA_FILE"]), kw["pid"])
+    if par.get("CONTROL_CLUMP_COMPONENTS_FILE", "").strip():
+        kw.update(read_clump_components(resolve(par["CONTROL_CLUMP_COMPONENTS_FILE"]), kw["pid"]))
+    elif np.any(kw["pid"] < 0):
+        raise RuntimeError("Default component ID requires a non-negative particle ID")
     if int(par.get("CONTROL_WALL_FLAG", "0")):
         kw["wall_origin"], kw["wall_normal"] = read_walls_bt(resolve(par["CONTROL_WALL_FILE"]))
     if int(par.get("CONTROL_MULTIBODY_FLAG", "0")):

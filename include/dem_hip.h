@@ -128,7 +128,10 @@ void dem_destroy(dem_solver* solver);
  * walls/bodies may be NULL or have count 0.  pp/pt/pw are the particle-
  * particle, particle-triangle and particle-wall MECH blocks.  Fails with the
  * reference's messages on re-initialisation, duplicate particle ids, bad
- * component topology (1..8 components per owner, ids >= 0, radius > 0). */
+ * component topology (1..8 components per owner, ids >= 0, radius > 0).
+ * Owners with several components (clumps) are stepped with the contact
+ * grid, scan and laws over the components, each owner summing its
+ * components in (owner, component id) order. */
 int dem_initialize(dem_solver* solver, const dem_particles_view* particles,
                    const dem_components_view* components,
                    const dem_walls_view* walls, const dem_bodies_view* bodies,
@@ -146,7 +149,8 @@ int dem_initialize(dem_solver* solver, const dem_particles_view* particles,
  * scene's largest radius).  Particles interact only within their scene, and
  * every per-scene result equals that of a solver initialised with the scene
  * alone (bit for bit).  Particle ids must be unique within a scene.  Wall
- * history keys are the wall's index within its scene + 1. */
+ * history keys are the wall's index within its scene + 1.  A batch holds
+ * single-sphere owners only (clumps: DEM_E_UNSUPPORTED). */
 typedef struct dem_batch_view {
     int32_t scene_count;
     const int64_t* particle_start; /* [scene_count + 1], 0 .. particle count */

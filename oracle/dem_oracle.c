@@ -158,8 +158,8 @@ struct ora_solver {
     /* forces (contact.cuh:54-78) */
     int keepBodyParticle, keepBodyDetail;
     double *fcontact, *ffield, *fwall, *ftotal, *fbp, *fbd, *fbr;
-    double *bodyRow, *fzRow; /* [nb][6][n], [nb][n]: per-owner body load and Fz */
-    double* rowScratch;      /* [n] rows in cell order for the tree sums */
+    double *bodyRow, *fzRow; /* [nb][6][nc], [nb][nc]: per-component body load and Fz */
+    double* rowScratch;      /* [nc] rows in cell order for the tree sums */
     int pcount, bcount, pover, bover, phigh, bhigh;
     double bmaxdepth, bfz;
     int32_t* regimeState;
@@ -372,9 +372,9 @@ int ora_initialize(ora_solver* s, const dem_particles_view* P,
     if (s->keepBodyParticle) ALLOC(s->fbp, (size_t)s->nb * n * 6);
     if (s->keepBodyDetail) ALLOC(s->fbd, (size_t)s->nb * n * 19);
     ALLOC(s->fbr, 6 * s->nb);
-    ALLOC(s->bodyRow, (size_t)6 * s->nb * n);
-    ALLOC(s->fzRow, (size_t)s->nb * n);
-    ALLOC(s->rowScratch, n);
+    ALLOC(s->bodyRow, (size_t)6 * s->nb * nc);
+    ALLOC(s->fzRow, (size_t)s->nb * nc);
+    ALLOC(s->rowScratch, nc);
     ALLOC(s->regimeState, (size_t)s->nb * nc);
     ALLOC(s->regimeInstant, NREG * s->nb); ALLOC(s->regimeInterval, NREG * s->nb);
     ALLOC(s->microState, (size_t)s->nb * nc * NMICRO);
@@ -555,8 +555,8 @@ static void clear_forces(ora_solver* s)
     memset(s->microState, 0, sizeof(double) * (size_t)s->nb * s->nc * NMICRO);
     memset(s->microInstant, 0, sizeof(double) * NMICRO * s->nb);
     memset(s->fbr, 0, sizeof(double) * 6 * s->nb);
-    memset(s->bodyRow, 0, sizeof(double) * 6 * (size_t)s->nb * s->n);
-    memset(s->fzRow, 0, sizeof(double) * (size_t)s->nb * s->n);
+    memset(s->bodyRow, 0, sizeof(double) * 6 * (size_t)s->nb * s->nc);
+    memset(s->fzRow, 0, sizeof(double) * (size_t)s->nb * s->nc);
     s->pcount = s->bcount = s->phigh = s->bhigh = 0;
     s->bmaxdepth = 0.0; s->bfz = 0.0;
 }
@@ -1022,13 +1022,13 @@ static void body_contact(ora_solver* s, const Mech* m, int comp, int t, int slot
         if (d[18] < depth) d[18] = depth;
     }
     /* the reference adds these with atomics (contact:843-852); here each
-       owner's contributions are summed in triangle order and the owners are
-       reduced afterwards in a fixed tree (tree_sum) */
-    const size_t N = (size_t)s->n;
-    double* r = s->bodyRow + (size_t)body * 6 * N + particle;
+       component's contributions are summed in triangle order and the
+       components are reduced afterwards in a fixed tree (tree_sum) */
+    const size_t N = (size_t)s->nc;
+    double* r = s->bodyRow + (size_t)body * 6 * N + comp;
     r[0] += bnf.x + bfr.x; r[N] += bnf.y + bfr.y; r[2 * N] += bnf.z + bfr.z;
     r[3 * N] += btq.x; r[4 * N] += btq.y; r[5 * N] += btq.z;
-    s->fzRow[(size_t)body * N + particle] += bnf.z + bfr.z;
+    s->fzRow[(size_t)body * N + comp] += bnf.z + bfr.z;
     ++s->bcount;
     if (s->bmaxdepth < depth) s->bmaxdepth = depth;
 }
@@ -1267,7 +1267,7 @@ static double tree_sum(ora_solver* s, const double* row);
 
 static void reduce_body_rows(ora_solver* s, int interval)
 {
-    const size_t N = (size_t)s->n;
+    const size_t N = (size_t)s->nc;
     double* row = malloc(sizeof(double) * N);
     s->bfz = 0.0;
     for (int b = 0; b < s->nb; ++b) {
@@ -1283,7 +1283,7 @@ static void reduce_body_rows(ora_solver* s, int interval)
     free(row);
 }
 
-/* Sum of one per-owner row taken in cell order (the sorted grid order) with
+/* Sum of one per-component row taken in cell order (the sorted grid order) with
  * a fixed tree: tiles of 4096 rows; in a tile, 256 partial sums over rows
  * tid, tid + 256, ...; each group of 64 partials folded pairwise with
  * partner lane ^ 32, ^ 16, ..., ^ 1; the 4 group results added in order; the
@@ -1291,9 +1291,9 @@ static void reduce_body_rows(ora_solver* s, int interval)
  * reference's atomic sums; this one is deterministic. */
 static double tree_sum(ora_solver* s, const double* row)
 {
-    const long n = s->n;
+    const long n = s->nc;
     double* v = s->rowScratch;
-    for (long t = 0; t < n; ++t) v[t] = row[s->cowner[s->gid[t]]];
+    for (long t = 0; t < n; ++t) v[t] = row[s->gid[t]];
     double total = 0.0;
     for (long base = 0; base < n; base += 4096) {
         double part[256];
