@@ -1,18 +1,20 @@
 """FP64 DEM step throughput on one MI355X (particle-steps/s).
 
-Workload (default): pure DEM, 1,000,000 spheres, radius lognormal (median
-1 mm, sigma 0.2, clipped to [0.5, 2] mm), density 2500 kg/m^3, kN = E * median
-radius with E = 1e7 Pa, mu 0.5, restitution 0.5, seeded random sequential
-insertion into a 200 x 200 mm hopper (floor + 4 walls inclined 10 degrees),
-gravity -9.81 z, dt = 5e-7 s, grid cell = 2 x max radius.  --packing dense
-replaces the loose insertion with a near-touching bed (contact-heavy).
-A step is the reference's pure-DEM loop body: advanceParticles,
-computeForces, finishParticleStep, all state resident in HBM.
+Workload: pure DEM, 1,000,000 spheres, radius lognormal (median 1 mm,
+sigma 0.2, clipped to [0.5, 2] mm), density 2500 kg/m^3, kN = E * median
+radius with E = 1e7 Pa, mu 0.5, restitution 0.5, in a 200 x 200 mm hopper
+(floor + 4 walls inclined 10 degrees), gravity -9.81 z, dt = 5e-7 s, grid
+cell = 2 x max radius.  The headline value is the settled state (a
+near-touching bed, about 3.7 stored contacts per sphere; --packing picks
+it), averaged over 1,000 steps; the loose seeded-insertion state (no
+contacts yet) is measured beside it.  A step is the reference's pure-DEM
+loop body: advanceParticles, computeForces, finishParticleStep, all state
+resident in HBM, replayed from captured HIP graphs.
 
 Prints one JSON line with the throughput, a roofline entry for the dominant
-kernel (the contact-law kernel, timed with HIP events on the solver's stream)
-and a CPU baseline (the CPU oracle, single thread, on a bounded sample of the
-same workload).
+kernel (the per-contact law k_pairs, timed with HIP events on the solver's
+stream), the per-step algorithmic-byte roofline, and a CPU baseline (the CPU
+oracle, single thread, on a bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -28,20 +30,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def law_bytes_per_particle(cbar: float, walls: int) -> float:
-    """Algorithmic HBM bytes of the contact-law kernel per particle: own state
-    read (24 f64 + 7 i32 = 220 B), contact list count (4 B), history masks
-    (PP/PW occupancy + seen words read and written, 48 B), outputs (27 f64 =
-    216 B), wall parameters are cached; per stored contact 4 B list entry,
-    4 B key and 72 B history values read + 72 B written (152 B)."""
-    return 220.0 + 4.0 + 48.0 + 216.0 + 152.0 * cbar
+def pairs_bytes(contacts: float, n: int) -> float:
+    """Algorithmic HBM bytes of one k_pairs launch: per stored contact the
+    list entry (8 B), the history slot's 9 f64 read and written (144 B) and
+    the 6 f64 result (48 B); per sphere its 160-byte contact record (read
+    once, as owner and as partner), its sorted-order and history-row indices
+    (8 B)."""
+    return 200.0 * contacts + 168.0 * n
 
 
-def run_steps(s, dt, steps):
-    for _ in range(steps):
-        s.advance(dt)
-        s.compute_forces(collect=False)
-        s.finish(dt)
+def step_bytes_per_particle(cbar: float, passes: int = 3) -> float:
+    """Algorithmic bytes of a whole step per particle: 29 f64 of state read
+    and written (464 B), mass/radius/inertia/id/status (48 B), cell key and
+    index plus `passes` radix passes (12 + 24 P B), and per stored contact
+    its history key and 9 f64 read and written (160 B)."""
+    return 232.0 + 232.0 + 48.0 + 12.0 + 24.0 * passes + 160.0 * cbar
 
 
 def measure_sweep(dm, scene, steps, warmup, seq_steps, count, seed):
@@ -131,19 +134,93 @@ def measure_adjoint(dm, scene, count, n, steps, K, seed):
                         f"d(bed height + plate load)/d(mu, restitution, E) per scene"}
 
 
+def measure_hopper(dm, scene, args, packing, steps, warmup, timing_steps):
+    """Builds the 1M hopper in the given packing, runs `warmup` steps (the
+    autotuner picks candidate lists / reorder interval / pair grid), times
+    `steps` graph-replayed steps, then `timing_steps` eager steps with HIP
+    events around each phase (grid, scan, k_pairs, k_law, reductions)."""
+    import numpy as np
+
+    t0 = time.time()
+    sc = scene.hopper_scene(n=args.n, seed=2, mesh_scale=args.mesh_scale, packing=packing)
+    print(f"# {packing}: scene built in {time.time() - t0:.1f}s, {sc.n} spheres", file=sys.stderr, flush=True)
+    dt = sc.dt
+    s = dm.CudaDemSolver()
+    s.initialize(sc)
+    s.compute_forces(collect=True)
+    s.init_half_step(0.5 * dt)
+    s.run_steps(warmup, dt, graph=True, autotune=True)
+    s.synchronize()
+    t1 = time.perf_counter()
+    s.run_steps(steps, dt, graph=True)
+    s.synchronize()
+    elapsed = time.perf_counter() - t1
+    tms = []
+    for _ in range(timing_steps):
+        s.advance(dt)
+        s.compute_forces(collect=False)
+        s.finish(dt)
+        tms.append(s.last_timings())
+    tm = np.mean(np.array(tms), axis=0)
+    s.advance(dt)
+    s.compute_forces(collect=True)  # a diagnostics step for the contact count
+    s.finish(dt)
+    st = s.stats()
+    s.close()
+    out = {"sc": sc, "value": sc.n * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "phase_ms": {"grid": tm[0], "scan": tm[1], "pairs": tm[2], "law": tm[3]},
+           "contacts": st["particle_contacts"], "cbar": st["particle_contacts"] / sc.n}
+    print(f"# {packing}: ms/step {out['ms_per_step']:.3f}; grid {tm[0]:.3f} ms, scan {tm[1]:.3f} ms, "
+          f"k_pairs {tm[2]:.3f} ms, k_law {tm[3]:.3f} ms; contacts/particle {out['cbar']:.3f}",
+          file=sys.stderr, flush=True)
+    return out
+
+
+def copy_calibration(gib=1):
+    """A plain device copy of `gib` GiB (read + write), the reference kernel
+    for scaling FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    a = torch.ones((gib << 27,), dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, each scaled by the
+    same counter on a copy of known size), when it is for this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or t.get("workload") != workload:
+        return None
+    return t.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=60,
-                    help="untimed steps; the first ones autotune the reorder interval and contact grid")
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=120,
+                    help="untimed steps; from 104 on they include the autotuner's trial spans")
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mesh-scale", type=float, default=2.0,
-                    help="grid cell / max radius (the workload specifies 2; the reference uses 2.5)")
-    ap.add_argument("--packing", choices=("rsa", "dense"), default="rsa",
-                    help="rsa: the specified seeded insertion (loose); dense: near-touching bed")
+                    help="grid cell in max radii (the 1M configuration states 2)")
+    ap.add_argument("--packing", choices=("dense", "rsa"), default="dense",
+                    help="headline state: settled near-touching bed (dense) or loose seeded insertion (rsa)")
+    ap.add_argument("--no-insertion", action="store_true", help="skip the loose-insertion figure")
+    ap.add_argument("--timing-steps", type=int, default=20, help="eager steps timed per phase with HIP events")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy first (counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter-sweep figure")
     ap.add_argument("--sweep-scenes", type=int, default=256)
     ap.add_argument("--sweep-steps", type=int, default=100)
@@ -159,51 +236,32 @@ def main():
               "(the DEM step runs on a single device)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-
     import dem_mbd_cuda_amd as dm
     from dem_mbd_cuda_amd import scene
 
-    t0 = time.time()
-    sc = scene.hopper_scene(n=args.n, seed=2, mesh_scale=args.mesh_scale, packing=args.packing)
-    print(f"# scene built in {time.time() - t0:.1f}s, {sc.n} spheres", file=sys.stderr, flush=True)
-    dt = sc.dt
-    s = dm.CudaDemSolver()
-    s.initialize(sc)
-    s.compute_forces(collect=True)
-    s.init_half_step(0.5 * dt)
-    # steps run as dem_run_steps: (advance, computeForces, finish) replayed from
-    # captured HIP graphs; the warmup autotunes the reorder interval and grid
-    s.run_steps(args.warmup, dt, graph=True, autotune=True)
-    s.synchronize()
-    print(f"# warmup done", file=sys.stderr, flush=True)
-    t1 = time.perf_counter()
-    s.run_steps(args.steps, dt, graph=True)
-    s.synchronize()
-    t2 = time.perf_counter()
-    elapsed = t2 - t1
-    value = sc.n * args.steps / elapsed
-    # kernel timing (HIP events on the solver's stream) over a few more steps
-    tms = []
-    for _ in range(5):
-        s.advance(dt)
-        s.compute_forces(collect=False)
-        s.finish(dt)
-        tms.append(s.last_timings())
-    tm = np.mean(np.array(tms), axis=0)
-    grid, scan, pairs, law = tm[0], tm[1], tm[2], tm[3]
-    s.advance(dt)
-    s.compute_forces(collect=True)  # a diagnostics step for the contact count
-    s.finish(dt)
-    st = s.stats()
-    cbar = st["particle_contacts"] / sc.n
-    law_ms = float(pairs + law)
-    bytes_law = law_bytes_per_particle(cbar, len(sc.wall_origin)) * sc.n
-    achieved = bytes_law / (law_ms * 1e-3) / 1e9
-    print(f"# ms/step {1e3 * elapsed / args.steps:.3f}; grid {grid:.3f} ms, scan {scan:.3f} ms, "
-          f"pairs {pairs:.3f} ms, sums {law:.3f} ms; contacts/particle {cbar:.3f}",
-          file=sys.stderr, flush=True)
-    s.close()
+    if args.copy_calibration:
+        copy_calibration()
+    head = measure_hopper(dm, scene, args, args.packing, args.steps, args.warmup, args.timing_steps)
+    sc = head["sc"]
+    other = None
+    if not args.no_insertion:
+        packing = "rsa" if args.packing == "dense" else "dense"
+        o = measure_hopper(dm, scene, args, packing, min(args.steps, 500), args.warmup, 5)
+        other = {"packing": packing, "value": o["value"], "ms_per_step": o["ms_per_step"],
+                 "contacts_per_particle": o["cbar"], "phase_ms": o["phase_ms"], "steps": min(args.steps, 500)}
+        del o
+
+    state = ("settled near-touching bed (lattice layers relaxed to contact)" if args.packing == "dense"
+             else "loose seeded random sequential insertion")
+    workload = (f"pure DEM hopper, {sc.n} spheres lognormal r (median 1 mm, sigma 0.2, clipped 0.5-2 mm), "
+                f"floor + 4 walls inclined 10 deg, gravity, dt 5e-7, grid cell {args.mesh_scale} x max radius, "
+                + state)
+    pairs_ms = float(head["phase_ms"]["pairs"])
+    bytes_pairs = pairs_bytes(head["contacts"], sc.n)
+    achieved = bytes_pairs / (pairs_ms * 1e-3) / 1e9
+    traffic = load_traffic("k_pairs", workload)
+    step_b = step_bytes_per_particle(head["cbar"])
+    step_gbs = head["value"] * step_b / 1e9
 
     sweep = None
     if not args.no_sweep:
@@ -231,39 +289,46 @@ def main():
         o = lib.solver()
         o.initialize(sc)
         c0 = time.perf_counter()
-        host.run_pure(o, sc, steps=args.cpu_steps)
+        o.compute_forces(collect=True)  # the initial evaluation of the reference loop (untimed)
+        o.init_half_step(0.5 * sc.dt)
         c1 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            o.advance(sc.dt)
+            o.compute_forces(collect=False)
+            o.finish(sc.dt)
+        c2 = time.perf_counter()
         o.close()
-        # run_pure does one extra initial force evaluation
-        cpu = {"value": sc.n * args.cpu_steps / (c1 - c0), "unit": "particle-steps/s", "cores": 1,
+        cpu = {"value": sc.n * args.cpu_steps / (c2 - c1), "unit": "particle-steps/s", "cores": 1,
                "kind": "port",
                "sample": f"CPU oracle (C restatement of the reference law, 1 thread) on the same "
-                         f"{sc.n}-sphere scene, initial force evaluation + {args.cpu_steps} steps"}
+                         f"{sc.n}-sphere {args.packing} scene: {args.cpu_steps} full step(s) timed "
+                         f"({c2 - c1:.1f} s) after the untimed initial evaluation ({c1 - c0:.1f} s)"}
+        print(f"# cpu oracle: {cpu['value']:.3g} particle-steps/s ({c2 - c1:.1f} s timed, {c1 - c0:.1f} s set-up)",
+              file=sys.stderr, flush=True)
 
     out = {
         "metric": "FP64 particle-steps/s @1M",
-        "value": value,
+        "value": head["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded RSA packing)",
-        "config": {"workload": f"pure DEM hopper, {sc.n} spheres lognormal r (median 1 mm, sigma 0.2, "
-                               f"clipped 0.5-2 mm), floor + 4 walls inclined 10 deg, gravity, dt 5e-7, "
-                               f"grid cell {args.mesh_scale} x max radius, "
-                               + ("seeded random sequential insertion" if args.packing == "rsa"
-                                  else "dense near-touching packing"),
-                   "particles": sc.n, "dt": dt, "packing": args.packing,
-                   "contacts_per_particle": cbar},
+        "data": "synthetic (seeded lognormal radii and packing)",
+        "config": {"workload": workload, "particles": sc.n, "dt": sc.dt, "packing": args.packing,
+                   "contacts_per_particle": head["cbar"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_law", "kernel_ms": law_ms,
-                     "bytes_per_particle": law_bytes_per_particle(cbar, len(sc.wall_origin))},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_pairs", "kernel_ms": pairs_ms, "bytes_per_launch": bytes_pairs,
+                     "contacts": head["contacts"]},
+        "step_roofline": {"bytes_per_particle_step": step_b, "achieved": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS, "unit": "GB/s"},
+        "phase_ms": head["phase_ms"],
+        "insertion_state": other,
         "cpu_baseline": cpu,
         "batched_sweep": sweep,
         "adjoint": adjoint,

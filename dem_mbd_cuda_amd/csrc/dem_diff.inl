@@ -188,6 +188,34 @@ __device__ __forceinline__ void add_body_grad(double* z, long n, long p, const B
     }
 }
 
+// Owner-side body adjoint of the contacts held by a wave: the contacts of
+// one owner are contiguous in the tape, so each run of lanes with the same
+// owner is summed with shuffles (suffix sums within the run) and the run's
+// first lane adds the total; one set of atomics per owner instead of one
+// per contact.
+__device__ __forceinline__ void add_body_grad_owner(double* z, long n, long p, bool active, const BodyGrad& g)
+{
+    const int lane = threadIdx.x & 63;
+    const int key = active ? (int)p : -1 - lane;  // idle lanes are runs of their own
+    double v[15] = {g.pos.x, g.pos.y, g.pos.z, g.vel.x, g.vel.y, g.vel.z, g.vh.x, g.vh.y, g.vh.z,
+                    g.w.x,   g.w.y,   g.w.z,   g.wh.x,  g.wh.y,  g.wh.z};
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int next = __shfl_down(key, o, 64);  // every lane takes part in the shuffle
+        const bool same = lane + o < 64 && next == key;
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+            const double t = __shfl_down(v[k], o, 64);
+            if (same) v[k] += t;
+        }
+    }
+    const int before = __shfl_up(key, 1, 64);
+    if (active && (lane == 0 || before != key)) {
+#pragma unroll
+        for (int k = 0; k < 15; ++k) unsafeAtomicAdd(&z[(long)(Z_B + k) * n + p], v[k]);
+    }
+}
+
 // Adds a material adjoint to the scene's accumulator: one atomic per wave
 // when the wave's lanes share a scene.
 __device__ __forceinline__ void scene_add(double* acc, int scene, bool active, const MatGrad& g)
@@ -215,6 +243,8 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pairs(long count, const int2* pai
     const long e = (long)blockIdx.x * BLOCK + threadIdx.x;
     const bool active = e < count;
     MatGrad g{0.0, 0.0, 0.0, 0.0};
+    BodyGrad Ib = zero_body_grad();
+    long owner = 0;
     int scene = 0;
     if (active) {
         const int2 pr = pair[e];
@@ -228,13 +258,14 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pairs(long count, const int2* pai
         double hbv[9];
         for (int k = 0; k < 9; ++k) hbv[k] = hb[k];
         double prevb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        BodyGrad Ib = zero_body_grad(), Jb = zero_body_grad();
+        BodyGrad Jb = zero_body_grad();
         pp_law_vjp(I, J, m, prevTape + e * 9, ld3(z, Z_F, n, p), ld3(z, Z_T, n, p), hbv, Ib, Jb, prevb, g);
         const bool fresh = !((occTape[p] >> slot) & 1ull);
         for (int k = 0; k < 9; ++k) hb[k] = fresh ? 0.0 : prevb[k];
-        add_body_grad(z, n, p, Ib);
         add_body_grad(z, n, q, Jb);
+        owner = p;
     }
+    add_body_grad_owner(z, n, owner, active, Ib);
     scene_add(grads, scene, active, g);
 }
 
@@ -502,6 +533,33 @@ int diff_high(dem_solver* s, DiffRun& R, int& ppW, int& pwW)
     return DEM_OK;
 }
 
+// Puts the rows in cell order once, before a differentiable run records
+// anything: input rows may carry no locality, and the run keeps its rows
+// fixed from here on (every tape entry is a row).  Only with empty history
+// tables (no checkpoint loaded), whose rows are then reset to the rows.
+int diff_reorder(dem_solver* s)
+{
+    const long n = s->n;
+    hipStream_t st = s->stream;
+    Soa S = s->soa();
+    hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
+    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dSp, s->offs, s->sc, s->boxPart,
+                       s->partScene, s->sceneAcc, (const double*)nullptr, 0.0);
+    hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,
+                       s->partScene, s->sceneAcc, s->dGrid, s->nscenes, s->cellCap, s->gridDone);
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dGrid, s->keyA, s->valA);
+    TRY(launch_check("grid keys"));
+    TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->nbits,
+                   s->rsDigits));
+    hipLaunchKernelGGL(k_fixup, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->keyA, s->valA, n,
+                       S.i + (long)I_ORIG * n);
+    Soa D{s->dState[1 - s->cur], s->iState[1 - s->cur], n};
+    hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valA);
+    s->cur = 1 - s->cur;
+    hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, st, D.i + (long)I_HIST * n, n);
+    return launch_check("differentiable run row order");
+}
+
 int diff_step(dem_solver* s, double dt)
 {
     const long n = s->n;
@@ -551,12 +609,13 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
     R.steps = steps;
     R.K = checkpoint_interval;
     R.plate = plate_wall;
-    s->reorderInterval = 0;  // rows stay put: row = input index in every tape
+    s->reorderInterval = 0;  // rows stay put (after diff_reorder): every tape entry is a row
     s->verlet = false;       // (candidate-list rebuilds reorder rows)
     s->countAlways = true;   // checkpoints size history copies by the high-water marks
     s->vlistValid = false;
     const long n = s->n;
     const double dt = s->pp.dt;
+    if (!s->historyLoaded) TRY(diff_reorder(s));
     TRY(dalloc(R.dPP, steps + 1, "hipMalloc(run counts)"));
     TRY(dalloc(R.dPW, steps + 1, "hipMalloc(run counts)"));
     TRY(dalloc(R.dHigh, 2, "hipMalloc(run counts)"));
@@ -595,17 +654,22 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
     R.pwCount.assign(steps + 1, 0);
     HIP_TRY(hipMemcpy(R.ppCount.data(), R.dPP, sizeof(long long) * (steps + 1), hipMemcpyDeviceToHost), "download counts");
     HIP_TRY(hipMemcpy(R.pwCount.data(), R.dPW, sizeof(long long) * (steps + 1), hipMemcpyDeviceToHost), "download counts");
-    // observables, summed per scene in row order
+    // observables, summed per scene in input-row order (a scene's input rows
+    // and rows are the same range)
     std::vector<double> zpos(n), acc(n);
+    std::vector<int> orig(n), row(n);
     HIP_TRY(hipMemcpy(zpos.data(), s->dState[s->cur] + (size_t)F_Z * n, sizeof(double) * n, hipMemcpyDeviceToHost),
             "download heights");
     HIP_TRY(hipMemcpy(acc.data(), R.plateAcc, sizeof(double) * n, hipMemcpyDeviceToHost), "download plate load");
+    HIP_TRY(hipMemcpy(orig.data(), s->iState[s->cur] + (size_t)I_ORIG * n, sizeof(int) * n, hipMemcpyDeviceToHost),
+            "download rows");
+    for (long p = 0; p < n; ++p) row[orig[p]] = (int)p;
     for (int k = 0; k < s->nscenes; ++k) {
         const SceneParams& q = s->hsp[k];
         double hz = 0.0, f = 0.0;
-        for (long p = q.row0; p < q.row1; ++p) {
-            hz += zpos[p];
-            f += acc[p];
+        for (long i = q.row0; i < q.row1; ++i) {
+            hz += zpos[row[i]];
+            f += acc[row[i]];
         }
         if (height) height[k] = hz / (double)(q.row1 - q.row0);
         if (plate_load) plate_load[k] = f / (double)steps;

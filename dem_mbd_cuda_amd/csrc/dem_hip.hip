@@ -1831,6 +1831,7 @@ struct dem_solver {
     uint64_t intervalSteps = 0;
     size_t freeBefore = 0;
     bool restored = false;
+    bool historyLoaded = false;  // a checkpoint filled the history tables
     bool legacySphere = false;  // one zero-offset component per owner, id = owner id
     // clumps (an owner with several sphere components): the contact pipeline
     // runs over component rows [nc] that carry copies of their owner's state
@@ -2419,6 +2420,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     const char* env = getenv("DEM_REORDER_INTERVAL");
     if (env) s->reorderInterval = std::max(0, atoi(env));
     if (clump) s->reorderInterval = 0;  // owners keep their rows (k_law_clump indexes them directly)
+    if (const char* pb = getenv("DEM_PAIR_BLOCKS")) s->pairBlocks = std::max(1, atoi(pb));
     s->initialized = true;
     return DEM_OK;
 }
@@ -3188,6 +3190,7 @@ int dem_load_contact_history(dem_solver* s, const char* path)
     (void)st;
     s->stepIndex = 0;
     s->restored = v2;
+    s->historyLoaded = true;
     s->vlistValid = false;
     return DEM_OK;
 }
