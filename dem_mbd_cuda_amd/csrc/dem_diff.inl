@@ -497,6 +497,7 @@ int restore_checkpoint(dem_solver* s, const Checkpoint& c)
 {
     const long n = s->n;
     hipStream_t st = s->stream;
+    s->recFresh = false;
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
     };
@@ -556,6 +557,7 @@ int diff_reorder(dem_solver* s)
     Soa D{s->dState[1 - s->cur], s->iState[1 - s->cur], n};
     hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valA);
     s->cur = 1 - s->cur;
+    s->recFresh = false;
     hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, st, D.i + (long)I_HIST * n, n);
     return launch_check("differentiable run row order");
 }
@@ -563,7 +565,8 @@ int diff_reorder(dem_solver* s)
 int diff_step(dem_solver* s, double dt)
 {
     const long n = s->n;
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, (Rec*)nullptr);
+    s->recFresh = false;
     TRY(dem_compute_forces(s, nullptr, nullptr, 0));
     hipLaunchKernelGGL(k_finish, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
     return launch_check("differentiable step");
@@ -613,6 +616,7 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
     s->verlet = false;       // (candidate-list rebuilds reorder rows)
     s->countAlways = true;   // checkpoints size history copies by the high-water marks
     s->vlistValid = false;
+    s->recFresh = false;
     const long n = s->n;
     const double dt = s->pp.dt;
     if (!s->historyLoaded) TRY(diff_reorder(s));

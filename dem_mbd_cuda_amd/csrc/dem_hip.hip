@@ -698,6 +698,24 @@ __device__ __forceinline__ Body rec_body(const Rec* R, long p)
     return B;
 }
 
+// Contact record of a zero-offset row from its state values (what k_pack
+// writes for it; the advance kernels call it with the values they just
+// stored, saving k_pack's second pass over the state).
+__device__ __forceinline__ void put_rec(Rec* R, const Soa& S, long p, V3 pos, V3 vel, V3 vh, double qw, double qx,
+                                        double qy, double qz, V3 w, V3 wh)
+{
+    Rec q;
+    const V3 ww = rot(qw, qx, qy, qz, w), wwh = rot(qw, qx, qy, qz, wh);
+    q.px = pos.x; q.py = pos.y; q.pz = pos.z;
+    q.vx = vel.x; q.vy = vel.y; q.vz = vel.z;
+    q.hx = vh.x; q.hy = vh.y; q.hz = vh.z;
+    q.wx = ww.x; q.wy = ww.y; q.wz = ww.z;
+    q.ux = wwh.x; q.uy = wwh.y; q.uz = wwh.z;
+    q.mass = S.f(F_MASS, p); q.r = S.f(F_CR, p); q.status = S.k(I_STATUS, p); q.cid = (uint32_t)S.k(I_CID, p);
+    q.scene = S.k(I_SCENE, p); q.pad0 = 0; q.pad1 = 0.0;
+    R[p] = q;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Rec* R)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
@@ -804,6 +822,7 @@ struct LawArgs {
     Soa R;                       // contact rows: the owners, or their components (clumps)
     const int* rank;             // [rows] sorted position of each contact row (clumps)
     const int* ownerStart;       // [owners + 1] first component of each owner (clumps)
+    const double4* posr;         // [rows] centre and radius in cell order
     const Rec* rec;
     const int* sorted;
     const int* ncount;
@@ -1308,11 +1327,34 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
     int bodyHigh = 0, pwOver = 0;
     if (t < n) {
         const long p = A.sorted[t];
-        Body I = rec_body(A.rec, p);
-        if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
+        const double mass = S.f(F_MASS, p);
         V3 cF = zero3(), cT = zero3(), wF = zero3(), wT = zero3(), bF = zero3(), bT = zero3();
-        const V3 fF = field_force(A, p, I.mass);
-        wall_pass(A, I, p, S.k(I_HIST, p), S.k(I_SCENE, p), wF, wT, bodyHigh, pwOver);
+        const V3 fF = field_force(A, p, mass);
+        // walls: the depth test of wall_pass on the packed centre and radius;
+        // the full contact record is read only by rows that touch a wall
+        const int scene = S.k(I_SCENE, p);
+        const long hr = S.k(I_HIST, p);
+        const double4 cr = A.posr[t];
+        const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
+        bool touch = false;
+        for (int w = 0; w < nw; ++w) {
+            const int g = w0 + w;
+            const V3 outward = mk(A.W.nrm[3 * g], A.W.nrm[3 * g + 1], A.W.nrm[3 * g + 2]);
+            const V3 origin = mk(A.W.o[3 * g], A.W.o[3 * g + 1], A.W.o[3 * g + 2]);
+            touch |= cr.w - dot(sub(mk(cr.x, cr.y, cr.z), origin), outward) > 0.0;
+        }
+        if (touch) {
+            Body I = rec_body(A.rec, p);
+            if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
+            wall_pass(A, I, p, hr, scene, wF, wT, bodyHigh, pwOver);
+        } else {
+            // expiry only: no wall slot is seen this step
+            const History& H = A.H;
+            const unsigned occ = H.pwOcc[hr] & H.pwSeen[(long)((A.step + 1) & 1) * n + hr];
+            if (A.tape.pwOcc) A.tape.pwOcc[hr] = occ;
+            H.pwOcc[hr] = occ;
+            H.pwSeen[(long)(A.step & 1) * n + hr] = 0u;
+        }
         // particle pairs, summed in candidate order from the per-contact results
         const int cnt = A.ncount[t];
         const long base = A.off[t];
@@ -1327,7 +1369,7 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
             bF = mk(ob_[0 * n + p], ob_[1 * n + p], ob_[2 * n + p]);
             bT = mk(ob_[3 * n + p], ob_[4 * n + p], ob_[5 * n + p]);
         }
-        owner_out(A, p, I.mass, cF, cT, fF, wF, wT, bF, bT);
+        owner_out(A, p, mass, cF, cT, fF, wF, wT, bF, bT);
     }
     if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
@@ -1571,24 +1613,36 @@ __global__ void __launch_bounds__(BLOCK) k_init_half(Soa S, double h)
     half_quat(S, p, h);
 }
 
-// advancePositionKernel (particle:213-238)
-__global__ void __launch_bounds__(BLOCK) k_advance(Soa S, double dt)
+// advancePositionKernel (particle:213-238); with R, also the row's contact
+// record for the force evaluation that follows.
+__global__ void __launch_bounds__(BLOCK) k_advance(Soa S, double dt, Rec* R)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (p >= S.n || S.k(I_STATUS, p) == -1) return;
-    S.f(F_X, p) += S.f(F_VHX, p) * dt;
-    S.f(F_Y, p) += S.f(F_VHY, p) * dt;
-    S.f(F_Z, p) += S.f(F_VHZ, p) * dt;
-    double qw = S.f(F_QW, p) + S.f(F_HQW, p) * dt, qx = S.f(F_QX, p) + S.f(F_HQX, p) * dt,
-           qy = S.f(F_QY, p) + S.f(F_HQY, p) * dt, qz = S.f(F_QZ, p) + S.f(F_HQZ, p) * dt;
-    quat_unit(qw, qx, qy, qz);
-    S.f(F_QW, p) = qw; S.f(F_QX, p) = qx; S.f(F_QY, p) = qy; S.f(F_QZ, p) = qz;
-    S.f(F_VX, p) = S.f(F_VHX, p) + S.f(F_AX, p) * (0.5 * dt);
-    S.f(F_VY, p) = S.f(F_VHY, p) + S.f(F_AY, p) * (0.5 * dt);
-    S.f(F_VZ, p) = S.f(F_VHZ, p) + S.f(F_AZ, p) * (0.5 * dt);
-    S.f(F_WX, p) = S.f(F_WHX, p) + S.f(F_AWX, p) * (0.5 * dt);
-    S.f(F_WY, p) = S.f(F_WHY, p) + S.f(F_AWY, p) * (0.5 * dt);
-    S.f(F_WZ, p) = S.f(F_WHZ, p) + S.f(F_AWZ, p) * (0.5 * dt);
+    if (p >= S.n) return;
+    const V3 vh = mk(S.f(F_VHX, p), S.f(F_VHY, p), S.f(F_VHZ, p));
+    const V3 wh = mk(S.f(F_WHX, p), S.f(F_WHY, p), S.f(F_WHZ, p));
+    V3 pos = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p));
+    double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+    V3 vel, w;
+    if (S.k(I_STATUS, p) != -1) {
+        pos = mk(pos.x + vh.x * dt, pos.y + vh.y * dt, pos.z + vh.z * dt);
+        S.f(F_X, p) = pos.x; S.f(F_Y, p) = pos.y; S.f(F_Z, p) = pos.z;
+        qw = qw + S.f(F_HQW, p) * dt; qx = qx + S.f(F_HQX, p) * dt;
+        qy = qy + S.f(F_HQY, p) * dt; qz = qz + S.f(F_HQZ, p) * dt;
+        quat_unit(qw, qx, qy, qz);
+        S.f(F_QW, p) = qw; S.f(F_QX, p) = qx; S.f(F_QY, p) = qy; S.f(F_QZ, p) = qz;
+        vel = mk(vh.x + S.f(F_AX, p) * (0.5 * dt), vh.y + S.f(F_AY, p) * (0.5 * dt),
+                 vh.z + S.f(F_AZ, p) * (0.5 * dt));
+        w = mk(wh.x + S.f(F_AWX, p) * (0.5 * dt), wh.y + S.f(F_AWY, p) * (0.5 * dt),
+               wh.z + S.f(F_AWZ, p) * (0.5 * dt));
+        S.f(F_VX, p) = vel.x; S.f(F_VY, p) = vel.y; S.f(F_VZ, p) = vel.z;
+        S.f(F_WX, p) = w.x; S.f(F_WY, p) = w.y; S.f(F_WZ, p) = w.z;
+    } else {
+        if (!R) return;
+        vel = mk(S.f(F_VX, p), S.f(F_VY, p), S.f(F_VZ, p));
+        w = mk(S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p));
+    }
+    if (R) put_rec(R, S, p, pos, vel, vh, qw, qx, qy, qz, w, wh);
 }
 
 // advanceHalfVelocityKernel (particle:240-265)
@@ -1609,7 +1663,7 @@ __global__ void __launch_bounds__(BLOCK) k_finish(Soa S, double dt)
 // advancePositionKernel of the next one (particle:213-265), in one pass over
 // the state: the reference's loop calls them back to back with only host work
 // (the body solve) in between.
-__global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, double dt)
+__global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, double dt, Rec* R)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (p >= S.n) return;
@@ -1628,19 +1682,25 @@ __global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, dou
     S.f(F_VHX, p) = vhx; S.f(F_VHY, p) = vhy; S.f(F_VHZ, p) = vhz;
     S.f(F_WHX, p) = whx; S.f(F_WHY, p) = why; S.f(F_WHZ, p) = whz;
     S.f(F_HQW, p) = gw; S.f(F_HQX, p) = gx; S.f(F_HQY, p) = gy; S.f(F_HQZ, p) = gz;
-    if (S.k(I_STATUS, p) == -1) return;
-    S.f(F_X, p) += vhx * dt;
-    S.f(F_Y, p) += vhy * dt;
-    S.f(F_Z, p) += vhz * dt;
-    qw = qw + gw * dt; qx = qx + gx * dt; qy = qy + gy * dt; qz = qz + gz * dt;
-    quat_unit(qw, qx, qy, qz);
-    S.f(F_QW, p) = qw; S.f(F_QX, p) = qx; S.f(F_QY, p) = qy; S.f(F_QZ, p) = qz;
-    S.f(F_VX, p) = vhx + S.f(F_AX, p) * (0.5 * dt);
-    S.f(F_VY, p) = vhy + S.f(F_AY, p) * (0.5 * dt);
-    S.f(F_VZ, p) = vhz + S.f(F_AZ, p) * (0.5 * dt);
-    S.f(F_WX, p) = whx + S.f(F_AWX, p) * (0.5 * dt);
-    S.f(F_WY, p) = why + S.f(F_AWY, p) * (0.5 * dt);
-    S.f(F_WZ, p) = whz + S.f(F_AWZ, p) * (0.5 * dt);
+    V3 pos = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p)), vel, w;
+    if (S.k(I_STATUS, p) != -1) {
+        pos = mk(pos.x + vhx * dt, pos.y + vhy * dt, pos.z + vhz * dt);
+        S.f(F_X, p) = pos.x; S.f(F_Y, p) = pos.y; S.f(F_Z, p) = pos.z;
+        qw = qw + gw * dt; qx = qx + gx * dt; qy = qy + gy * dt; qz = qz + gz * dt;
+        quat_unit(qw, qx, qy, qz);
+        S.f(F_QW, p) = qw; S.f(F_QX, p) = qx; S.f(F_QY, p) = qy; S.f(F_QZ, p) = qz;
+        vel = mk(vhx + S.f(F_AX, p) * (0.5 * dt), vhy + S.f(F_AY, p) * (0.5 * dt),
+                 vhz + S.f(F_AZ, p) * (0.5 * dt));
+        w = mk(whx + S.f(F_AWX, p) * (0.5 * dt), why + S.f(F_AWY, p) * (0.5 * dt),
+               whz + S.f(F_AWZ, p) * (0.5 * dt));
+        S.f(F_VX, p) = vel.x; S.f(F_VY, p) = vel.y; S.f(F_VZ, p) = vel.z;
+        S.f(F_WX, p) = w.x; S.f(F_WY, p) = w.y; S.f(F_WZ, p) = w.z;
+    } else {
+        if (!R) return;
+        vel = mk(S.f(F_VX, p), S.f(F_VY, p), S.f(F_VZ, p));
+        w = mk(S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p));
+    }
+    if (R) put_rec(R, S, p, pos, vel, mk(vhx, vhy, vhz), qw, qx, qy, qz, w, mk(whx, why, whz));
 }
 
 // ------------------------------------------------------------ triangles
@@ -1872,6 +1932,7 @@ struct dem_solver {
     uint32_t* cidS = nullptr;
     Rec* rec = nullptr;
     bool pendingFinish = false;
+    bool recFresh = false;   // the contact records were written by the last advance (zero-offset rows)
     double pendingDt = 0.0;
     int* nbr = nullptr;      // k-major contact lists [MAX_LIST][n]
     int* ncount = nullptr;   // list length per owner (sorted index)
@@ -2567,6 +2628,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             s->cur = 1 - s->cur;
             S = s->soa();
             R = S;
+            s->recFresh = false;  // the records are in the old row order
             if (verlet && s->H2.ppOcc) {
                 hipLaunchKernelGGL(k_hist_move, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->H, s->H2);
                 std::swap(s->H, s->H2);
@@ -2578,7 +2640,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     }
     hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr, s->cidS,
                        verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank);
-    hipLaunchKernelGGL(k_pack, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->offs, s->rec);
+    if (!s->recFresh) hipLaunchKernelGGL(k_pack, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->offs, s->rec);
     TRY(launch_check("grid build"));
     bool useLists = verlet;
     if (verlet && rebuild) {
@@ -2629,7 +2691,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     TRY(launch_check("particle contact law"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
-    A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.rec = s->rec; A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
+    A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.posr = s->posr; A.rec = s->rec; A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
     A.pairF = s->pairF; A.H = s->H;
     A.W = Walls{s->nw, s->wallBuf, s->wallBuf + 3 * s->nw};
     A.T = triset(s); A.O = s->O; A.sc = s->sc;
@@ -2782,6 +2844,7 @@ int flush_finish(const dem_solver* cs)
     dem_solver* s = const_cast<dem_solver*>(cs);
     if (!s->pendingFinish) return DEM_OK;
     s->pendingFinish = false;
+    s->recFresh = false;
     hipLaunchKernelGGL(k_finish, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), s->pendingDt);
     return launch_check("advanceHalfVelocity");
 }
@@ -2791,6 +2854,7 @@ int dem_init_half_step(dem_solver* s, double h)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     TRY(flush_finish(s));
+    s->recFresh = false;
     hipLaunchKernelGGL(k_init_half, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), h);
     return launch_check("initializeLeapfrog");
 }
@@ -2798,13 +2862,16 @@ int dem_init_half_step(dem_solver* s, double h)
 int dem_advance_particles(dem_solver* s, double dt, double* vmax)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    // zero-offset single-sphere rows: the advance also writes the contact records
+    Rec* R = (!s->offs && !s->clump) ? s->rec : nullptr;
     if (s->pendingFinish) {
         s->pendingFinish = false;
         hipLaunchKernelGGL(k_finish_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(),
-                           s->pendingDt, dt);
+                           s->pendingDt, dt, R);
     } else {
-        hipLaunchKernelGGL(k_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
+        hipLaunchKernelGGL(k_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, R);
     }
+    s->recFresh = R != nullptr;
     if (vmax) *vmax = 0.0;
     return launch_check("advancePosition");
 }
@@ -3191,6 +3258,7 @@ int dem_load_contact_history(dem_solver* s, const char* path)
     s->stepIndex = 0;
     s->restored = v2;
     s->historyLoaded = true;
+    s->recFresh = false;
     s->vlistValid = false;
     return DEM_OK;
 }
