@@ -931,6 +931,77 @@ __device__ __forceinline__ int assign_slots(const Soa& S, long t, long p, int fo
     return stored;
 }
 
+// More overlapping partners than a contact list holds (more than there are
+// history slots, so the reference runs out of slots): expiry, then each
+// contact's slot acquired while `visit` walks the partners again in
+// candidate order, as the reference's loop does; a contact left without a
+// slot is counted (pp_overflow) and dropped.  At most PP_SLOTS contacts can
+// hold a slot, so the list never overflows.
+template <typename Visit>
+__device__ __forceinline__ int assign_inline(const Soa& S, long t, long p, const int* sorted, const uint32_t* cidS,
+                             const History& H, int step, int* nbr, unsigned long long* occTape, int& high,
+                             int& over, Visit&& visit)
+{
+    const long n = S.n;
+    const long o = S.k(I_HIST, p);
+    unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
+    if (occTape) occTape[o] = occ;
+    unsigned long long cur = 0;
+    uint32_t* keys = H.ppKey + o * PP_SLOTS;
+    double* vals = H.ppVal + o * PP_SLOTS * 9;
+    int stored = 0;
+    visit([&](long k) {
+        const int slot = acquire(occ, cur, keys, vals, PP_SLOTS, cidS[k] + 1u, high, over);
+        if (slot < 0) return;
+        nbr[(long)stored * n + t] = (sorted[k] << SLOT_BITS) | slot;
+        ++stored;
+    });
+    H.ppOcc[o] = occ;
+    H.ppSeen[(long)(step & 1) * n + o] = cur;
+    return stored;
+}
+
+// The overlapping candidates of sorted row t in the reference's cell and
+// entry order (see k_scan), handed to f one at a time.
+template <bool CLUMP, typename F>
+__device__ __forceinline__ void scan_overlaps(const Soa& S, long t, long p, int mine, const double4* posr,
+                                              const int* ownS, const unsigned long long* key,
+                                              const unsigned* start, const SceneGrid* grid, bool dense, F&& f)
+{
+    const long n = S.n;
+    const double4 me = posr[t];
+    const SceneGrid& g = grid[S.k(I_SCENE, p)];
+    const long X = (long)g.dims[0], Y = (long)g.dims[1], Z = (long)g.dims[2];
+    const long cx = S.k(I_CX, p) - g.lo[0], cy = S.k(I_CY, p) - g.lo[1], cz = S.k(I_CZ, p) - g.lo[2];
+    const long zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < Z ? cz + 1 : Z - 1;
+    for (long dx = -1; dx <= 1; ++dx) {
+        const long xx = cx + dx;
+        if (xx < 0 || xx >= X) continue;
+        for (long dy = -1; dy <= 1; ++dy) {
+            const long yy = cy + dy;
+            if (yy < 0 || yy >= Y) continue;
+            const unsigned long long col = g.base + ((unsigned long long)xx * Y + yy) * Z;
+            long b, e;
+            if (dense) {
+                b = start[col + zlo];
+                e = start[col + zhi + 1];
+            } else {
+                b = lower_bound_u64(key, n, col + zlo);
+                e = lower_bound_u64(key, n, col + zhi + 1);
+            }
+            for (long k = b; k < e; ++k) {
+                if (CLUMP ? ownS[k] == mine : k == t) continue;
+                const double4 c = posr[k];
+                const double dx_ = c.x - me.x, dy_ = c.y - me.y, dz_ = c.z - me.z;
+                const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+                const double rs = me.w + c.w;
+                if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
+                f(k);
+            }
+        }
+    }
+}
+
 // 27-cell candidate scan (particleContact's overlap test, contact:579-591, in
 // the reference's cell and entry order: the nine (dx, dy) columns, each one
 // contiguous run of the sorted array over dz = -1..1) followed by the
@@ -955,47 +1026,19 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
     if (t < n) {
         const long p = sorted[t];
         const int mine = CLUMP ? ownS[t] : 0;
-        if (S.k(I_VALID, p)) {
-            const double4 me = posr[t];
-            const SceneGrid& g = grid[S.k(I_SCENE, p)];
-            const long X = (long)g.dims[0], Y = (long)g.dims[1], Z = (long)g.dims[2];
-            const long cx = S.k(I_CX, p) - g.lo[0], cy = S.k(I_CY, p) - g.lo[1],
-                       cz = S.k(I_CZ, p) - g.lo[2];
-            const bool dense = sc->dense;
-            const long zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < Z ? cz + 1 : Z - 1;
-            for (long dx = -1; dx <= 1; ++dx) {
-                const long xx = cx + dx;
-                if (xx < 0 || xx >= X) continue;
-                for (long dy = -1; dy <= 1; ++dy) {
-                    const long yy = cy + dy;
-                    if (yy < 0 || yy >= Y) continue;
-                    const unsigned long long col = g.base + ((unsigned long long)xx * Y + yy) * Z;
-                    long b, e;
-                    if (dense) {
-                        b = start[col + zlo];
-                        e = start[col + zhi + 1];
-                    } else {
-                        b = lower_bound_u64(key, n, col + zlo);
-                        e = lower_bound_u64(key, n, col + zhi + 1);
-                    }
-                    for (long k = b; k < e; ++k) {
-                        if (CLUMP ? ownS[k] == mine : k == t) continue;
-                        const double4 c = posr[k];
-                        const double dx_ = c.x - me.x, dy_ = c.y - me.y, dz_ = c.z - me.z;
-                        const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
-                        const double rs = me.w + c.w;
-                        if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
-                        if (found < MAX_LIST) nbr[(long)found * n + t] = (int)k;
-                        ++found;
-                    }
-                }
-            }
-        }
-        if (found > MAX_LIST) {
-            atomicAdd(&sc->list_overflow, 1);
-            found = MAX_LIST;
-        }
-        stored = assign_slots(S, t, p, found, sorted, cidS, H, step, nbr, occTape, high, over);
+        const bool valid = S.k(I_VALID, p) != 0;
+        const bool dense = sc->dense;
+        if (valid)
+            scan_overlaps<CLUMP>(S, t, p, mine, posr, ownS, key, start, grid, dense, [&](long k) {
+                if (found < MAX_LIST) nbr[(long)found * n + t] = (int)k;
+                ++found;
+            });
+        if (found > MAX_LIST)
+            stored = assign_inline(S, t, p, sorted, cidS, H, step, nbr, occTape, high, over, [&](auto&& acq) {
+                scan_overlaps<CLUMP>(S, t, p, mine, posr, ownS, key, start, grid, dense, acq);
+            });
+        else
+            stored = assign_slots(S, t, p, found, sorted, cidS, H, step, nbr, occTape, high, over);
         ncount[t] = stored;
     }
     // per-step statistics only when they are read; overflow counts are
@@ -2759,8 +2802,6 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             s->hScalars->grid_overflow = 0;
             return set_error(DEM_E_CAPACITY, "component grid coordinate exceeds signed 21-bit hash range");
         }
-        if (h.list_overflow != 0)
-            return set_error(DEM_E_CAPACITY, "contact list overflow (more partners than history slots)");
         dem_step_stats stt{};
         stt.particle_contacts = h.particle_contacts;
         stt.body_contacts = h.body_contacts;

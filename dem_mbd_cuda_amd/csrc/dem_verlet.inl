@@ -113,10 +113,44 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
             }
         }
         if (found > MAX_LIST) {
-            atomicAdd(&sc->list_overflow, 1);
-            found = MAX_LIST;
+            // more contacts than a list holds: visit them in the reference's
+            // order by selection (the next larger order key each time) while
+            // the slots are acquired
+            const double4 me = posr[t];
+            const int4 mine = meta[t];
+            const int m = ncand[t];
+            const int total = found;
+            stored = assign_inline(S, t, sorted[t], sorted, cidS, H, step, nbr, occTape, high, over,
+                                   [&](auto&& acq) {
+                uint32_t last = 0;
+                for (int r = 0; r < total; ++r) {
+                    uint32_t best = 0xFFFFFFFFu;
+                    long bk = -1;
+                    for (int c = 0; c < m; ++c) {
+                        const long k = cand[(long)c * n + t];
+                        const double4 o = posr[k];
+                        const double dx_ = o.x - me.x, dy_ = o.y - me.y, dz_ = o.z - me.z;
+                        const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+                        const double rs = me.w + o.w;
+                        if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
+                        const int4 other = meta[k];
+                        const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
+                        if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
+                        const uint32_t kk = ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) |
+                                            (uint32_t)other.w;
+                        if ((r == 0 || kk > last) && kk < best) {
+                            best = kk;
+                            bk = k;
+                        }
+                    }
+                    if (bk < 0) break;
+                    acq(bk);
+                    last = best;
+                }
+            });
+        } else {
+            stored = assign_slots(S, t, sorted[t], found, sorted, cidS, H, step, nbr, occTape, high, over);
         }
-        stored = assign_slots(S, t, sorted[t], found, sorted, cidS, H, step, nbr, occTape, high, over);
         ncount[t] = stored;
     }
     if (counters) {
