@@ -141,3 +141,65 @@ def test_spring_takes_the_half_step_rate_damping_the_endpoint_rate(oracle, tmp_p
     # normal force (hence the absolute tolerance)
     friction = f[:3] - n * (f[:3] @ n)
     assert np.allclose(friction, spring + cS * end, rtol=1e-9, atol=1e-12 * np.linalg.norm(f[:3]))
+
+
+def test_history_expires_one_step_after_contact_is_lost(oracle, tmp_path):
+    """Expiry lag (cuda_contact_backend.cu:217-229): a slot restored for a
+    pair that is apart at the next evaluation is not renewed there and is
+    dropped at the start of the one after; the pair, back in contact by
+    then, starts a fresh spring instead of carrying the planted one."""
+    dt = 1e-4
+    pos_b = np.array([0.0, 0.0, 2.05e-3])        # 0.05 mm apart
+    sc = two_spheres(pos_b, status_a=-1)
+    vh_b = np.array([0.01, 0.0, -1.0])           # 0.1 mm closer after one step
+    tan0 = np.array([-1.0e-3, 0.0, 0.0])
+    va = np.concatenate([np.zeros(3), tan0, np.zeros(3)])
+    path = str(tmp_path / "h.bt")
+    write_dmhist2(path, sc, [state23((0, 0, 0)), state23(pos_b, vh=vh_b)], {1: [(0, 2, va)]})
+    s = oracle.solver()
+    s.initialize(sc)
+    s.load_history(path)
+    s.compute_forces(collect=True)
+    kept = s.debug_history()["pp_keys"][1, 0]
+    s.advance(dt)
+    s.compute_forces(collect=True)
+    st = s.state()
+    h = s.debug_history()
+    s.close()
+    assert kept == 2                               # not yet expired at the first evaluation
+    assert h["pp_keys"][1, 0] == 2                 # the pair holds a slot again ...
+    pb = st["pos"][1]
+    n = -pb / np.linalg.norm(pb)
+    mid = -(vh_b - n * (vh_b @ n))
+    spring = h["pp_values"][1, 0, 3:6]             # ... with a spring grown from zero
+    kS = 2.0 * KN / 7.0
+    # (the normal component of vh cancels in `mid`: absolute tolerance on its scale)
+    assert np.allclose(spring, kS * dt * mid, rtol=1e-12, atol=1e-12 * kS * dt * np.linalg.norm(vh_b))
+    kept_spring = kS * dt * mid + tan0  # what a slot kept through the gap would hold
+    assert np.linalg.norm(spring - kept_spring) > 0.5 * np.linalg.norm(tan0)
+
+
+def test_yield_with_no_increment_rescales_the_stored_spring(oracle, tmp_path):
+    """Yield fallback (cuda_contact_backend.cu:281-291): with no increment
+    (the dt = 0 evaluation) and a stored spring above mu |Fn|, the friction
+    and the stored spring are the stored spring scaled to mu |Fn| (the
+    legacy CPU law returns it unscaled)."""
+    pos_b = np.array([0.0, 0.0, 2.0e-3 - 1.0e-8])  # 10 nm overlap: mu |Fn| ~ 8e-5 N
+    sc = two_spheres(pos_b, status_a=-1)
+    tan0 = np.array([1.0e-3, 5.0e-4, 0.0])
+    va = np.concatenate([np.zeros(3), tan0, np.zeros(3)])
+    path = str(tmp_path / "h.bt")
+    write_dmhist2(path, sc, [state23((0, 0, 0)), state23(pos_b)], {1: [(0, 2, va)]})
+    s = oracle.solver()
+    s.initialize(sc)
+    s.load_history(path)
+    s.compute_forces(collect=True)
+    f = s.forces()["contact"][1]
+    h = s.debug_history()
+    s.close()
+    fn = abs(f[2])
+    limit = MU * fn
+    assert limit < 0.5 * np.linalg.norm(tan0)
+    expect = tan0 * (limit / np.linalg.norm(tan0))
+    assert np.allclose(f[:2], expect[:2], rtol=1e-12, atol=0.0)
+    assert np.allclose(h["pp_values"][1, 0, 3:6], expect, rtol=1e-12, atol=0.0)
