@@ -9,8 +9,14 @@
  *
  * Deliberate choices (same in the HIP path):
  *   - Sums that the reference forms with device atomics (body loads, body
- *     contact counters, regime and microstate aggregates) are formed here in
- *     owner-index order; the reference's own order is nondeterministic.
+ *     contact counters, regime and microstate aggregates) are formed here per
+ *     component and then over the components in cell order with a fixed tree
+ *     (tree_sum); the reference's own order is nondeterministic.
+ *
+ * Parity unpinned: observe_microstate (the 22 body-microstate fields,
+ * contact:1226-1356) has no check independent of this restatement; every
+ * other function is pinned by the reference's validation cases and known
+ * answers, its CPU binary at zero velocity, or closed-form mechanics.
  *   - thrust::sort_by_key on integer keys is a stable radix sort, so the
  *     particle grid order is (key, component index) and the triangle grid
  *     order is (key, fill position) = (key, triangle index).
