@@ -13,10 +13,10 @@
  *     component and then over the components in cell order with a fixed tree
  *     (tree_sum); the reference's own order is nondeterministic.
  *
- * Parity unpinned: observe_microstate (the 22 body-microstate fields,
- * contact:1226-1356) has no check independent of this restatement; every
- * other function is pinned by the reference's validation cases and known
- * answers, its CPU binary at zero velocity, or closed-form mechanics.
+ * Every function is pinned by a check independent of this restatement: the
+ * reference's validation cases and known answers, its CPU binary at zero
+ * velocity, or closed-form mechanics (tests/test_oracle.py,
+ * test_transport_pins.py, test_microstate_pins.py, test_clumps.py).
  *   - thrust::sort_by_key on integer keys is a stable radix sort, so the
  *     particle grid order is (key, component index) and the triangle grid
  *     order is (key, fill position) = (key, triangle index).
