@@ -583,16 +583,14 @@ def read_walls_bt(path):
     return vals[:, :3].copy(), vals[:, 3:].copy()
 
 
-def read_mesh_bt(path):
-    """Vertex count, triangle count, vertices, 1-based faces; per triangle the
-    inner apex v1 = centroid - normal * mean edge (reference multibody.cpp:54-145).
-    Returns [t][15] = v1, v2, v3, v4, unit normal in the body frame."""
-    tok = open(path).read().split()
-    nv, nt = int(tok[0]), int(tok[1])
-    pts = np.array(tok[2:2 + 3 * nv], float).reshape(nv, 3)
-    tri = np.array(tok[2 + 3 * nv:2 + 3 * nv + 3 * nt], int).reshape(nt, 3) - 1
-    out = np.zeros((nt, 15))
-    for k in range(nt):
+def mesh_rows(pts, tri):
+    """Per triangle (0-based vertex indices) the inner apex v1 = centroid -
+    normal * mean edge (reference multibody.cpp:54-145).  Returns [t][15] =
+    v1, v2, v3, v4, unit normal (v3 - v2) x (v4 - v2) in the body frame."""
+    pts = np.asarray(pts, float)
+    tri = np.asarray(tri, int)
+    out = np.zeros((len(tri), 15))
+    for k in range(len(tri)):
         v2, v3, v4 = pts[tri[k, 0]], pts[tri[k, 1]], pts[tri[k, 2]]
         a1, a2, a3 = v3 - v2, v4 - v2, v3 - v4
         center = (v2 + v3 + v4) / 3
@@ -605,6 +603,97 @@ def read_mesh_bt(path):
         v1 = center - normal * mean
         out[k] = np.concatenate([v1, v2, v3, v4, normal])
     return out
+
+
+def read_mesh_bt(path):
+    """Vertex count, triangle count, vertices, 1-based faces (reference
+    multibody.cpp:54-145); see mesh_rows."""
+    tok = open(path).read().split()
+    nv, nt = int(tok[0]), int(tok[1])
+    pts = np.array(tok[2:2 + 3 * nv], float).reshape(nv, 3)
+    tri = np.array(tok[2 + 3 * nv:2 + 3 * nv + 3 * nt], int).reshape(nt, 3) - 1
+    return mesh_rows(pts, tri)
+
+
+def rectangle_mesh(origin, u, v, nu, nv):
+    """origin + [0,1] u + [0,1] v split into nu x nv squares of two triangles,
+    each facing u x v (the side a contact must come from).  Returns mesh rows."""
+    origin, u, v = (np.asarray(a, float) for a in (origin, u, v))
+    pts = np.array([origin + u * (i / nu) + v * (j / nv) for j in range(nv + 1) for i in range(nu + 1)])
+    idx = lambda i, j: j * (nu + 1) + i
+    tri = []
+    for j in range(nv):
+        for i in range(nu):
+            tri.append((idx(i, j), idx(i + 1, j), idx(i + 1, j + 1)))
+            tri.append((idx(i, j), idx(i + 1, j + 1), idx(i, j + 1)))
+    return mesh_rows(pts, np.array(tri))
+
+
+def mixer_scene(n=200_000, seed=6, side=0.22, rmin=1.0e-3, rmax=2.0e-3, density=2500.0, E=1.0e7, mu=0.5,
+                restitution=0.5, dt=1.0e-6, steps=3000, muT=0.1, muR=0.1, beta=0.5, fill=0.3,
+                gravity=(0.0, 0.0, -9.81), blade_clear=2.0e-3, plate_gap=3.0e-3):
+    """DEM-MBD coupled mixer: n spheres, radius U[rmin, rmax], seeded RSA in a
+    square bin, and three triangle-mesh bodies:
+      * the bin (fixed, state 1): floor 32 x 32 x 2 and four walls 16 x 16 x 2
+        triangles (4,096), all facing inward;
+      * a paddle (prescribed, state 2) turning at omega = (0, 0, 6.28) rad/s
+        about the bin axis: a vertical blade, both faces 16 x 32 x 2 triangles
+        (2,048);
+      * a plate (dynamic, state 0, 1 kg, identity inertia) resting above the
+        bed, 16 x 32 x 2 triangles (1,024) facing down.
+    Spheres closer than blade_clear to the blade are left out of the packing;
+    the plate starts plate_gap above the highest sphere.  kN = E * mean
+    radius, epsS = epsN = restitution, no cohesion."""
+    rng = np.random.RandomState(seed)
+    h = 1.5 * side
+    # bin: floor at z = 0, walls to z = h, origin at the floor centre
+    a = 0.5 * side
+    bin_rows = np.concatenate([
+        rectangle_mesh((-a, -a, 0.0), (side, 0, 0), (0, side, 0), 32, 32),          # floor, +z
+        rectangle_mesh((-a, -a, 0.0), (0, side, 0), (0, 0, h), 16, 16),             # x = -a, +x
+        rectangle_mesh((a, -a, 0.0), (0, 0, h), (0, side, 0), 16, 16),              # x = +a, -x
+        rectangle_mesh((-a, -a, 0.0), (0, 0, h), (side, 0, 0), 16, 16),             # y = -a, +y
+        rectangle_mesh((-a, a, 0.0), (side, 0, 0), (0, 0, h), 16, 16)])             # y = +a, -y
+    # paddle blade in the plane y = 0 of its body frame, centre at (0, 0, zc)
+    bw, bh, zc = 0.8 * side, 0.25 * side, 0.15 * side
+    paddle_rows = np.concatenate([
+        rectangle_mesh((-0.5 * bw, 0.0, -0.5 * bh), (bw, 0, 0), (0, 0, bh), 16, 32),   # faces -y
+        rectangle_mesh((-0.5 * bw, 0.0, -0.5 * bh), (0, 0, bh), (bw, 0, 0), 32, 16)])  # faces +y
+    # spheres: RSA in the bin, leaving out those that cut the blade slab
+    radius = np.sort(rng.uniform(rmin, rmax, int(n * 1.08) + 64))[::-1].copy()  # largest first
+    vol = float(np.sum(4.0 / 3.0 * np.pi * radius ** 3))
+    top = vol / (fill * side * side)
+    if top > h:
+        raise ValueError("the bin is too small for the bed")
+    lo = np.array([-a, -a, 0.0])
+    hi = np.array([a, a, top])
+    pos = rsa_positions(radius, lo, hi, seed=seed * 7919 + 5, max_attempts=20000)
+    perm = np.random.RandomState(seed + 1).permutation(len(radius))
+    radius, pos = radius[perm], pos[perm]
+    clear = blade_clear
+    cut = (np.abs(pos[:, 1]) < radius + clear) & (np.abs(pos[:, 0]) < 0.5 * bw + radius + clear) & \
+          (np.abs(pos[:, 2] - zc) < 0.5 * bh + radius + clear)
+    keep = np.nonzero(~cut)[0][:n]
+    if len(keep) < n:
+        raise RuntimeError("mixer packing placed too few spheres")
+    radius, pos = radius[keep].copy(), pos[keep].copy()
+    bed_top = float(np.max(pos[:, 2] + radius))
+    # plate above the bed, facing down, body centre at its middle
+    pw, pl = 0.4 * side, 0.8 * side
+    plate_rows = rectangle_mesh((-0.5 * pl, -0.5 * pw, 0.0), (0, pw, 0), (pl, 0, 0), 16, 32)
+    kN = E * 0.5 * (rmin + rmax)
+    m = mech(epsS=restitution, mu=mu, epsN=restitution, cohesion=0.0, beta=beta, muT=muT, muR=muR, kN=kN)
+    ident = [1.0, 0.0, 0.0, 0.0]
+    pose = np.array([[0.0, 0.0, 0.0, *ident, 0, 0, 0, 0, 0, 0],
+                     [0.0, 0.0, zc, *ident, 0, 0, 0, 0.0, 0.0, 6.28],
+                     [0.0, 0.0, bed_top + plate_gap, *ident, 0, 0, 0, 0, 0, 0]])
+    eye = np.eye(3)
+    return Scene(**sphere_particles(pos, radius, density), pp=m, pt=dict(m), pw=dict(m), dt=dt,
+                 gravity=tuple(gravity), gravity_enabled=1, steps=steps,
+                 body_mass=np.array([1000.0, 10.0, 1.0]), body_pose=pose,
+                 body_tri_count=np.array([len(bin_rows), len(paddle_rows), len(plate_rows)], np.int32),
+                 tri_local=np.concatenate([bin_rows, paddle_rows, plate_rows]),
+                 body_state=np.array([1, 2, 0], np.int32), body_inertia=np.stack([eye, eye, eye]))
 
 
 def read_bodies_csv(path, resolve):
