@@ -177,9 +177,10 @@ def measure_hopper(dm, scene, args, packing, steps, warmup, timing_steps):
 
 
 def measure_coupled(dm, scene, n, steps, warmup):
-    """DEM-MBD coupled mixer (n spheres in a fixed 4,096-triangle bin, a
+    """DEM-MBD coupled mixer (n spheres in a settled-bed stand-in resting on
+    the floor and walls of a fixed 4,096-triangle bin and on both faces of a
     2,048-triangle paddle in prescribed rotation, a dynamic 1,024-triangle
-    plate): the reference's coupled loop (host body predictor and
+    plate on top): the reference's coupled loop (host body predictor and
     Newton-Euler solve between steps, 13 + 6 doubles per body through pinned
     memory each way, diagnostics every `warmup` steps) timed over `steps`
     steps after `warmup`."""
@@ -188,7 +189,7 @@ def measure_coupled(dm, scene, n, steps, warmup):
     from dem_mbd_cuda_amd import host
 
     t0 = time.time()
-    sc = scene.mixer_scene(n=n)
+    sc = scene.mixer_scene(n=n, packing="dense")
     built = time.time() - t0
     s = dm.CudaDemSolver()
     s.initialize(sc)
@@ -209,9 +210,10 @@ def measure_coupled(dm, scene, n, steps, warmup):
             "ms_per_step": 1e3 * per, "particle_steps_per_s": sc.n / per,
             "particle_contacts": int(marks["contacts"][0]), "body_contacts": int(marks["contacts"][1]),
             "scene_build_s": built,
-            "workload": f"mixer: {sc.n} spheres r ~ U[1, 2] mm, seeded RSA in a 220 mm bin (4,096 triangles, "
-                        "fixed), paddle (2,048, prescribed omega_z = 6.28 rad/s), plate (1,024, dynamic, 1 kg), "
-                        "gravity, dt 1e-6, host body solve every step"}
+            "workload": f"mixer: {sc.n} spheres r ~ U[1, 2] mm, relaxed lattice bed pressed onto a 220 mm bin "
+                        "(4,096 triangles, fixed) and both faces of a paddle (2,048, prescribed omega_z = 6.28 "
+                        "rad/s), plate (1,024, dynamic, 1 kg) resting on the bed, gravity, dt 1e-6, host body "
+                        "solve every step"}
 
 
 def copy_calibration(gib=1):

@@ -370,8 +370,25 @@ struct Tri {
     int body;
 };
 
-// bodyTriangleAreaScale (contact:383-407)
-DEM_FN double tri_area_scale(V3 pos, double r, const Tri& t)
+// True unless the ball (c, r) stays clear of the axis-aligned box
+// box[0..2]-box[3..5] by more than a relative 1e-6 plus 1e-12 m, margins
+// many orders above the rounding of the coordinates involved.
+DEM_FN bool ball_meets_box(V3 c, double r, const double* box)
+{
+    const double m = r * (1.0 + 1.0e-6) + 1.0e-12;
+    const double ex = fmax(0.0, fmax(box[0] - c.x, c.x - box[3]));
+    const double ey = fmax(0.0, fmax(box[1] - c.y, c.y - box[4]));
+    const double ez = fmax(0.0, fmax(box[2] - c.z, c.z - box[5]));
+    return ex * ex + ey * ey + ez * ez <= m * m;
+}
+
+// bodyTriangleAreaScale (contact:383-407).  With `box` (a bounding box of
+// the triangle), a contact circle clear of the box returns 0 without the
+// area evaluation: the circle then misses every edge and encloses no
+// vertex, so the three signed arc terms of circle_triangle_area cancel to
+// rounding (about 1e-16 of the circle's area), and the 1e-12 threshold below
+// gives 0 as well.
+DEM_FN double tri_area_scale(V3 pos, double r, const Tri& t, const double* box = nullptr)
 {
     const double distance = dot(t.n, sub(pos, t.a));
     if (distance <= 0.0) return 0.0;
@@ -379,6 +396,7 @@ DEM_FN double tri_area_scale(V3 pos, double r, const Tri& t)
     if (crs <= 1.0e-30) return 0.0;
     const double cr = sqrt(crs);
     const V3 cc = sub(pos, scl(t.n, distance));
+    if (box && !ball_meets_box(cc, cr, box)) return 0.0;
     const double area = circle_triangle_area(cc, t.n, t.a, t.b, t.c, cr);
     if (area <= 0.0) return 0.0;
     const double s = fmin(1.0, area / (kPi * crs));

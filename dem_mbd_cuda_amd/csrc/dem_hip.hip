@@ -19,7 +19,10 @@
 //                particle, then history expiry and one slot per contact
 //   k_off_*      prefix sum of list lengths, compacted (owner, entry) list
 //   k_pairs      particle-particle law, one thread per contact
-//   k_law_pt     triangle manifold and contacts per body (coupled runs)
+//   k_tri_sift   (coupled runs) rows whose cell holds no triangle entry take
+//                the no-contact path; the others go to a work list
+//   k_law_pt     per listed row and body: triangle manifold, manifold law,
+//                microstate terms, read-only triangle contacts
 //   k_law        walls, field, ordered sum of the pair results, triangle
 //                accumulator, accelerations
 //   k_reduce     deterministic row sums for the body loads and diagnostics
@@ -131,9 +134,11 @@ struct Scalars {
     unsigned long long body_max_depth_bits;
     int tri_overflow;
     int pad;
+    int tri_lo[3], tri_hi[3];  // cell box of the triangle grid entries
     long long total_pairs;
     int need_rebuild;   // some component moved more than skin/2 since the last rebuild
     int cand_overflow;  // a rebuild listed more partners than CAND_MAX
+    int tri_work;       // rows listed by k_tri_sift (their cell holds triangle entries)
 };
 
 struct History {
@@ -165,8 +170,13 @@ struct TriSet {
     double* aabb;           // [nt][6]
     const double* pose;     // [nb][13]
     int entries;
-    const unsigned long long* tkey;  // sorted packCell keys
+    // entries sorted by cell: keys are the cell's index in the cell box lo +
+    // [0, dims) (x-major, the packed key's order), values the triangles
+    const unsigned long long* tkey;
     const int* ttri;
+    int lo[3];
+    long long dims[3];
+    const unsigned* start;  // [cells + 1] first entry of each cell, or null (binary search)
 };
 
 struct Outputs {
@@ -222,6 +232,27 @@ __device__ __forceinline__ void atomic_max_wave(int* dst, int v)
     v = wave_max(v);
     if ((threadIdx.x & 63) == 0 && v) atomicMax(dst, v);
 }
+__device__ __forceinline__ void atomic_max_wave_u64(unsigned long long* dst, unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    if ((threadIdx.x & 63) == 0 && v) atomicMax(dst, v);
+}
+// Appends `item` for the lanes with `on` set to list[*count ...], one atomic
+// per wave; the order within the list is unspecified.
+__device__ __forceinline__ void wave_append(bool on, int item, int* count, int* list)
+{
+    const unsigned long long m = __ballot(on);
+    if (m == 0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (on) list[base + __popcll(m & ((1ull << lane) - 1ull))] = item;
+}
 
 // Component view of physical particle p (updateComponentsKernel semantics,
 // particle:116-147, folded into the consumer).
@@ -268,6 +299,11 @@ __global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc,
         sc->body_contacts = 0;
         sc->body_max_depth_bits = 0ull;
         sc->tri_overflow = 0;
+        sc->tri_work = 0;
+        for (int a = 0; a < 3; ++a) {
+            sc->tri_lo[a] = INT_MAX;
+            sc->tri_hi[a] = INT_MIN;
+        }
     }
 }
 
@@ -1162,6 +1198,23 @@ __device__ __forceinline__ long upper_bound_u64(const unsigned long long* k, lon
     return first;
 }
 
+// Entries [b, e) of the triangle grid for cell (x, y, z).
+__device__ __forceinline__ void tri_range(const TriSet& T, int x, int y, int z, long& b, long& e)
+{
+    b = e = 0;
+    if (T.entries <= 0) return;
+    const long long dx = (long long)x - T.lo[0], dy = (long long)y - T.lo[1], dz = (long long)z - T.lo[2];
+    if (dx < 0 || dy < 0 || dz < 0 || dx >= T.dims[0] || dy >= T.dims[1] || dz >= T.dims[2]) return;
+    const unsigned long long c = ((unsigned long long)dx * T.dims[1] + dy) * T.dims[2] + dz;
+    if (T.start) {
+        b = T.start[c];
+        e = T.start[c + 1];
+    } else {
+        b = lower_bound_u64(T.tkey, T.entries, c);
+        e = upper_bound_u64(T.tkey, T.entries, c);
+    }
+}
+
 // Field force on owner row p: gravity, then the O(N^2) universal
 // gravitation over the owners in input-row order (contact:1083-1106).
 __device__ __forceinline__ V3 field_force(const LawArgs& A, long p, double mass)
@@ -1224,15 +1277,56 @@ __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long 
     H.pwSeen[(long)(s & 1) * nr + hr] = pwCur;
 }
 
+// observeBodyMicrostateKernel (contact:1226-1356) for one (row, body) in
+// contact: the 22 microstate terms from the row's manifold (normal, centre,
+// distance, area total) and its manifold slot after this step's update,
+// written to micro[f * stride] (the [body][22][t] rows reduced in cell order).
+__device__ __forceinline__ void micro_terms(const Body& I, const Pose& P, const Material& m, V3 normal, V3 cc,
+                                            double distance, double total, const double* sv, double* ms,
+                                            long stride)
+{
+    const V3 arm = sub(cc, P.c);
+    const V3 bv = add(P.v, cross(P.w, arm));
+    const V3 rv = sub(bv, I.vel);
+    const double cN = damping(m.logN, m.kN, I.mass);
+    const double radius = I.r;
+    const double brs = m.beta * radius * m.beta * radius;
+    const double depth = radius - distance;
+    V3 fn = add(scl(normal, -m.kN * depth), scl(normal, cN * dot(rv, normal)));
+    fn = add(fn, scl(normal, m.cohesion * 4.0 * brs));
+    const double fnn = len(fn);
+    const double msc = m.beta * radius * fnn;
+    const double tlim = m.mu * fnn;
+    const double wlim = m.mu * m.muT * msc;
+    const V3 th = hget(sv, 1), wh = hget(sv, 2);
+    const V3 axisZ = rot(P.qw, P.qx, P.qy, P.qz, mk(0.0, 0.0, 1.0));
+    const V3 radial = sub(arm, scl(axisZ, dot(arm, axisZ)));
+    const double tu = tlim > 1.0e-30 ? fmin(1.0, len(th) / tlim) : 0.0;
+    const double wu = wlim > 1.0e-30 ? fmin(1.0, len(wh) / wlim) : 0.0;
+    const V3 btw = scl(wh, -total), bnl = scl(fn, -total), bth = scl(th, -total);
+    const V3 btt = cross(arm, bth);
+    const double v[22] = {fnn * total, fnn * radius * total, fnn * len(radial) * total,
+                          btw.x, btw.y, btw.z, len(wh) * total, wlim * total, len(th) * total,
+                          tlim * total, wu * total, tu * total, total, bnl.x, bnl.y, bnl.z,
+                          bth.x, bth.y, bth.z, btt.x, btt.y, btt.z};
+    for (int f = 0; f < 22; ++f) ms[(long)f * stride] = v[f];
+}
+
 // Particle-triangle part of computeContactsKernel (contact:1148-1208) for
 // contact row r at sorted position t (history row hr, owner input row o):
 // per body, the area-weighted manifold over the triangles of the row's own
 // cell advances one history slot, then every triangle contributes its
 // read-only contact.  Adds the row's force and torque to bF / bT and the
 // owner's per-body outputs at o; writes the row's body load and Fz at t
-// (reduced over the rows later) and its regime state at r.
+// (reduced over the rows later), its microstate terms at t and its regime
+// state at r; depthBits keeps the largest contact depth (as bits) for one
+// atomic per wave.
+// A cell's entries are in triangle order and a body's triangles are
+// contiguous in body order, so each body's triangles form one run of the
+// cell's entries: one walk over the entries visits the bodies in order.
 __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r, long t, long hr, long o,
-                                         V3& bF, V3& bT, int& bodyHigh, int& ptOver, int& bcount)
+                                         V3& bF, V3& bT, int& bodyHigh, int& ptOver, int& bcount,
+                                         unsigned long long& depthBits)
 {
     const Soa& R = A.R;
     const long nr = R.n, no = A.S.n;
@@ -1242,18 +1336,25 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
     unsigned long long ptCur = 0;
     const TriSet& T = A.T;
     if (T.nt > 0 && T.entries > 0 && R.k(I_VALID, r)) {
-        const unsigned long long key = pack_cell(R.k(I_CX, r), R.k(I_CY, r), R.k(I_CZ, r));
-        const long b0 = lower_bound_u64(T.tkey, T.entries, key);
-        const long e0 = upper_bound_u64(T.tkey, T.entries, key);
+        long b0, e0;
+        tri_range(T, R.k(I_CX, r), R.k(I_CY, r), R.k(I_CZ, r), b0, e0);
+        long ks = b0;  // first entry of a body >= the current one
         for (int body = 0; body < A.nb; ++body) {
             double total = 0.0, wD = 0.0;
             V3 wN = zero3(), wC = zero3();
-            for (long k = b0; k < e0; ++k) {
-                const int tr = T.ttri[k];
-                if (T.body[tr] != body) continue;
+            // entries ks + j (j < 64) with a positive area scale: pt_law hits
+            // exactly these (its depth test follows from r^2 - d^2 > 1e-30)
+            unsigned long long hits = 0ull;
+            long ke = ks;
+            for (; ke < e0; ++ke) {
+                const int tr = T.ttri[ke];
+                if (T.body[tr] != body) break;
+                const double* box = T.aabb + (long)tr * 6;
+                if (!ball_meets_box(I.pos, I.r, box)) continue;  // the contact circle lies on the sphere
                 const Tri tri = load_tri(T, tr);
-                const double scv = tri_area_scale(I.pos, I.r, tri);
+                const double scv = tri_area_scale(I.pos, I.r, tri, box);
                 if (scv <= 0.0) continue;
+                if (ke - ks < 64) hits |= 1ull << (ke - ks);
                 const double distance = dot(tri.n, sub(I.pos, tri.a));
                 const V3 cc = sub(I.pos, scl(tri.n, distance));
                 total += scv;
@@ -1263,6 +1364,8 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
             }
             double bl[6] = {0, 0, 0, 0, 0, 0};
             double fz = 0.0;
+            double* ms = A.O.micro + (long)body * 22 * nr + t;
+            bool observed = false;
             if (total > 0.0) {
                 const V3 mN = unit(wN);
                 const V3 mC = scl(wC, 1.0 / total);
@@ -1275,12 +1378,13 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
                     bool slide = false, twisty = false;
                     manifold_law(I, P, A.pt, mN, mC, mD, sv, &slide, &twisty);
                     A.O.regime[(long)body * nr + r] = 1 | (slide ? 2 : 0) | (twisty ? 4 : 0);
+                    micro_terms(I, P, A.pt, mN, mC, mD, total, sv, ms, nr);
+                    observed = true;
                     Material ro = A.pt;
                     ro.dt = 0.0;
-                    for (long k = b0; k < e0; ++k) {
-                        const int tr = T.ttri[k];
-                        if (T.body[tr] != body) continue;
-                        const Tri tri = load_tri(T, tr);
+                    for (long k = ks; k < ke; ++k) {
+                        if (k - ks < 64 && ((hits >> (k - ks)) & 1ull) == 0ull) continue;
+                        const Tri tri = load_tri(T, T.ttri[k]);
                         const PtOut q = pt_law(I, tri, P, ro, sv);
                         if (!q.hit) continue;
                         bF = add(bF, q.pf);
@@ -1294,7 +1398,8 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
                         bl[5] += btq.z;
                         fz += q.bnf.z + q.bfr.z;
                         ++bcount;
-                        atomicMax(&A.sc->body_max_depth_bits, (unsigned long long)__double_as_longlong(q.depth));
+                        const unsigned long long db = (unsigned long long)__double_as_longlong(q.depth);
+                        depthBits = db > depthBits ? db : depthBits;
                         if (A.O.bodyParticle) {
                             double* d = A.O.bodyParticle + ((long)body * no + o) * 6;
                             d[0] += q.pf.x; d[1] += q.pf.y; d[2] += q.pf.z;
@@ -1315,11 +1420,15 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
             }
             for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * nr + t] = bl[k];
             A.O.fzPart[(long)body * nr + t] = fz;
+            if (!observed)
+                for (int f = 0; f < 22; ++f) ms[(long)f * nr] = 0.0;
+            ks = ke;
         }
     } else {
         for (int body = 0; body < A.nb; ++body) {
             for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * nr + t] = 0.0;
             A.O.fzPart[(long)body * nr + t] = 0.0;
+            for (int f = 0; f < 22; ++f) A.O.micro[((long)body * 22 + f) * nr + t] = 0.0;
         }
     }
     H.ptOcc[hr] = ptOcc;
@@ -1418,20 +1527,56 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
 }
 
-// Triangle contacts of every single-sphere owner (tri_pass).  Writes the
-// body accumulator (bF, bT) read by k_law.
-__global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
+// Single-sphere owners whose cell holds no triangle entry: tri_pass's
+// no-contact path (triangle-slot expiry, zero body load, Fz and body
+// accumulator) and zero microstate rows.  The others are listed in `work`
+// (sorted positions) for k_law_pt and k_regime.
+__global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const Soa& S = A.S;
     const long n = S.n;
-    int bodyHigh = 0, ptOver = 0, bcount = 0;
+    bool near = false;
     if (t < n) {
+        const long p = A.sorted[t];
+        long b0 = 0, e0 = 0;
+        if (A.T.nt > 0 && A.T.entries > 0 && S.k(I_VALID, p))
+            tri_range(A.T, S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p), b0, e0);
+        near = e0 > b0;
+        if (!near) {
+            const History& H = A.H;
+            const long hr = S.k(I_HIST, p);
+            H.ptOcc[hr] = H.ptOcc[hr] & H.ptSeen[(long)((A.step + 1) & 1) * n + hr];
+            H.ptSeen[(long)(A.step & 1) * n + hr] = 0ull;
+            for (int body = 0; body < A.nb; ++body) {
+                for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * n + t] = 0.0;
+                A.O.fzPart[(long)body * n + t] = 0.0;
+                for (int f = 0; f < 22; ++f) A.O.micro[((long)body * 22 + f) * n + t] = 0.0;
+            }
+            for (int k = 0; k < 6; ++k) A.O.body[(long)k * n + p] = 0.0;
+        }
+    }
+    wave_append(near, (int)t, &A.sc->tri_work, work);
+}
+
+// Triangle contacts (tri_pass) of the single-sphere owners listed by
+// k_tri_sift, one per lane in 64-lane blocks (a few hundred waves spread
+// over every CU).  Writes the body accumulator (bF, bT) read by k_law.
+constexpr int TRI_BLOCK = 64;
+__global__ void __launch_bounds__(TRI_BLOCK) k_law_pt(LawArgs A, const int* work)
+{
+    const long i = (long)blockIdx.x * TRI_BLOCK + threadIdx.x;
+    const Soa& S = A.S;
+    const long n = S.n;
+    int bodyHigh = 0, ptOver = 0, bcount = 0;
+    unsigned long long depthBits = 0ull;
+    if (i < A.sc->tri_work) {
+        const long t = work[i];
         const long p = A.sorted[t];
         Body I = rec_body(A.rec, p);
         if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
         V3 bF = zero3(), bT = zero3();
-        tri_pass(A, I, p, t, S.k(I_HIST, p), S.k(I_ORIG, p), bF, bT, bodyHigh, ptOver, bcount);
+        tri_pass(A, I, p, t, S.k(I_HIST, p), S.k(I_ORIG, p), bF, bT, bodyHigh, ptOver, bcount, depthBits);
         double* ob = A.O.body;
         ob[0 * n + p] = bF.x; ob[1 * n + p] = bF.y; ob[2 * n + p] = bF.z;
         ob[3 * n + p] = bT.x; ob[4 * n + p] = bT.y; ob[5 * n + p] = bT.z;
@@ -1439,6 +1584,7 @@ __global__ void __launch_bounds__(BLOCK) k_law_pt(LawArgs A)
     atomic_add_wave(&A.sc->body_contacts, bcount);
     atomic_max_wave(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pt_overflow, ptOver);
+    atomic_max_wave_u64(&A.sc->body_max_depth_bits, depthBits);
 }
 
 // computeContactsKernel (contact:1065-1221) for owners made of several
@@ -1454,6 +1600,7 @@ __global__ void __launch_bounds__(BLOCK) k_law_clump(LawArgs A)
     const Soa& S = A.S;
     const Soa& R = A.R;
     int bodyHigh = 0, pwOver = 0, ptOver = 0, bcount = 0;
+    unsigned long long depthBits = 0ull;
     if (i < S.n) {
         const double mass = S.f(F_MASS, i);
         V3 cF = zero3(), cT = zero3(), wF = zero3(), wT = zero3(), bF = zero3(), bT = zero3();
@@ -1471,7 +1618,7 @@ __global__ void __launch_bounds__(BLOCK) k_law_clump(LawArgs A)
                 cF = add(cF, mk(pf[0], pf[1], pf[2]));
                 cT = add(cT, mk(pf[3], pf[4], pf[5]));
             }
-            if (A.nb > 0) tri_pass(A, I, c, t, hr, i, bF, bT, bodyHigh, ptOver, bcount);
+            if (A.nb > 0) tri_pass(A, I, c, t, hr, i, bF, bT, bodyHigh, ptOver, bcount, depthBits);
         }
         if (A.nb > 0) {
             double* ob = A.O.body;
@@ -1485,112 +1632,34 @@ __global__ void __launch_bounds__(BLOCK) k_law_clump(LawArgs A)
     if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
     atomic_add_wave(&A.sc->pt_overflow, ptOver);
+    atomic_max_wave_u64(&A.sc->body_max_depth_bits, depthBits);
 }
 
-// observeBodyMicrostateKernel (contact:1226-1356), one thread per
-// (body, particle in cell order); writes [body][22][t] so the per-body sums
-// run over the same order as the body loads.
-__global__ void __launch_bounds__(BLOCK) k_observe(Soa S, const int* sorted, History H, TriSet T,
-                                                   Outputs O, Material m, int step, bool offs)
+// regime counters (accumulateBodyRegimeKernel, contact:530-567) of body
+// blockIdx.y: integer sums, counted per wave (ballots) and added with one
+// atomic per wave and counter, so any order gives the same result.  With
+// `work`, over the rows listed by k_tri_sift (no other row is in contact);
+// else over every row.
+__global__ void __launch_bounds__(TRI_BLOCK) k_regime(const int* regime, const int* sorted, long n,
+                                                      const int* work, const Scalars* sc,
+                                                      unsigned long long* instant,
+                                                      unsigned long long* interval, int accumulate)
 {
-    const long n = S.n;
-    const long idx = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (idx >= (long)T.nb * n) return;
-    const int body = (int)(idx / n);
-    const long t = idx - (long)body * n;
-    const long p = sorted[t];
-    double* ms = O.micro + (long)body * 22 * n + t;
-    for (int f = 0; f < 22; ++f) ms[(long)f * n] = 0.0;
-    if ((O.regime[(long)body * n + p] & 1) == 0) return;
-    const Body I = load_body(S, p, offs);
-    const unsigned long long key = pack_cell(S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p));
-    const long b0 = lower_bound_u64(T.tkey, T.entries, key);
-    const long e0 = upper_bound_u64(T.tkey, T.entries, key);
-    double total = 0.0, wD = 0.0;
-    V3 wN = zero3(), wC = zero3();
-    for (long k = b0; k < e0; ++k) {
-        const int tr = T.ttri[k];
-        if (T.body[tr] != body) continue;
-        const Tri tri = load_tri(T, tr);
-        const double scv = tri_area_scale(I.pos, I.r, tri);
-        if (scv <= 0.0) continue;
-        const double distance = dot(tri.n, sub(I.pos, tri.a));
-        const V3 cc = sub(I.pos, scl(tri.n, distance));
-        total += scv;
-        wN = add(wN, scl(tri.n, -scv));
-        wC = add(wC, scl(cc, scv));
-        wD += distance * scv;
-    }
-    if (total <= 0.0) return;
-    const long o = S.k(I_HIST, p);  // history row
-    const unsigned long long seen = H.ptSeen[(long)(step & 1) * n + o];
-    int slot = -1;
-    for (int b = 0; b < PT_SLOTS; ++b) {
-        if (((H.ptOcc[o] >> b) & 1ull) && H.ptKey[o * PT_SLOTS + b] == (uint32_t)(body + 1) &&
-            ((seen >> b) & 1ull)) {
-            slot = b;
-            break;
+    const long i = (long)blockIdx.x * TRI_BLOCK + threadIdx.x;
+    const int body = blockIdx.y;
+    int st = 0;
+    if (work ? i < sc->tri_work : i < n) st = regime[(long)body * n + (work ? (long)sorted[work[i]] : i)];
+    const bool on = (st & 1) != 0;
+    const bool sl = on && (st & 2) != 0, tw = on && (st & 4) != 0;
+    // total, primary regime (stick / slide / twist), slide, twist, both
+    const bool c[7] = {on, on && !sl && !tw, on && sl && !tw, tw, sl, tw, sl && tw};
+    for (int k = 0; k < 7; ++k) {
+        const unsigned long long cnt = (unsigned long long)__popcll(__ballot(c[k]));
+        if ((threadIdx.x & 63) == 0 && cnt) {
+            atomicAdd(&instant[body * 7 + k], cnt);
+            if (accumulate) atomicAdd(&interval[body * 7 + k], cnt);
         }
     }
-    if (slot < 0) return;
-    const V3 normal = unit(wN);
-    const V3 cc = scl(wC, 1.0 / total);
-    const double distance = wD / total;
-    const Pose P = load_pose(T, body);
-    const V3 arm = sub(cc, P.c);
-    const V3 bv = add(P.v, cross(P.w, arm));
-    const V3 rv = sub(bv, I.vel);
-    const double cN = damping(m.logN, m.kN, I.mass);
-    const double radius = I.r;
-    const double brs = m.beta * radius * m.beta * radius;
-    const double depth = radius - distance;
-    V3 fn = add(scl(normal, -m.kN * depth), scl(normal, cN * dot(rv, normal)));
-    fn = add(fn, scl(normal, m.cohesion * 4.0 * brs));
-    const double fnn = len(fn);
-    const double msc = m.beta * radius * fnn;
-    const double tlim = m.mu * fnn;
-    const double wlim = m.mu * m.muT * msc;
-    const double* sv = H.ptVal + (o * PT_SLOTS + slot) * 9;
-    const V3 th = hget(sv, 1), wh = hget(sv, 2);
-    const V3 axisZ = rot(P.qw, P.qx, P.qy, P.qz, mk(0.0, 0.0, 1.0));
-    const V3 radial = sub(arm, scl(axisZ, dot(arm, axisZ)));
-    const double tu = tlim > 1.0e-30 ? fmin(1.0, len(th) / tlim) : 0.0;
-    const double wu = wlim > 1.0e-30 ? fmin(1.0, len(wh) / wlim) : 0.0;
-    const V3 btw = scl(wh, -total), bnl = scl(fn, -total), bth = scl(th, -total);
-    const V3 btt = cross(arm, bth);
-    const double v[22] = {fnn * total, fnn * radius * total, fnn * len(radial) * total,
-                          btw.x, btw.y, btw.z, len(wh) * total, wlim * total, len(th) * total,
-                          tlim * total, wu * total, tu * total, total, bnl.x, bnl.y, bnl.z,
-                          bth.x, bth.y, bth.z, btt.x, btt.y, btt.z};
-    for (int f = 0; f < 22; ++f) ms[(long)f * n] = v[f];
-}
-
-// regime counters (accumulateBodyRegimeKernel, contact:530-567): integer
-// sums, so atomics give the same result in any order.
-__global__ void __launch_bounds__(BLOCK) k_regime(const int* regime, long n, int nb,
-                                                  unsigned long long* instant,
-                                                  unsigned long long* interval, int accumulate)
-{
-    const long idx = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (idx >= (long)nb * n) return;
-    const int st = regime[idx];
-    if ((st & 1) == 0) return;
-    const int body = (int)(idx / n);
-    const bool sl = (st & 2) != 0, tw = (st & 4) != 0;
-    const int primary = tw ? 3 : (sl ? 2 : 1);
-    unsigned long long* ri = instant + body * 7;
-    atomicAdd(&ri[0], 1ull);
-    atomicAdd(&ri[primary], 1ull);
-    if (sl) atomicAdd(&ri[4], 1ull);
-    if (tw) atomicAdd(&ri[5], 1ull);
-    if (sl && tw) atomicAdd(&ri[6], 1ull);
-    if (!accumulate) return;
-    unsigned long long* rv = interval + body * 7;
-    atomicAdd(&rv[0], 1ull);
-    atomicAdd(&rv[primary], 1ull);
-    if (sl) atomicAdd(&rv[4], 1ull);
-    if (tw) atomicAdd(&rv[5], 1ull);
-    if (sl && tw) atomicAdd(&rv[6], 1ull);
 }
 
 // Deterministic row sums: partial[row][block] = tree sum of rows[row][block
@@ -1804,9 +1873,14 @@ __global__ void __launch_bounds__(BLOCK) k_tri_count(TriSet T, double inv, doubl
         return;
     }
     counts[i] = (int)(nx * ny * nz);
+    atomicMin(&sc->tri_lo[0], (int)mnx); atomicMin(&sc->tri_lo[1], (int)mny); atomicMin(&sc->tri_lo[2], (int)mnz);
+    atomicMax(&sc->tri_hi[0], (int)mxx); atomicMax(&sc->tri_hi[1], (int)mxy); atomicMax(&sc->tri_hi[2], (int)mxz);
 }
 
-// fillTriangleCellsKernel (contact:453-476): x -> y -> z nested order
+// fillTriangleCellsKernel (contact:453-476): x -> y -> z nested order.  The
+// key is the cell's index in the box of all entries (x-major like the
+// reference's packed key, so the sorted order is the same), which needs only
+// ceil(log2 cells) bits of sorting.
 __global__ void __launch_bounds__(BLOCK) k_tri_fill(TriSet T, double inv, double pad,
                                                     const int* counts, const long long* offsets,
                                                     unsigned long long* key, int* tri)
@@ -1823,10 +1897,20 @@ __global__ void __launch_bounds__(BLOCK) k_tri_fill(TriSet T, double inv, double
     for (int x = mnx; x <= mxx && local < c; ++x)
         for (int y = mny; y <= mxy && local < c; ++y)
             for (int z = mnz; z <= mxz && local < c; ++z) {
-                key[out + local] = pack_cell(x, y, z);
+                key[out + local] = (((unsigned long long)(x - T.lo[0]) * (unsigned long long)T.dims[1] +
+                                     (unsigned long long)(y - T.lo[1])) * (unsigned long long)T.dims[2]) +
+                                   (unsigned long long)(z - T.lo[2]);
                 tri[out + local] = i;
                 ++local;
             }
+}
+
+// First entry of every cell of the triangle grid's box (cells + 1 entries).
+__global__ void __launch_bounds__(BLOCK) k_tri_start(const unsigned long long* key, long n, unsigned* start,
+                                                     long long cells)
+{
+    for (long long c = (long long)blockIdx.x * BLOCK + threadIdx.x; c <= cells; c += (long long)gridDim.x * BLOCK)
+        start[c] = (unsigned)lower_bound_u64(key, n, (unsigned long long)c);
 }
 
 // ------------------------------------------------------------ downloads
@@ -1991,6 +2075,7 @@ struct dem_solver {
     Outputs O{};
     // triangles
     int* tBody = nullptr;
+    int* triWork = nullptr;  // [nr] sorted positions of the rows near triangles
     double* tLocal = nullptr;
     double* tWorld = nullptr;
     double* tAabb = nullptr;
@@ -2006,6 +2091,14 @@ struct dem_solver {
     unsigned* tsCounts = nullptr;
     int tsBlocks = 0;
     int tEntries = 0;
+    int tLo[3] = {0, 0, 0};        // cell box of the triangle grid entries
+    long long tDims[3] = {0, 0, 0};
+    unsigned* tStart = nullptr;    // dense cell -> first entry table of that box
+    long long tStartCap = 0;       // entries of the table valid this step (0: binary search)
+    long long tStartCapAlloc = 0;
+    int* tBits = nullptr;          // device: key bits of the triangle sort
+    int* hTri = nullptr;           // pinned: [nt] counts, then the sort bits
+    long long* hTriOff = nullptr;  // pinned: [nt] entry offsets
     int* d64 = nullptr;  // device constant 64 (triangle sort bits)
     unsigned long long* regimeInstant = nullptr;
     unsigned long long* regimeInterval = nullptr;
@@ -2047,16 +2140,18 @@ struct dem_solver {
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
                         H2.pwVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart, O.fzPart,
-                        O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, tLocal, tWorld, tAabb,
+                        O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, triWork, tLocal, tWorld, tAabb,
                         pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
                         regimeInstant, regimeInterval, microInstant, microInterval, bodyResult,
-                        redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS};
+                        redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tStart, tBits};
         for (void* p : ptrs)
             if (p) hipFree(p);
         if (hPose) hipHostFree(hPose);
         if (hLoads) hipHostFree(hLoads);
         if (hScalars) hipHostFree(hScalars);
         if (hFlag) hipHostFree(hFlag);
+        if (hTri) hipHostFree(hTri);
+        if (hTriOff) hipHostFree(hTriOff);
         for (auto& e : ev)
             if (e) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
@@ -2086,13 +2181,15 @@ int launch_check(const char* what)
     return DEM_OK;
 }
 
-// Stable LSD radix sort of (key, val) pairs held in (ka, va); (kb, vb) is
-// scratch.  Passes over bits >= *nbits (device) are skipped in pairs.
+// Stable LSD radix sort of (key, val) pairs held in (ka, va) in 8-bit
+// digits; (kb, vb) is scratch.  Passes over bits >= *nbits (device) return at
+// once on the device, and `passes` bounds the launches when the host knows
+// the bits.
 int radix_sort(hipStream_t st, unsigned long long* ka, int* va, unsigned long long* kb, int* vb,
-               long n, unsigned* counts, int nblocks, const int* nbits_dev, unsigned* digits)
+               long n, unsigned* counts, int nblocks, const int* nbits_dev, unsigned* digits, int passes = 8)
 {
     if (n <= 0) return DEM_OK;
-    for (int shift = 0; shift < 64; shift += 8) {
+    for (int shift = 0; shift < 8 * passes; shift += 8) {
         const bool even = ((shift / 8) & 1) == 0;
         const unsigned long long* kin = even ? ka : kb;
         const int* vin = even ? va : vb;
@@ -2486,6 +2583,10 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
             HIP_TRY(hipMemcpy(s->tLocal, B->tri_local, sizeof(double) * 15 * s->nt, hipMemcpyHostToDevice),
                     "upload triangles");
         }
+        TRY(dalloc(s->tBits, 1, "hipMalloc"));
+        HIP_TRY(hipHostMalloc((void**)&s->hTri, sizeof(int) * (s->nt + 1), hipHostMallocDefault), "hipHostMalloc");
+        HIP_TRY(hipHostMalloc((void**)&s->hTriOff, sizeof(long long) * (s->nt + 1), hipHostMallocDefault),
+                "hipHostMalloc");
         TRY(dalloc(s->d64, 1, "hipMalloc"));
         const int sixtyfour = 64;
         HIP_TRY(hipMemcpy(s->d64, &sixtyfour, sizeof(int), hipMemcpyHostToDevice), "upload");
@@ -2500,6 +2601,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         TRY(dalloc(s->O.fzPart, (size_t)nb * nr, "hipMalloc(body load partials)"));
         TRY(dalloc(s->O.regime, (size_t)nb * nr, "hipMalloc(body regime state)"));
         TRY(dalloc(s->O.micro, (size_t)22 * nb * nr, "hipMalloc(body microstate)"));
+        TRY(dalloc(s->triWork, nr, "hipMalloc(triangle work list)"));
         s->redBlocks = (int)((nr + RED_TILE - 1) / RED_TILE);
         TRY(dalloc(s->redPartial, (size_t)22 * nb * s->redBlocks, "hipMalloc(reduction partials)"));
         if (s->keepBodyParticle) TRY(dalloc(s->O.bodyParticle, (size_t)nb * n * 6, "hipMalloc(body-particle forces)"));
@@ -2563,27 +2665,33 @@ TriSet triset(const dem_solver* s)
     TriSet T{};
     T.nt = s->nt; T.nb = s->nb; T.body = s->tBody; T.local = s->tLocal; T.world = s->tWorld;
     T.aabb = s->tAabb; T.pose = s->pose; T.entries = s->tEntries; T.tkey = s->tKeyA; T.ttri = s->tTriA;
+    for (int a = 0; a < 3; ++a) {
+        T.lo[a] = s->tLo[a];
+        T.dims[a] = s->tDims[a];
+    }
+    T.start = s->tStartCap > 0 ? s->tStart : nullptr;
     return T;
 }
 
 int build_triangle_grid(dem_solver* s)
 {
-    TriSet T = triset(s);
     hipStream_t st = s->stream;
+    s->tEntries = 0;
+    s->tStartCap = 0;
+    TriSet T = triset(s);
     hipLaunchKernelGGL(k_tri_count, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
                        s->tCounts, s->sc);
     TRY(launch_check("countTriangleCells"));
-    std::vector<int> counts(s->nt);
-    HIP_TRY(hipMemcpyAsync(counts.data(), s->tCounts, sizeof(int) * s->nt, hipMemcpyDeviceToHost, st),
+    int* counts = s->hTri;
+    HIP_TRY(hipMemcpyAsync(counts, s->tCounts, sizeof(int) * s->nt, hipMemcpyDeviceToHost, st),
             "copy triangle grid counts");
     HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
     HIP_TRY(hipStreamSynchronize(st), "triangle grid counts");
     unsigned long long total = 0;
-    std::vector<long long> off(s->nt);
+    long long* off = s->hTriOff;
     for (int i = 0; i < s->nt; ++i) { off[i] = (long long)total; total += (unsigned long long)counts[i]; }
     if (s->hScalars->tri_overflow != 0 || total > (unsigned long long)INT_MAX)
         return set_error(DEM_E_CAPACITY, "triangle grid entry count exceeds the supported 32-bit index range");
-    s->tEntries = (int)total;
     if ((long long)total > s->tCap) {
         for (void* p : {(void*)s->tKeyA, (void*)s->tKeyB, (void*)s->tTriA, (void*)s->tTriB, (void*)s->tsCounts})
             if (p) hipFree(p);
@@ -2597,15 +2705,43 @@ int build_triangle_grid(dem_solver* s)
         s->tCap = cap;
     }
     if (total == 0) return DEM_OK;
-    HIP_TRY(hipMemcpyAsync(s->tOffsets, off.data(), sizeof(long long) * s->nt, hipMemcpyHostToDevice, st),
+    s->tEntries = (int)total;
+    // cell box of all entries: compact x-major keys of ceil(log2 cells) bits
+    unsigned long long cells = 1;
+    for (int a = 0; a < 3; ++a) {
+        s->tLo[a] = s->hScalars->tri_lo[a];
+        s->tDims[a] = (long long)s->hScalars->tri_hi[a] - s->tLo[a] + 1;
+        cells *= (unsigned long long)s->tDims[a];  // each dim < 2^21: no overflow
+    }
+    int bits = 1;
+    while (bits < 63 && (1ull << bits) < cells) ++bits;
+    int* hbits = s->hTri + s->nt;
+    *hbits = bits;
+    HIP_TRY(hipMemcpyAsync(s->tBits, hbits, sizeof(int), hipMemcpyHostToDevice, st), "upload sort bits");
+    HIP_TRY(hipMemcpyAsync(s->tOffsets, off, sizeof(long long) * s->nt, hipMemcpyHostToDevice, st),
             "upload triangle offsets");
+    T = triset(s);
     hipLaunchKernelGGL(k_tri_fill, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
                        s->tCounts, s->tOffsets, s->tKeyA, s->tTriA);
     TRY(launch_check("fillTriangleCells"));
     const int nblk = (int)((total + RS_TILE - 1) / RS_TILE);
-    TRY(radix_sort(st, s->tKeyA, s->tTriA, s->tKeyB, s->tTriB, (long)total, s->tsCounts, nblk, s->d64,
-                   s->rsDigits));
-    HIP_TRY(hipStreamSynchronize(st), "triangle grid");  // host vector `off` goes out of scope
+    TRY(radix_sort(st, s->tKeyA, s->tTriA, s->tKeyB, s->tTriB, (long)total, s->tsCounts, nblk, s->tBits,
+                   s->rsDigits, (bits + 7) / 8));
+    // dense cell table when the box is small enough (else binary search)
+    constexpr unsigned long long kMaxTableCells = 1ull << 24;
+    if (cells <= kMaxTableCells) {
+        if ((long long)cells + 1 > s->tStartCapAlloc) {
+            if (s->tStart) hipFree(s->tStart);
+            s->tStart = nullptr;
+            s->tStartCapAlloc = (long long)cells + 1 + (long long)cells / 4;
+            TRY(dalloc(s->tStart, (size_t)s->tStartCapAlloc, "hipMalloc(triangle cell table)"));
+        }
+        const int tb = (int)std::min<unsigned long long>((cells + 1 + BLOCK - 1) / BLOCK, 65535ull);
+        hipLaunchKernelGGL(k_tri_start, dim3(tb), dim3(BLOCK), 0, st, s->tKeyA, (long)total, s->tStart,
+                           (long long)cells);
+        TRY(launch_check("triangle cell table"));
+        s->tStartCap = (long long)cells + 1;
+    }
     return DEM_OK;
 }
 
@@ -2652,13 +2788,22 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
         rebuild = s->hFlag[0] != 0;
     }
+    bool triDone = false;
     if (rebuild) {
         hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,
                            s->partScene, s->sceneAcc, s->dGrid, s->nscenes, s->cellCap, s->gridDone);
+        // with triangles the step reads counts back anyway: doing that now
+        // also brings this step's key bits, so the sort launches only its passes
+        int passes = 8;
+        if (s->nt > 0) {
+            TRY(build_triangle_grid(s));
+            triDone = true;
+            passes = (s->hScalars->nbits + 7) / 8;
+        }
         hipLaunchKernelGGL(k_keys, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dGrid, s->keyA, s->valA);
         TRY(launch_check("grid keys"));
         TRY(radix_sort(st, s->keyA, s->valA, s->keyB, s->valB, nr, s->rsCounts, s->rsBlocks, &s->sc->nbits,
-                       s->rsDigits));
+                       s->rsDigits, passes));
         hipLaunchKernelGGL(k_fixup, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, s->keyA, s->valA, nr,
                            R.i + (long)I_ORIG * nr);
         // with candidate lists the rows are put in cell order at every rebuild
@@ -2705,7 +2850,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             s->vlistValid = true;
         }
     }
-    if (s->nt > 0) TRY(build_triangle_grid(s));
+    if (s->nt > 0 && !triDone) TRY(build_triangle_grid(s));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
     const int step = s->stepIndex++;
     const int nblk = blocks_for(nr);
@@ -2755,7 +2900,9 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         hipLaunchKernelGGL(k_law_clump, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     } else {
         if (nb > 0) {
-            hipLaunchKernelGGL(k_law_pt, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+            hipLaunchKernelGGL(k_tri_sift, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A, s->triWork);
+            hipLaunchKernelGGL(k_law_pt, dim3((unsigned)((n + TRI_BLOCK - 1) / TRI_BLOCK)), dim3(TRI_BLOCK), 0, st, A,
+                               (const int*)s->triWork);
             TRY(launch_check("particle-triangle contact law"));
         }
         hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
@@ -2764,10 +2911,11 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[4], st), "event");
     const bool interval = A.pt.dt > 0.0;
     if (nb > 0) {
-        hipLaunchKernelGGL(k_observe, dim3(blocks_for((long)nb * nr)), dim3(BLOCK), 0, st, R, s->valA, s->H,
-                           A.T, s->O, A.pt, step, s->offs);
-        hipLaunchKernelGGL(k_regime, dim3(blocks_for((long)nb * nr)), dim3(BLOCK), 0, st, s->O.regime, nr, nb,
-                           s->regimeInstant, s->regimeInterval, interval ? 1 : 0);
+        // single-sphere owners: only the rows k_tri_sift listed; clumps: every row
+        const int* work = s->clump ? nullptr : s->triWork;
+        const dim3 rowGrid((unsigned)((nr + TRI_BLOCK - 1) / TRI_BLOCK), (unsigned)nb);
+        hipLaunchKernelGGL(k_regime, rowGrid, dim3(TRI_BLOCK), 0, st, s->O.regime, s->valA, nr, work,
+                           (const Scalars*)s->sc, s->regimeInstant, s->regimeInterval, interval ? 1 : 0);
         // body loads [nb*6] then fz [nb] (rows laid out contiguously)
         hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 6 * nb), dim3(BLOCK), 0, st, s->O.bodyPart, nr,
                            6 * nb, s->redPartial, s->redBlocks);
@@ -3112,7 +3260,19 @@ int dem_debug_triangle_grid(const dem_solver* s, uint64_t* keys, int32_t* tris)
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     HIP_TRY(hipStreamSynchronize(s->stream), "debug triangle grid");
     if (s->tEntries == 0) return DEM_OK;
-    if (keys) HIP_TRY(hipMemcpy(keys, s->tKeyA, sizeof(uint64_t) * s->tEntries, hipMemcpyDeviceToHost), "download");
+    if (keys) {
+        // compact box index -> the reference's packed 21-bit cell key
+        HIP_TRY(hipMemcpy(keys, s->tKeyA, sizeof(uint64_t) * s->tEntries, hipMemcpyDeviceToHost), "download");
+        const unsigned long long Y = (unsigned long long)s->tDims[1], Z = (unsigned long long)s->tDims[2];
+        const unsigned long long mask = (1ull << 21) - 1ull;
+        for (int k = 0; k < s->tEntries; ++k) {
+            const unsigned long long c = keys[k];
+            const long long x = (long long)(c / (Y * Z)) + s->tLo[0], y = (long long)((c / Z) % Y) + s->tLo[1],
+                            z = (long long)(c % Z) + s->tLo[2];
+            keys[k] = (((unsigned long long)(x + (1 << 20)) & mask) << 42) |
+                      (((unsigned long long)(y + (1 << 20)) & mask) << 21) | ((unsigned long long)(z + (1 << 20)) & mask);
+        }
+    }
     if (tris) HIP_TRY(hipMemcpy(tris, s->tTriA, sizeof(int) * s->tEntries, hipMemcpyDeviceToHost), "download");
     return DEM_OK;
 }

@@ -631,7 +631,8 @@ def rectangle_mesh(origin, u, v, nu, nv):
 
 def mixer_scene(n=200_000, seed=6, side=0.22, rmin=1.0e-3, rmax=2.0e-3, density=2500.0, E=1.0e7, mu=0.5,
                 restitution=0.5, dt=1.0e-6, steps=3000, muT=0.1, muR=0.1, beta=0.5, fill=0.3,
-                gravity=(0.0, 0.0, -9.81), blade_clear=2.0e-3, plate_gap=3.0e-3):
+                gravity=(0.0, 0.0, -9.81), blade_clear=2.0e-3, plate_gap=3.0e-3, packing="rsa",
+                snap=0.5, press=1.0e-3):
     """DEM-MBD coupled mixer: n spheres, radius U[rmin, rmax], seeded RSA in a
     square bin, and three triangle-mesh bodies:
       * the bin (fixed, state 1): floor 32 x 32 x 2 and four walls 16 x 16 x 2
@@ -641,9 +642,16 @@ def mixer_scene(n=200_000, seed=6, side=0.22, rmin=1.0e-3, rmax=2.0e-3, density=
         (2,048);
       * a plate (dynamic, state 0, 1 kg, identity inertia) resting above the
         bed, 16 x 32 x 2 triangles (1,024) facing down.
-    Spheres closer than blade_clear to the blade are left out of the packing;
-    the plate starts plate_gap above the highest sphere.  kN = E * mean
-    radius, epsS = epsN = restitution, no cohesion."""
+    packing="rsa": seeded random sequential insertion at volume fraction
+    `fill`; spheres closer than blade_clear to the blade are left out and the
+    plate starts plate_gap above the highest sphere (no initial contacts).
+    packing="dense": a settled-bed stand-in; each side of the blade plane
+    y = 0 gets half the spheres as a jittered lattice relaxed to near
+    contact (as the dense hopper), then every sphere within snap * r of the
+    floor, a bin wall or the blade plane is moved onto it with an overlap of
+    press * r, and the plate rests on the highest spheres under it, those
+    within snap * r of it pressed against it the same way.
+    kN = E * mean radius, epsS = epsN = restitution, no cohesion."""
     rng = np.random.RandomState(seed)
     h = 1.5 * side
     # bin: floor at z = 0, walls to z = h, origin at the floor centre
@@ -659,34 +667,56 @@ def mixer_scene(n=200_000, seed=6, side=0.22, rmin=1.0e-3, rmax=2.0e-3, density=
     paddle_rows = np.concatenate([
         rectangle_mesh((-0.5 * bw, 0.0, -0.5 * bh), (bw, 0, 0), (0, 0, bh), 16, 32),   # faces -y
         rectangle_mesh((-0.5 * bw, 0.0, -0.5 * bh), (0, 0, bh), (bw, 0, 0), 32, 16)])  # faces +y
-    # spheres: RSA in the bin, leaving out those that cut the blade slab
-    radius = np.sort(rng.uniform(rmin, rmax, int(n * 1.08) + 64))[::-1].copy()  # largest first
-    vol = float(np.sum(4.0 / 3.0 * np.pi * radius ** 3))
-    top = vol / (fill * side * side)
-    if top > h:
-        raise ValueError("the bin is too small for the bed")
-    lo = np.array([-a, -a, 0.0])
-    hi = np.array([a, a, top])
-    pos = rsa_positions(radius, lo, hi, seed=seed * 7919 + 5, max_attempts=20000)
-    perm = np.random.RandomState(seed + 1).permutation(len(radius))
-    radius, pos = radius[perm], pos[perm]
-    clear = blade_clear
-    cut = (np.abs(pos[:, 1]) < radius + clear) & (np.abs(pos[:, 0]) < 0.5 * bw + radius + clear) & \
-          (np.abs(pos[:, 2] - zc) < 0.5 * bh + radius + clear)
-    keep = np.nonzero(~cut)[0][:n]
-    if len(keep) < n:
-        raise RuntimeError("mixer packing placed too few spheres")
-    radius, pos = radius[keep].copy(), pos[keep].copy()
-    bed_top = float(np.max(pos[:, 2] + radius))
-    # plate above the bed, facing down, body centre at its middle
     pw, pl = 0.4 * side, 0.8 * side
+    if packing == "dense":
+        radius = rng.uniform(rmin, rmax, n)
+        halves = []
+        for k, (ylo, yhi) in enumerate(((-a, 0.0), (0.0, a))):
+            lo = np.array([-a, ylo, 0.0])
+            r, p = layered_lattice(radius[k::2], lo, (a, yhi), factor=0.985)
+            relax_packing(r, p, lo, (a, yhi), max_iter=60, tol=2e-3)
+            for axis, c, inward in ((0, -a, 1.0), (0, a, -1.0), (1, ylo, 1.0), (1, yhi, -1.0), (2, 0.0, 1.0)):
+                near = inward * (p[:, axis] - c) - r < snap * r
+                p[near, axis] = c + inward * r[near] * (1.0 - press)
+            halves.append((r, p))
+        radius = np.concatenate([halves[0][0], halves[1][0]])
+        pos = np.concatenate([halves[0][1], halves[1][1]])
+        if np.max(pos[:, 2] + radius) > h:
+            raise ValueError("the bin is too small for the bed")
+        under = (np.abs(pos[:, 0]) < 0.5 * pl) & (np.abs(pos[:, 1]) < 0.5 * pw)
+        plate_z = float(np.max(pos[under, 2] + radius[under]))
+        lift = under & (plate_z - pos[:, 2] - radius < snap * radius)
+        pos[lift, 2] = plate_z - radius[lift] * (1.0 - press)
+        perm = np.random.RandomState(seed + 1).permutation(n)
+        radius, pos = radius[perm].copy(), pos[perm].copy()
+    else:
+        # spheres: RSA in the bin, leaving out those that cut the blade slab
+        radius = np.sort(rng.uniform(rmin, rmax, int(n * 1.08) + 64))[::-1].copy()  # largest first
+        vol = float(np.sum(4.0 / 3.0 * np.pi * radius ** 3))
+        top = vol / (fill * side * side)
+        if top > h:
+            raise ValueError("the bin is too small for the bed")
+        lo = np.array([-a, -a, 0.0])
+        hi = np.array([a, a, top])
+        pos = rsa_positions(radius, lo, hi, seed=seed * 7919 + 5, max_attempts=20000)
+        perm = np.random.RandomState(seed + 1).permutation(len(radius))
+        radius, pos = radius[perm], pos[perm]
+        clear = blade_clear
+        cut = (np.abs(pos[:, 1]) < radius + clear) & (np.abs(pos[:, 0]) < 0.5 * bw + radius + clear) & \
+              (np.abs(pos[:, 2] - zc) < 0.5 * bh + radius + clear)
+        keep = np.nonzero(~cut)[0][:n]
+        if len(keep) < n:
+            raise RuntimeError("mixer packing placed too few spheres")
+        radius, pos = radius[keep].copy(), pos[keep].copy()
+        plate_z = float(np.max(pos[:, 2] + radius)) + plate_gap
+    # plate on or above the bed, facing down, body centre at its middle
     plate_rows = rectangle_mesh((-0.5 * pl, -0.5 * pw, 0.0), (0, pw, 0), (pl, 0, 0), 16, 32)
     kN = E * 0.5 * (rmin + rmax)
     m = mech(epsS=restitution, mu=mu, epsN=restitution, cohesion=0.0, beta=beta, muT=muT, muR=muR, kN=kN)
     ident = [1.0, 0.0, 0.0, 0.0]
     pose = np.array([[0.0, 0.0, 0.0, *ident, 0, 0, 0, 0, 0, 0],
                      [0.0, 0.0, zc, *ident, 0, 0, 0, 0.0, 0.0, 6.28],
-                     [0.0, 0.0, bed_top + plate_gap, *ident, 0, 0, 0, 0, 0, 0]])
+                     [0.0, 0.0, plate_z, *ident, 0, 0, 0, 0, 0, 0]])
     eye = np.eye(3)
     return Scene(**sphere_particles(pos, radius, density), pp=m, pt=dict(m), pw=dict(m), dt=dt,
                  gravity=tuple(gravity), gravity_enabled=1, steps=steps,

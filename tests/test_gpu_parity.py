@@ -155,3 +155,18 @@ def test_mixer_with_all_three_body_states(hip, oracle):
     assert o[0]["stats"]["body_contacts"] > 0 and o[60]["stats"]["particle_contacts"] > 0
     compare_frames(g, o)
 
+
+def test_dense_mixer_bed_pressed_onto_every_body(hip, oracle):
+    """The settled-bed mixer (4,000 spheres on the bin floor and walls, on
+    both faces of the turning blade and under the plate), 60 coupled steps:
+    the layout the coupled throughput figure runs, bit for bit."""
+    from dem_mbd_cuda_amd import scene
+
+    sc = scene.mixer_scene(n=4000, side=0.05, seed=10, dt=1e-5, packing="dense")
+    sc.keep_body_particle = 1
+    sc.keep_body_detail = 1
+    g = run_frames(hip, sc, steps=60, interval=20, debug=True)
+    o = run_frames(oracle, sc, steps=60, interval=20, debug=True)
+    assert o[0]["stats"]["body_contacts"] > 0.1 * sc.n
+    compare_frames(g, o)
+
