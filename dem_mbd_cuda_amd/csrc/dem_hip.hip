@@ -1342,19 +1342,35 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
         for (int body = 0; body < A.nb; ++body) {
             double total = 0.0, wD = 0.0;
             V3 wN = zero3(), wC = zero3();
-            // entries ks + j (j < 64) with a positive area scale: pt_law hits
-            // exactly these (its depth test follows from r^2 - d^2 > 1e-30)
-            unsigned long long hits = 0ull;
+            // This body's run [ks, ke): the entries ks + j (j < 64) whose
+            // bounding box meets the sphere (the contact circle lies on it) as
+            // bits of `cand`; a longer run's rest [kx, ke) is visited entry by
+            // entry.  The heavy work below then loops over set bits, lowest
+            // first, so every lane runs only its own few entries and the
+            // sums keep the entry order.
+            unsigned long long cand = 0ull;
             long ke = ks;
             for (; ke < e0; ++ke) {
                 const int tr = T.ttri[ke];
                 if (T.body[tr] != body) break;
+                if (ke - ks < 64 && ball_meets_box(I.pos, I.r, T.aabb + (long)tr * 6)) cand |= 1ull << (ke - ks);
+            }
+            const long kx = ke < ks + 64 ? ke : ks + 64;
+            // entries ks + j (j < 64) with a positive area scale: pt_law hits
+            // exactly these (its depth test follows from r^2 - d^2 > 1e-30)
+            unsigned long long hits = 0ull;
+            for (unsigned long long m = cand, kt = (unsigned long long)kx;;) {
+                long k;
+                if (m) { k = ks + __builtin_ctzll(m); m &= m - 1ull; }
+                else if ((long)kt < ke) k = (long)kt++;
+                else break;
+                const int tr = T.ttri[k];
                 const double* box = T.aabb + (long)tr * 6;
-                if (!ball_meets_box(I.pos, I.r, box)) continue;  // the contact circle lies on the sphere
+                if (!ball_meets_box(I.pos, I.r, box)) continue;
                 const Tri tri = load_tri(T, tr);
                 const double scv = tri_area_scale(I.pos, I.r, tri, box);
                 if (scv <= 0.0) continue;
-                if (ke - ks < 64) hits |= 1ull << (ke - ks);
+                if (k - ks < 64) hits |= 1ull << (k - ks);
                 const double distance = dot(tri.n, sub(I.pos, tri.a));
                 const V3 cc = sub(I.pos, scl(tri.n, distance));
                 total += scv;
@@ -1382,8 +1398,11 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
                     observed = true;
                     Material ro = A.pt;
                     ro.dt = 0.0;
-                    for (long k = ks; k < ke; ++k) {
-                        if (k - ks < 64 && ((hits >> (k - ks)) & 1ull) == 0ull) continue;
+                    for (unsigned long long m = hits, kt = (unsigned long long)kx;;) {
+                        long k;
+                        if (m) { k = ks + __builtin_ctzll(m); m &= m - 1ull; }
+                        else if ((long)kt < ke) k = (long)kt++;
+                        else break;
                         const Tri tri = load_tri(T, T.ttri[k]);
                         const PtOut q = pt_law(I, tri, P, ro, sv);
                         if (!q.hit) continue;
@@ -1636,28 +1655,42 @@ __global__ void __launch_bounds__(BLOCK) k_law_clump(LawArgs A)
 }
 
 // regime counters (accumulateBodyRegimeKernel, contact:530-567) of body
-// blockIdx.y: integer sums, counted per wave (ballots) and added with one
-// atomic per wave and counter, so any order gives the same result.  With
+// blockIdx.y: integer sums, so counting per thread, then per block, with one
+// atomic per block and counter gives the same result in any order.  With
 // `work`, over the rows listed by k_tri_sift (no other row is in contact);
 // else over every row.
-__global__ void __launch_bounds__(TRI_BLOCK) k_regime(const int* regime, const int* sorted, long n,
-                                                      const int* work, const Scalars* sc,
-                                                      unsigned long long* instant,
-                                                      unsigned long long* interval, int accumulate)
+constexpr int REGIME_BLOCKS = 64;
+__global__ void __launch_bounds__(BLOCK) k_regime(const int* regime, const int* sorted, long n,
+                                                  const int* work, const Scalars* sc,
+                                                  unsigned long long* instant,
+                                                  unsigned long long* interval, int accumulate)
 {
-    const long i = (long)blockIdx.x * TRI_BLOCK + threadIdx.x;
     const int body = blockIdx.y;
-    int st = 0;
-    if (work ? i < sc->tri_work : i < n) st = regime[(long)body * n + (work ? (long)sorted[work[i]] : i)];
-    const bool on = (st & 1) != 0;
-    const bool sl = on && (st & 2) != 0, tw = on && (st & 4) != 0;
+    const long lim = work ? (long)sc->tri_work : n;
     // total, primary regime (stick / slide / twist), slide, twist, both
-    const bool c[7] = {on, on && !sl && !tw, on && sl && !tw, tw, sl, tw, sl && tw};
+    unsigned c[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < lim; i += (long)gridDim.x * BLOCK) {
+        const int st = regime[(long)body * n + (work ? (long)sorted[work[i]] : i)];
+        if ((st & 1) == 0) continue;
+        const bool sl = (st & 2) != 0, tw = (st & 4) != 0;
+        ++c[0];
+        ++c[tw ? 3 : (sl ? 2 : 1)];
+        c[4] += sl;
+        c[5] += tw;
+        c[6] += sl && tw;
+    }
+    __shared__ unsigned part[BLOCK / 64][7];
     for (int k = 0; k < 7; ++k) {
-        const unsigned long long cnt = (unsigned long long)__popcll(__ballot(c[k]));
-        if ((threadIdx.x & 63) == 0 && cnt) {
-            atomicAdd(&instant[body * 7 + k], cnt);
-            if (accumulate) atomicAdd(&interval[body * 7 + k], cnt);
+        const unsigned v = wave_sum(c[k]);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        unsigned long long v = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) v += part[w][threadIdx.x];
+        if (v) {
+            atomicAdd(&instant[body * 7 + threadIdx.x], v);
+            if (accumulate) atomicAdd(&interval[body * 7 + threadIdx.x], v);
         }
     }
 }
@@ -1687,15 +1720,17 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const double* rows, long 
     }
 }
 
-__global__ void k_reduce_final(const double* partial, int nrows, int nblocks, double* out,
-                               int accumulate)
+// out[row] = sum of the row's partials in block order; rows from accFrom on
+// are also added to acc[row - accFrom] when acc is given.
+__global__ void k_reduce_final(const double* partial, int nrows, int nblocks, double* out, double* acc,
+                               int accFrom)
 {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= nrows) return;
     double s = 0.0;
     for (int b = 0; b < nblocks; ++b) s += partial[(long)row * nblocks + b];
-    if (accumulate) out[row] += s;
-    else out[row] = s;
+    out[row] = s;
+    if (acc && row >= accFrom) acc[row - accFrom] += s;
 }
 
 // ------------------------------------------------------------ leapfrog
@@ -2104,7 +2139,7 @@ struct dem_solver {
     unsigned long long* regimeInterval = nullptr;
     double* microInstant = nullptr;
     double* microInterval = nullptr;
-    double* bodyResult = nullptr;  // [nb*6 + nb]  loads, then fz per body
+    double* bodyResult = nullptr;  // [nb*6 + nb]  loads, then fz per body; microInstant follows
     double* redPartial = nullptr;
     int redBlocks = 0;
     double* dl = nullptr;  // download staging [max rows][n]
@@ -2139,10 +2174,10 @@ struct dem_solver {
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
-                        H2.pwVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart, O.fzPart,
-                        O.bodyParticle, O.bodyDetail, O.regime, O.micro, tBody, triWork, tLocal, tWorld, tAabb,
+                        H2.pwVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart,
+                        O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, tLocal, tWorld, tAabb,
                         pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
-                        regimeInstant, regimeInterval, microInstant, microInterval, bodyResult,
+                        regimeInstant, regimeInterval, microInterval, bodyResult,
                         redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tStart, tBits};
         for (void* p : ptrs)
             if (p) hipFree(p);
@@ -2592,18 +2627,19 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         HIP_TRY(hipMemcpy(s->d64, &sixtyfour, sizeof(int), hipMemcpyHostToDevice), "upload");
         TRY(dalloc(s->regimeInstant, (size_t)7 * nb, "hipMalloc(regime counters)"));
         TRY(dalloc(s->regimeInterval, (size_t)7 * nb, "hipMalloc(regime counters)"));
-        TRY(dalloc(s->microInstant, (size_t)22 * nb, "hipMalloc(microstate)"));
         TRY(dalloc(s->microInterval, (size_t)22 * nb, "hipMalloc(microstate)"));
         HIP_TRY(hipMemset(s->regimeInterval, 0, sizeof(unsigned long long) * 7 * nb), "clear regime");
         HIP_TRY(hipMemset(s->microInterval, 0, sizeof(double) * 22 * nb), "clear microstate");
-        TRY(dalloc(s->bodyResult, (size_t)7 * nb, "hipMalloc(body resultant)"));
-        TRY(dalloc(s->O.bodyPart, (size_t)6 * nb * nr, "hipMalloc(body load partials)"));
-        TRY(dalloc(s->O.fzPart, (size_t)nb * nr, "hipMalloc(body load partials)"));
+        // loads, Fz and microstate sums side by side (one reduction over 29 nb rows)
+        TRY(dalloc(s->bodyResult, (size_t)29 * nb, "hipMalloc(body resultant)"));
+        s->microInstant = s->bodyResult + 7 * nb;
+        TRY(dalloc(s->O.bodyPart, (size_t)29 * nb * nr, "hipMalloc(body load partials)"));
+        s->O.fzPart = s->O.bodyPart + (size_t)6 * nb * nr;
+        s->O.micro = s->O.bodyPart + (size_t)7 * nb * nr;
         TRY(dalloc(s->O.regime, (size_t)nb * nr, "hipMalloc(body regime state)"));
-        TRY(dalloc(s->O.micro, (size_t)22 * nb * nr, "hipMalloc(body microstate)"));
         TRY(dalloc(s->triWork, nr, "hipMalloc(triangle work list)"));
         s->redBlocks = (int)((nr + RED_TILE - 1) / RED_TILE);
-        TRY(dalloc(s->redPartial, (size_t)22 * nb * s->redBlocks, "hipMalloc(reduction partials)"));
+        TRY(dalloc(s->redPartial, (size_t)29 * nb * s->redBlocks, "hipMalloc(reduction partials)"));
         if (s->keepBodyParticle) TRY(dalloc(s->O.bodyParticle, (size_t)nb * n * 6, "hipMalloc(body-particle forces)"));
         if (s->keepBodyDetail) TRY(dalloc(s->O.bodyDetail, (size_t)nb * n * 19, "hipMalloc(body contact details)"));
         HIP_TRY(hipHostMalloc((void**)&s->hPose, sizeof(double) * 13 * nb, hipHostMallocDefault), "hipHostMalloc");
@@ -2913,25 +2949,14 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (nb > 0) {
         // single-sphere owners: only the rows k_tri_sift listed; clumps: every row
         const int* work = s->clump ? nullptr : s->triWork;
-        const dim3 rowGrid((unsigned)((nr + TRI_BLOCK - 1) / TRI_BLOCK), (unsigned)nb);
-        hipLaunchKernelGGL(k_regime, rowGrid, dim3(TRI_BLOCK), 0, st, s->O.regime, s->valA, nr, work,
+        hipLaunchKernelGGL(k_regime, dim3(REGIME_BLOCKS, (unsigned)nb), dim3(BLOCK), 0, st, s->O.regime, s->valA, nr, work,
                            (const Scalars*)s->sc, s->regimeInstant, s->regimeInterval, interval ? 1 : 0);
-        // body loads [nb*6] then fz [nb] (rows laid out contiguously)
-        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 6 * nb), dim3(BLOCK), 0, st, s->O.bodyPart, nr,
-                           6 * nb, s->redPartial, s->redBlocks);
-        hipLaunchKernelGGL(k_reduce_final, dim3((6 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 6 * nb, s->redBlocks,
-                           s->bodyResult, 0);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, nb), dim3(BLOCK), 0, st, s->O.fzPart, nr, nb,
-                           s->redPartial, s->redBlocks);
-        hipLaunchKernelGGL(k_reduce_final, dim3((nb + 255) / 256), dim3(256), 0, st, s->redPartial, nb, s->redBlocks,
-                           s->bodyResult + 6 * nb, 0);
-        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 22 * nb), dim3(BLOCK), 0, st, s->O.micro, nr,
-                           22 * nb, s->redPartial, s->redBlocks);
-        hipLaunchKernelGGL(k_reduce_final, dim3((22 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 22 * nb, s->redBlocks,
-                           s->microInstant, 0);
-        if (interval)
-            hipLaunchKernelGGL(k_reduce_final, dim3((22 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 22 * nb, s->redBlocks,
-                               s->microInterval, 1);
+        // body loads [6 nb], Fz [nb] and microstate [22 nb] rows, laid out side
+        // by side, in one reduction; the microstate sums also feed the interval
+        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 29 * nb), dim3(BLOCK), 0, st, s->O.bodyPart, nr,
+                           29 * nb, s->redPartial, s->redBlocks);
+        hipLaunchKernelGGL(k_reduce_final, dim3((29 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 29 * nb,
+                           s->redBlocks, s->bodyResult, interval ? s->microInterval : nullptr, 7 * nb);
         TRY(launch_check("body reductions"));
         ++s->intervalSteps;
         if (!interval) --s->intervalSteps;
