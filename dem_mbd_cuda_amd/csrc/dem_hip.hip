@@ -1940,12 +1940,157 @@ __global__ void __launch_bounds__(BLOCK) k_tri_fill(TriSet T, double inv, double
             }
 }
 
-// First entry of every cell of the triangle grid's box (cells + 1 entries).
-__global__ void __launch_bounds__(BLOCK) k_tri_start(const unsigned long long* key, long n, unsigned* start,
-                                                     long long cells)
+// Triangle grid as a counting sort over the cell box (boxes with a dense
+// table): one wave per triangle visits its cells in the x -> y -> z order of
+// fillTriangleCellsKernel; COUNT adds 1 to cur[cell], else the triangle is
+// placed at the cell's cursor (cur starts at the cell's first entry).
+template <bool COUNT>
+__global__ void __launch_bounds__(BLOCK) k_tri_cells(TriSet T, double inv, double pad, const int* counts,
+                                                     unsigned* cur, int* tri)
 {
-    for (long long c = (long long)blockIdx.x * BLOCK + threadIdx.x; c <= cells; c += (long long)gridDim.x * BLOCK)
-        start[c] = (unsigned)lower_bound_u64(key, n, (unsigned long long)c);
+    const int i = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    if (i >= T.nt) return;  // whole waves
+    const int c = counts[i];
+    const double* bb = T.aabb + (long)i * 6;
+    const int mnx = (int)floor((bb[0] - pad) * inv), mny = (int)floor((bb[1] - pad) * inv),
+              mnz = (int)floor((bb[2] - pad) * inv), mxy = (int)floor((bb[4] + pad) * inv),
+              mxz = (int)floor((bb[5] + pad) * inv);
+    const int ny = mxy - mny + 1, nz = mxz - mnz + 1;
+    for (int l = threadIdx.x & 63; l < c; l += 64) {
+        const int x = mnx + l / (ny * nz), y = mny + (l / nz) % ny, z = mnz + l % nz;
+        const unsigned long long key =
+            (((unsigned long long)(x - T.lo[0]) * (unsigned long long)T.dims[1] + (unsigned long long)(y - T.lo[1])) *
+             (unsigned long long)T.dims[2]) + (unsigned long long)(z - T.lo[2]);
+        if (COUNT) atomicAdd(&cur[key], 1u);
+        else tri[atomicAdd(&cur[key], 1u)] = i;
+    }
+}
+
+// Within each cell, entries in ascending triangle order (the reference's
+// stable sort of entries generated triangle by triangle).  A cell of up to
+// CELL_SORT entries is sorted in registers by its thread (odd-even
+// transposition network); larger cells are listed for k_tri_cell_order_big.
+constexpr int CELL_SORT = 16;
+__global__ void __launch_bounds__(BLOCK) k_tri_cell_order(const unsigned* start, long long cells, int* tri,
+                                                          int* bigCount, int* big)
+{
+    const long long c = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    bool large = false;
+    if (c < cells) {
+        const unsigned b = start[c], m = start[c + 1] - b;
+        if (m > (unsigned)CELL_SORT) {
+            large = true;
+        } else if (m > 1) {
+            int v[CELL_SORT];
+#pragma unroll
+            for (int j = 0; j < CELL_SORT; ++j) v[j] = (unsigned)j < m ? tri[b + j] : INT_MAX;
+#pragma unroll
+            for (int r = 0; r < CELL_SORT; ++r)
+#pragma unroll
+                for (int j = r & 1; j + 1 < CELL_SORT; j += 2) {
+                    const int lo = min(v[j], v[j + 1]), hi = max(v[j], v[j + 1]);
+                    v[j] = lo;
+                    v[j + 1] = hi;
+                }
+#pragma unroll
+            for (int j = 0; j < CELL_SORT; ++j)
+                if ((unsigned)j < m) tri[b + j] = v[j];
+        }
+    }
+    wave_append(large, (int)c, bigCount, big);
+}
+
+// One 64-lane block per listed cell (blocks stride over the list): each
+// entry goes to its rank among the cell's (distinct) triangle indices,
+// through `tmp`, then back.
+__global__ void __launch_bounds__(64) k_tri_cell_order_big(const unsigned* start, const int* bigCount,
+                                                           const int* big, int* tri, int* tmp)
+{
+    const int count = *bigCount;
+    for (int k = blockIdx.x; k < count; k += gridDim.x) {
+        const long long c = big[k];
+        const unsigned b = start[c], m = start[c + 1] - b;
+        for (unsigned i = threadIdx.x; i < m; i += 64) {
+            const int v = tri[b + i];
+            unsigned rank = 0;
+            for (unsigned j = 0; j < m; ++j) rank += tri[b + j] < v;
+            tmp[b + rank] = v;
+        }
+        __syncthreads();
+        for (unsigned i = threadIdx.x; i < m; i += 64) tri[b + i] = tmp[b + i];
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of n unsigned values: tiles of SCAN_TILE per block (block
+// scan of per-thread sums), a one-block scan of the tile totals, then the
+// tile offsets added (and, with `copy`, the result duplicated there).
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned& total)
+{
+    __shared__ unsigned wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    unsigned before = 0;
+    total = 0;
+    for (int k = 0; k < BLOCK / 64; ++k) {
+        if (k < w) before += wsum[k];
+        total += wsum[k];
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+__global__ void __launch_bounds__(BLOCK) k_scan_tiles(const unsigned* in, long n, unsigned* out, unsigned* tileSum)
+{
+    const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
+    unsigned v[SCAN_ITEMS];
+    unsigned sum = 0;
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        v[j] = base + j < n ? in[base + j] : 0u;
+        sum += v[j];
+    }
+    unsigned total;
+    unsigned run = block_exclusive_scan(sum, total);
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0) tileSum[blockIdx.x] = total;
+}
+__global__ void __launch_bounds__(BLOCK) k_scan_top(unsigned* tileSum, long tiles)
+{
+    unsigned carry = 0;
+    for (long base = 0; base < tiles; base += SCAN_TILE) {
+        const long b = base + (long)threadIdx.x * SCAN_ITEMS;
+        unsigned v[SCAN_ITEMS];
+        unsigned sum = 0;
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            v[j] = b + j < tiles ? tileSum[b + j] : 0u;
+            sum += v[j];
+        }
+        unsigned total;
+        unsigned run = carry + block_exclusive_scan(sum, total);
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            if (b + j < tiles) tileSum[b + j] = run;
+            run += v[j];
+        }
+        carry += total;
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_scan_add(unsigned* out, long n, const unsigned* tileOff, unsigned* copy)
+{
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * BLOCK) {
+        const unsigned v = out[i] + tileOff[i / SCAN_TILE];
+        out[i] = v;
+        if (copy) copy[i] = v;
+    }
 }
 
 // ------------------------------------------------------------ downloads
@@ -2129,6 +2274,8 @@ struct dem_solver {
     int tLo[3] = {0, 0, 0};        // cell box of the triangle grid entries
     long long tDims[3] = {0, 0, 0};
     unsigned* tStart = nullptr;    // dense cell -> first entry table of that box
+    unsigned* tCur = nullptr;      // per-cell counts, then fill cursors (same capacity)
+    unsigned* tScan = nullptr;     // tile totals of the cell-table scan
     long long tStartCap = 0;       // entries of the table valid this step (0: binary search)
     long long tStartCapAlloc = 0;
     int* tBits = nullptr;          // device: key bits of the triangle sort
@@ -2178,7 +2325,7 @@ struct dem_solver {
                         O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, tLocal, tWorld, tAabb,
                         pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
                         regimeInstant, regimeInterval, microInterval, bodyResult,
-                        redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tStart, tBits};
+                        redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tStart, tCur, tScan, tBits};
         for (void* p : ptrs)
             if (p) hipFree(p);
         if (hPose) hipHostFree(hPose);
@@ -2749,6 +2896,46 @@ int build_triangle_grid(dem_solver* s)
         s->tDims[a] = (long long)s->hScalars->tri_hi[a] - s->tLo[a] + 1;
         cells *= (unsigned long long)s->tDims[a];  // each dim < 2^21: no overflow
     }
+    constexpr unsigned long long kMaxTableCells = 1ull << 24;
+    if (cells <= kMaxTableCells) {
+        // counting sort over the box; its scan is the dense cell table
+        if ((long long)cells + 1 > s->tStartCapAlloc) {
+            for (void* p : {(void*)s->tStart, (void*)s->tCur, (void*)s->tScan})
+                if (p) hipFree(p);
+            s->tStart = s->tCur = s->tScan = nullptr;
+            s->tStartCapAlloc = (long long)cells + 1 + (long long)cells / 4;
+            TRY(dalloc(s->tStart, (size_t)s->tStartCapAlloc, "hipMalloc(triangle cell table)"));
+            TRY(dalloc(s->tCur, (size_t)s->tStartCapAlloc, "hipMalloc(triangle cell table)"));
+            TRY(dalloc(s->tScan, (size_t)((s->tStartCapAlloc + SCAN_TILE - 1) / SCAN_TILE), "hipMalloc(triangle cell table)"));
+        }
+        const long tableN = (long)cells + 1;
+        const int fillBlocks = (int)std::min<long>((tableN + BLOCK - 1) / BLOCK, 4096L);
+        const unsigned triBlocks = (unsigned)((s->nt + BLOCK / 64 - 1) / (BLOCK / 64));
+        T = triset(s);
+        hipLaunchKernelGGL(k_fill_u32, dim3(fillBlocks), dim3(BLOCK), 0, st, s->tCur, tableN);
+        hipLaunchKernelGGL(k_tri_cells<true>, dim3(triBlocks), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
+                           (const int*)s->tCounts, s->tCur, (int*)nullptr);
+        const long tiles = (tableN + SCAN_TILE - 1) / SCAN_TILE;
+        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(BLOCK), 0, st, (const unsigned*)s->tCur, tableN,
+                           s->tStart, s->tScan);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, st, s->tScan, tiles);
+        hipLaunchKernelGGL(k_scan_add, dim3(fillBlocks), dim3(BLOCK), 0, st, s->tStart, tableN,
+                           (const unsigned*)s->tScan, s->tCur);
+        hipLaunchKernelGGL(k_tri_cells<false>, dim3(triBlocks), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
+                           (const int*)s->tCounts, s->tCur, s->tTriA);
+        // the cursors are spent: tCur holds the list of large cells, its
+        // count sits in the last slot (cells <= 2^24 leaves it unused)
+        int* bigCount = (int*)s->tCur + s->tStartCapAlloc - 1;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (unsigned*)bigCount, 1L);
+        hipLaunchKernelGGL(k_tri_cell_order, dim3((unsigned)((cells + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st,
+                           (const unsigned*)s->tStart, (long long)cells, s->tTriA, bigCount, (int*)s->tCur);
+        hipLaunchKernelGGL(k_tri_cell_order_big, dim3(1024), dim3(64), 0, st, (const unsigned*)s->tStart,
+                           (const int*)bigCount, (const int*)s->tCur, s->tTriA, s->tTriB);
+        TRY(launch_check("fillTriangleCells"));
+        s->tStartCap = tableN;
+        return DEM_OK;
+    }
+    // a box too large for a table: sorted (key, triangle) pairs, searched
     int bits = 1;
     while (bits < 63 && (1ull << bits) < cells) ++bits;
     int* hbits = s->hTri + s->nt;
@@ -2763,21 +2950,6 @@ int build_triangle_grid(dem_solver* s)
     const int nblk = (int)((total + RS_TILE - 1) / RS_TILE);
     TRY(radix_sort(st, s->tKeyA, s->tTriA, s->tKeyB, s->tTriB, (long)total, s->tsCounts, nblk, s->tBits,
                    s->rsDigits, (bits + 7) / 8));
-    // dense cell table when the box is small enough (else binary search)
-    constexpr unsigned long long kMaxTableCells = 1ull << 24;
-    if (cells <= kMaxTableCells) {
-        if ((long long)cells + 1 > s->tStartCapAlloc) {
-            if (s->tStart) hipFree(s->tStart);
-            s->tStart = nullptr;
-            s->tStartCapAlloc = (long long)cells + 1 + (long long)cells / 4;
-            TRY(dalloc(s->tStart, (size_t)s->tStartCapAlloc, "hipMalloc(triangle cell table)"));
-        }
-        const int tb = (int)std::min<unsigned long long>((cells + 1 + BLOCK - 1) / BLOCK, 65535ull);
-        hipLaunchKernelGGL(k_tri_start, dim3(tb), dim3(BLOCK), 0, st, s->tKeyA, (long)total, s->tStart,
-                           (long long)cells);
-        TRY(launch_check("triangle cell table"));
-        s->tStartCap = (long long)cells + 1;
-    }
     return DEM_OK;
 }
 
@@ -3287,7 +3459,16 @@ int dem_debug_triangle_grid(const dem_solver* s, uint64_t* keys, int32_t* tris)
     if (s->tEntries == 0) return DEM_OK;
     if (keys) {
         // compact box index -> the reference's packed 21-bit cell key
-        HIP_TRY(hipMemcpy(keys, s->tKeyA, sizeof(uint64_t) * s->tEntries, hipMemcpyDeviceToHost), "download");
+        if (s->tStartCap > 0) {
+            // counting-sort layout: cell c holds entries [start[c], start[c + 1])
+            std::vector<unsigned> start((size_t)s->tStartCap);
+            HIP_TRY(hipMemcpy(start.data(), s->tStart, sizeof(unsigned) * start.size(), hipMemcpyDeviceToHost),
+                    "download");
+            for (size_t c = 0; c + 1 < start.size(); ++c)
+                for (unsigned k = start[c]; k < start[c + 1]; ++k) keys[k] = c;
+        } else {
+            HIP_TRY(hipMemcpy(keys, s->tKeyA, sizeof(uint64_t) * s->tEntries, hipMemcpyDeviceToHost), "download");
+        }
         const unsigned long long Y = (unsigned long long)s->tDims[1], Z = (unsigned long long)s->tDims[2];
         const unsigned long long mask = (1ull << 21) - 1ull;
         for (int k = 0; k < s->tEntries; ++k) {
