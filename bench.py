@@ -264,6 +264,8 @@ def main():
     ap.add_argument("--no-coupled", action="store_true", help="skip the DEM-MBD coupled mixer figure")
     ap.add_argument("--coupled-n", type=int, default=200_000)
     ap.add_argument("--coupled-steps", type=int, default=300)
+    ap.add_argument("--only-coupled", action="store_true",
+                    help="measure only the coupled mixer and print its own JSON line (for profiling it alone)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter-sweep figure")
     ap.add_argument("--sweep-scenes", type=int, default=256)
     ap.add_argument("--sweep-steps", type=int, default=100)
@@ -282,6 +284,14 @@ def main():
     import dem_mbd_cuda_amd as dm
     from dem_mbd_cuda_amd import scene
 
+    if args.only_coupled:
+        c = measure_coupled(dm, scene, args.coupled_n, args.coupled_steps, 50)
+        print(json.dumps({"metric": "FP64 particle-steps/s, coupled DEM-MBD mixer", "value": c["particle_steps_per_s"],
+                          "unit": "particle-steps/s", "n_gpus": 1, "steps": c["steps"], "warmup": c["warmup"],
+                          "ms_per_step": c["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded radii and packing)",
+                          "config": {k: v for k, v in c.items() if k not in ("ms_per_step", "particle_steps_per_s")}}))
+        return
     if args.copy_calibration:
         copy_calibration()
     head = measure_hopper(dm, scene, args, args.packing, args.steps, args.warmup, args.timing_steps)

@@ -417,26 +417,15 @@ struct PtOut {
     bool hit;
 };
 
-// bodyContact with the manifold history read-only (dt = 0 in m).
-DEM_FN PtOut pt_law(const Body& I, const Tri& t, const Pose& P, const Material& m,
-                               const double* slot)
+// bodyContact (contact:708-853) past its geometry: the triangle's outward
+// normal tn, the plane distance, the contact centre cc and the area scale sc
+// (> 1e-12) as pt_law forms them.
+DEM_FN PtOut pt_contact(const Body& I, V3 tn, double distance, V3 cc, double sc, const Pose& P,
+                        const Material& m, const double* slot)
 {
     PtOut o;
-    o.hit = false;
-    V3 normal = t.n;
-    const double distance = dot(normal, sub(I.pos, t.a));
-    if (distance <= 0.0) return o;
     const double depth = I.r - distance;
-    if (depth <= 0.0) return o;
-    const double crs = I.r * I.r - distance * distance;
-    if (crs <= 1.0e-30) return o;
-    const double cr = sqrt(crs);
-    const V3 cc = sub(I.pos, scl(normal, distance));
-    const double area = circle_triangle_area(cc, normal, t.a, t.b, t.c, cr);
-    if (area <= 0.0) return o;
-    const double sc = fmin(1.0, area / (kPi * crs));
-    if (sc <= 1.0e-12) return o;
-    normal = scl(normal, -1.0);
+    const V3 normal = scl(tn, -1.0);
     const V3 arm = sub(cc, P.c);
     const V3 bv = add(P.v, cross(P.w, arm));
     const V3 rv = sub(bv, I.vel);
@@ -474,6 +463,27 @@ DEM_FN PtOut pt_law(const Body& I, const Tri& t, const Pose& P, const Material& 
     o.depth = depth;
     o.hit = true;
     return o;
+}
+
+// bodyContact with the manifold history read-only (dt = 0 in m).
+DEM_FN PtOut pt_law(const Body& I, const Tri& t, const Pose& P, const Material& m,
+                               const double* slot)
+{
+    PtOut o;
+    o.hit = false;
+    const double distance = dot(t.n, sub(I.pos, t.a));
+    if (distance <= 0.0) return o;
+    const double depth = I.r - distance;
+    if (depth <= 0.0) return o;
+    const double crs = I.r * I.r - distance * distance;
+    if (crs <= 1.0e-30) return o;
+    const double cr = sqrt(crs);
+    const V3 cc = sub(I.pos, scl(t.n, distance));
+    const double area = circle_triangle_area(cc, t.n, t.a, t.b, t.c, cr);
+    if (area <= 0.0) return o;
+    const double sc = fmin(1.0, area / (kPi * crs));
+    if (sc <= 1.0e-12) return o;
+    return pt_contact(I, t.n, distance, cc, sc, P, m, slot);
 }
 
 // advanceBodyManifoldHistory (contact:855-949) after the slot is found.

@@ -1312,21 +1312,128 @@ __device__ __forceinline__ void micro_terms(const Body& I, const Pose& P, const 
     for (int f = 0; f < 22; ++f) ms[(long)f * stride] = v[f];
 }
 
+// Per-triangle geometry of one contact row for tri_pass, visited body by
+// body in entry order (a cell's entries are in triangle order and a body's
+// triangles are contiguous in body order, so each body's triangles form one
+// run of the cell's entries).  manifold(body, f) calls f(n, scale, distance,
+// centre) for the body's triangles with a positive area scale; contacts(...)
+// then calls g(q) with the read-only contact of each of them.
+//
+// EntryGeo evaluates the geometry in the lane: the run's entries whose
+// bounding box meets the sphere (the contact circle lies on it) as bits of a
+// mask (the first 64; a longer run's rest is visited entry by entry), then
+// the area scale of each, lowest bit first.  pt_law hits exactly the entries
+// with a positive scale (its depth test follows from r^2 - d^2 > 1e-30).
+struct EntryGeo {
+    const TriSet& T;
+    V3 pos;
+    double r;
+    long e0, ks, ke, kx;
+    unsigned long long hits;
+    __device__ EntryGeo(const TriSet& T_, V3 p, double r_, long b0, long e0_)
+        : T(T_), pos(p), r(r_), e0(e0_), ks(b0), ke(b0), kx(b0), hits(0ull) {}
+    template <class F>
+    __device__ __forceinline__ void manifold(int body, F&& f)
+    {
+        unsigned long long cand = 0ull;
+        ke = ks;
+        for (; ke < e0; ++ke) {
+            const int tr = T.ttri[ke];
+            if (T.body[tr] != body) break;
+            if (ke - ks < 64 && ball_meets_box(pos, r, T.aabb + (long)tr * 6)) cand |= 1ull << (ke - ks);
+        }
+        kx = ke < ks + 64 ? ke : ks + 64;
+        hits = 0ull;
+        for (unsigned long long m = cand, kt = (unsigned long long)kx;;) {
+            long k;
+            if (m) { k = ks + __builtin_ctzll(m); m &= m - 1ull; }
+            else if ((long)kt < ke) k = (long)kt++;
+            else break;
+            const int tr = T.ttri[k];
+            const double* box = T.aabb + (long)tr * 6;
+            if (!ball_meets_box(pos, r, box)) continue;
+            const Tri tri = load_tri(T, tr);
+            const double scv = tri_area_scale(pos, r, tri, box);
+            if (scv <= 0.0) continue;
+            if (k - ks < 64) hits |= 1ull << (k - ks);
+            const double distance = dot(tri.n, sub(pos, tri.a));
+            f(tri.n, scv, distance, sub(pos, scl(tri.n, distance)));
+        }
+    }
+    template <class G>
+    __device__ __forceinline__ void contacts(const Body& I, const Pose& P, const Material& ro, const double* sv, G&& g)
+    {
+        for (unsigned long long m = hits, kt = (unsigned long long)kx;;) {
+            long k;
+            if (m) { k = ks + __builtin_ctzll(m); m &= m - 1ull; }
+            else if ((long)kt < ke) k = (long)kt++;
+            else break;
+            const PtOut q = pt_law(I, load_tri(T, T.ttri[k]), P, ro, sv);
+            if (q.hit) g(q);
+        }
+    }
+    __device__ __forceinline__ void next() { ks = ke; }
+};
+
+// HitGeo reads the row's hits as k_tri_area stored them: the entries with a
+// positive area scale as a mask over the cell's entries, and (scale, plane
+// distance) per hit in entry order; normal and centre are formed as above.
+struct HitGeo {
+    const TriSet& T;
+    V3 pos;
+    long b0;
+    unsigned long long rest, run;  // hits not yet visited; the current body's run
+    const double2* val;
+    int rank, runRank;
+    __device__ HitGeo(const TriSet& T_, V3 p, long b0_, unsigned long long hits, const double2* v)
+        : T(T_), pos(p), b0(b0_), rest(hits), run(0ull), val(v), rank(0), runRank(0) {}
+    __device__ __forceinline__ V3 normal(int tr) const
+    {
+        const double* w = T.world + (long)tr * 15;
+        return mk(w[12], w[13], w[14]);
+    }
+    template <class F>
+    __device__ __forceinline__ void manifold(int body, F&& f)
+    {
+        run = 0ull;
+        runRank = rank;
+        while (rest) {
+            const int tr = T.ttri[b0 + __builtin_ctzll(rest)];
+            if (T.body[tr] != body) break;
+            const V3 n = normal(tr);
+            const double2 v = val[rank];
+            f(n, v.x, v.y, sub(pos, scl(n, v.y)));
+            run |= rest & (~rest + 1ull);
+            rest &= rest - 1ull;
+            ++rank;
+        }
+    }
+    template <class G>
+    __device__ __forceinline__ void contacts(const Body& I, const Pose& P, const Material& ro, const double* sv, G&& g)
+    {
+        int k = runRank;
+        for (unsigned long long m = run; m; m &= m - 1ull, ++k) {
+            const V3 n = normal(T.ttri[b0 + __builtin_ctzll(m)]);
+            const double2 v = val[k];
+            g(pt_contact(I, n, v.y, sub(I.pos, scl(n, v.y)), v.x, P, ro, sv));
+        }
+    }
+    __device__ __forceinline__ void next() {}
+};
+
 // Particle-triangle part of computeContactsKernel (contact:1148-1208) for
-// contact row r at sorted position t (history row hr, owner input row o):
-// per body, the area-weighted manifold over the triangles of the row's own
-// cell advances one history slot, then every triangle contributes its
-// read-only contact.  Adds the row's force and torque to bF / bT and the
-// owner's per-body outputs at o; writes the row's body load and Fz at t
-// (reduced over the rows later), its microstate terms at t and its regime
-// state at r; depthBits keeps the largest contact depth (as bits) for one
-// atomic per wave.
-// A cell's entries are in triangle order and a body's triangles are
-// contiguous in body order, so each body's triangles form one run of the
-// cell's entries: one walk over the entries visits the bodies in order.
-__device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r, long t, long hr, long o,
-                                         V3& bF, V3& bT, int& bodyHigh, int& ptOver, int& bcount,
-                                         unsigned long long& depthBits)
+// contact row r at sorted position t (history row hr, owner input row o),
+// over the row's cell entries [b0, e0) with geometry from `geo`: per body,
+// the area-weighted manifold advances one history slot, then every triangle
+// contributes its read-only contact.  Adds the row's force and torque to
+// bF / bT and the owner's per-body outputs at o; writes the row's body load
+// and Fz at t (reduced over the rows later), its microstate terms at t and
+// its regime state at r; depthBits keeps the largest contact depth (as bits)
+// for one atomic per wave.
+template <class Geo>
+__device__ __forceinline__ void tri_pass_geo(const LawArgs& A, const Body& I, long r, long t, long hr, long o,
+                                             bool listed, Geo& geo, V3& bF, V3& bT, int& bodyHigh, int& ptOver,
+                                             int& bcount, unsigned long long& depthBits)
 {
     const Soa& R = A.R;
     const long nr = R.n, no = A.S.n;
@@ -1335,49 +1442,16 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
     unsigned long long ptOcc = H.ptOcc[hr] & H.ptSeen[(long)((s + 1) & 1) * nr + hr];
     unsigned long long ptCur = 0;
     const TriSet& T = A.T;
-    if (T.nt > 0 && T.entries > 0 && R.k(I_VALID, r)) {
-        long b0, e0;
-        tri_range(T, R.k(I_CX, r), R.k(I_CY, r), R.k(I_CZ, r), b0, e0);
-        long ks = b0;  // first entry of a body >= the current one
+    if (listed) {
         for (int body = 0; body < A.nb; ++body) {
             double total = 0.0, wD = 0.0;
             V3 wN = zero3(), wC = zero3();
-            // This body's run [ks, ke): the entries ks + j (j < 64) whose
-            // bounding box meets the sphere (the contact circle lies on it) as
-            // bits of `cand`; a longer run's rest [kx, ke) is visited entry by
-            // entry.  The heavy work below then loops over set bits, lowest
-            // first, so every lane runs only its own few entries and the
-            // sums keep the entry order.
-            unsigned long long cand = 0ull;
-            long ke = ks;
-            for (; ke < e0; ++ke) {
-                const int tr = T.ttri[ke];
-                if (T.body[tr] != body) break;
-                if (ke - ks < 64 && ball_meets_box(I.pos, I.r, T.aabb + (long)tr * 6)) cand |= 1ull << (ke - ks);
-            }
-            const long kx = ke < ks + 64 ? ke : ks + 64;
-            // entries ks + j (j < 64) with a positive area scale: pt_law hits
-            // exactly these (its depth test follows from r^2 - d^2 > 1e-30)
-            unsigned long long hits = 0ull;
-            for (unsigned long long m = cand, kt = (unsigned long long)kx;;) {
-                long k;
-                if (m) { k = ks + __builtin_ctzll(m); m &= m - 1ull; }
-                else if ((long)kt < ke) k = (long)kt++;
-                else break;
-                const int tr = T.ttri[k];
-                const double* box = T.aabb + (long)tr * 6;
-                if (!ball_meets_box(I.pos, I.r, box)) continue;
-                const Tri tri = load_tri(T, tr);
-                const double scv = tri_area_scale(I.pos, I.r, tri, box);
-                if (scv <= 0.0) continue;
-                if (k - ks < 64) hits |= 1ull << (k - ks);
-                const double distance = dot(tri.n, sub(I.pos, tri.a));
-                const V3 cc = sub(I.pos, scl(tri.n, distance));
+            geo.manifold(body, [&](V3 n, double scv, double distance, V3 cc) {
                 total += scv;
-                wN = add(wN, scl(tri.n, -scv));
+                wN = add(wN, scl(n, -scv));
                 wC = add(wC, scl(cc, scv));
                 wD += distance * scv;
-            }
+            });
             double bl[6] = {0, 0, 0, 0, 0, 0};
             double fz = 0.0;
             double* ms = A.O.micro + (long)body * 22 * nr + t;
@@ -1398,14 +1472,7 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
                     observed = true;
                     Material ro = A.pt;
                     ro.dt = 0.0;
-                    for (unsigned long long m = hits, kt = (unsigned long long)kx;;) {
-                        long k;
-                        if (m) { k = ks + __builtin_ctzll(m); m &= m - 1ull; }
-                        else if ((long)kt < ke) k = (long)kt++;
-                        else break;
-                        const Tri tri = load_tri(T, T.ttri[k]);
-                        const PtOut q = pt_law(I, tri, P, ro, sv);
-                        if (!q.hit) continue;
+                    geo.contacts(I, P, ro, sv, [&](const PtOut& q) {
                         bF = add(bF, q.pf);
                         bT = add(bT, q.ptq);
                         const V3 btq = add(add(q.bnt, q.bft), add(q.btw, q.brl));
@@ -1434,14 +1501,14 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
                             }
                             if (d[18] < q.depth) d[18] = q.depth;
                         }
-                    }
+                    });
                 }
             }
             for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * nr + t] = bl[k];
             A.O.fzPart[(long)body * nr + t] = fz;
             if (!observed)
                 for (int f = 0; f < 22; ++f) ms[(long)f * nr] = 0.0;
-            ks = ke;
+            geo.next();
         }
     } else {
         for (int body = 0; body < A.nb; ++body) {
@@ -1452,6 +1519,19 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
     }
     H.ptOcc[hr] = ptOcc;
     H.ptSeen[(long)(s & 1) * nr + hr] = ptCur;
+}
+
+// tri_pass_geo with the geometry evaluated in the lane (rows of any size).
+__device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r, long t, long hr, long o,
+                                         V3& bF, V3& bT, int& bodyHigh, int& ptOver, int& bcount,
+                                         unsigned long long& depthBits)
+{
+    const TriSet& T = A.T;
+    long b0 = 0, e0 = 0;
+    const bool listed = T.nt > 0 && T.entries > 0 && A.R.k(I_VALID, r);
+    if (listed) tri_range(T, A.R.k(I_CX, r), A.R.k(I_CY, r), A.R.k(I_CZ, r), b0, e0);
+    EntryGeo geo(T, I.pos, I.r, b0, e0);
+    tri_pass_geo(A, I, r, t, hr, o, listed, geo, bF, bT, bodyHigh, ptOver, bcount, depthBits);
 }
 
 // Force outputs of owner row p (contact:1211-1220) and its acceleration
@@ -1578,11 +1658,61 @@ __global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work)
     wave_append(near, (int)t, &A.sc->tri_work, work);
 }
 
-// Triangle contacts (tri_pass) of the single-sphere owners listed by
+// Triangle geometry of the rows listed by k_tri_sift, one wave per row and
+// one lane per entry of the row's cell (waves stride over the list): the
+// bounding-box test and area scale of EntryGeo for all of the row's
+// triangles at once.  Stores the row's hit mask (entries with a positive
+// area scale) and (scale, plane distance) per hit in entry order; a row with
+// more than 64 entries or TRI_HITS hits gets hitN = -1 and is evaluated in
+// the lane by k_law_pt.
+constexpr int TRI_HITS = 16;
+constexpr int TRI_AREA_BLOCKS = 2048;
+__global__ void __launch_bounds__(BLOCK) k_tri_area(LawArgs A, const int* work, unsigned long long* hitMask,
+                                                    int* hitN, double2* hitVal)
+{
+    const int lane = threadIdx.x & 63;
+    const long count = A.sc->tri_work;
+    for (long i = (long)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); i < count;
+         i += (long)gridDim.x * (BLOCK / 64)) {
+        const long t = work[i];
+        const long p = A.sorted[t];
+        long b0, e0;
+        tri_range(A.T, A.S.k(I_CX, p), A.S.k(I_CY, p), A.S.k(I_CZ, p), b0, e0);
+        if (e0 - b0 > 64) {
+            if (lane == 0) hitN[i] = -1;
+            continue;
+        }
+        const double4 cr = A.posr[t];
+        const V3 pos = mk(cr.x, cr.y, cr.z);
+        double scv = 0.0, distance = 0.0;
+        if (lane < e0 - b0) {
+            const int tr = A.T.ttri[b0 + lane];
+            const double* box = A.T.aabb + (long)tr * 6;
+            if (ball_meets_box(pos, cr.w, box)) {
+                const Tri tri = load_tri(A.T, tr);
+                scv = tri_area_scale(pos, cr.w, tri, box);
+                if (scv > 0.0) distance = dot(tri.n, sub(pos, tri.a));
+            }
+        }
+        const unsigned long long hits = __ballot(scv > 0.0);
+        const int nh = __popcll(hits);
+        if (lane == 0) {
+            hitMask[i] = hits;
+            hitN[i] = nh <= TRI_HITS ? nh : -1;
+        }
+        if (scv > 0.0 && nh <= TRI_HITS)
+            hitVal[i * TRI_HITS + __popcll(hits & ((1ull << lane) - 1ull))] = make_double2(scv, distance);
+    }
+}
+
+// Triangle contacts (tri_pass_geo) of the single-sphere owners listed by
 // k_tri_sift, one per lane in 64-lane blocks (a few hundred waves spread
-// over every CU).  Writes the body accumulator (bF, bT) read by k_law.
+// over every CU), from the hits k_tri_area stored (or in the lane for the
+// rows it left).  Writes the body accumulator (bF, bT) read by k_law.
 constexpr int TRI_BLOCK = 64;
-__global__ void __launch_bounds__(TRI_BLOCK) k_law_pt(LawArgs A, const int* work)
+__global__ void __launch_bounds__(TRI_BLOCK) k_law_pt(LawArgs A, const int* work,
+                                                      const unsigned long long* hitMask, const int* hitN,
+                                                      const double2* hitVal)
 {
     const long i = (long)blockIdx.x * TRI_BLOCK + threadIdx.x;
     const Soa& S = A.S;
@@ -1595,7 +1725,15 @@ __global__ void __launch_bounds__(TRI_BLOCK) k_law_pt(LawArgs A, const int* work
         Body I = rec_body(A.rec, p);
         if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
         V3 bF = zero3(), bT = zero3();
-        tri_pass(A, I, p, t, S.k(I_HIST, p), S.k(I_ORIG, p), bF, bT, bodyHigh, ptOver, bcount, depthBits);
+        const long hr = S.k(I_HIST, p), o = S.k(I_ORIG, p);
+        if (hitN[i] >= 0) {
+            long b0, e0;
+            tri_range(A.T, S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p), b0, e0);
+            HitGeo geo(A.T, I.pos, b0, hitMask[i], hitVal + i * TRI_HITS);
+            tri_pass_geo(A, I, p, t, hr, o, true, geo, bF, bT, bodyHigh, ptOver, bcount, depthBits);
+        } else {
+            tri_pass(A, I, p, t, hr, o, bF, bT, bodyHigh, ptOver, bcount, depthBits);
+        }
         double* ob = A.O.body;
         ob[0 * n + p] = bF.x; ob[1 * n + p] = bF.y; ob[2 * n + p] = bF.z;
         ob[3 * n + p] = bT.x; ob[4 * n + p] = bT.y; ob[5 * n + p] = bT.z;
@@ -2256,6 +2394,9 @@ struct dem_solver {
     // triangles
     int* tBody = nullptr;
     int* triWork = nullptr;  // [nr] sorted positions of the rows near triangles
+    unsigned long long* triHitMask = nullptr;  // [nr] per listed row: entries with a positive area scale
+    int* triHitN = nullptr;                    // [nr] hits stored, or -1 (evaluated in the lane)
+    double2* triHitVal = nullptr;              // [nr][TRI_HITS] (area scale, plane distance)
     double* tLocal = nullptr;
     double* tWorld = nullptr;
     double* tAabb = nullptr;
@@ -2322,7 +2463,7 @@ struct dem_solver {
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
                         H2.pwVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart,
-                        O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, tLocal, tWorld, tAabb,
+                        O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, triHitMask, triHitN, triHitVal, tLocal, tWorld, tAabb,
                         pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
                         regimeInstant, regimeInterval, microInterval, bodyResult,
                         redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tStart, tCur, tScan, tBits};
@@ -2785,6 +2926,9 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         s->O.micro = s->O.bodyPart + (size_t)7 * nb * nr;
         TRY(dalloc(s->O.regime, (size_t)nb * nr, "hipMalloc(body regime state)"));
         TRY(dalloc(s->triWork, nr, "hipMalloc(triangle work list)"));
+        TRY(dalloc(s->triHitMask, nr, "hipMalloc(triangle work list)"));
+        TRY(dalloc(s->triHitN, nr, "hipMalloc(triangle work list)"));
+        TRY(dalloc(s->triHitVal, (size_t)TRI_HITS * nr, "hipMalloc(triangle work list)"));
         s->redBlocks = (int)((nr + RED_TILE - 1) / RED_TILE);
         TRY(dalloc(s->redPartial, (size_t)29 * nb * s->redBlocks, "hipMalloc(reduction partials)"));
         if (s->keepBodyParticle) TRY(dalloc(s->O.bodyParticle, (size_t)nb * n * 6, "hipMalloc(body-particle forces)"));
@@ -3109,8 +3253,11 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     } else {
         if (nb > 0) {
             hipLaunchKernelGGL(k_tri_sift, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A, s->triWork);
+            hipLaunchKernelGGL(k_tri_area, dim3(TRI_AREA_BLOCKS), dim3(BLOCK), 0, st, A, (const int*)s->triWork,
+                               s->triHitMask, s->triHitN, s->triHitVal);
             hipLaunchKernelGGL(k_law_pt, dim3((unsigned)((n + TRI_BLOCK - 1) / TRI_BLOCK)), dim3(TRI_BLOCK), 0, st, A,
-                               (const int*)s->triWork);
+                               (const int*)s->triWork, (const unsigned long long*)s->triHitMask,
+                               (const int*)s->triHitN, (const double2*)s->triHitVal);
             TRY(launch_check("particle-triangle contact law"));
         }
         hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
