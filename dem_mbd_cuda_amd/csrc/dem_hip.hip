@@ -139,6 +139,7 @@ struct Scalars {
     int need_rebuild;   // some component moved more than skin/2 since the last rebuild
     int cand_overflow;  // a rebuild listed more partners than CAND_MAX
     int tri_work;       // rows listed by k_tri_sift (their cell holds triangle entries)
+    unsigned long long tri_total;  // triangle grid entries
 };
 
 struct History {
@@ -300,6 +301,7 @@ __global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc,
         sc->body_max_depth_bits = 0ull;
         sc->tri_overflow = 0;
         sc->tri_work = 0;
+        sc->tri_total = 0ull;
         for (int a = 0; a < 3; ++a) {
             sc->tri_lo[a] = INT_MAX;
             sc->tri_hi[a] = INT_MIN;
@@ -1627,10 +1629,11 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
 }
 
 // Single-sphere owners whose cell holds no triangle entry: tri_pass's
-// no-contact path (triangle-slot expiry, zero body load, Fz and body
-// accumulator) and zero microstate rows.  The others are listed in `work`
-// (sorted positions) for k_law_pt and k_regime.
-__global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work)
+// no-contact path (triangle-slot expiry, zero body accumulator); their body
+// load, Fz and microstate rows are zero, which listed[t] = 0 tells the
+// reduction instead of storing them.  The others are listed in `work`
+// (sorted positions) for k_tri_area, k_law_pt and k_regime.
+__global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work, unsigned char* listed)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const Soa& S = A.S;
@@ -1642,16 +1645,12 @@ __global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work)
         if (A.T.nt > 0 && A.T.entries > 0 && S.k(I_VALID, p))
             tri_range(A.T, S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p), b0, e0);
         near = e0 > b0;
+        listed[t] = near ? 1 : 0;
         if (!near) {
             const History& H = A.H;
             const long hr = S.k(I_HIST, p);
             H.ptOcc[hr] = H.ptOcc[hr] & H.ptSeen[(long)((A.step + 1) & 1) * n + hr];
             H.ptSeen[(long)(A.step & 1) * n + hr] = 0ull;
-            for (int body = 0; body < A.nb; ++body) {
-                for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * n + t] = 0.0;
-                A.O.fzPart[(long)body * n + t] = 0.0;
-                for (int f = 0; f < 22; ++f) A.O.micro[((long)body * 22 + f) * n + t] = 0.0;
-            }
             for (int k = 0; k < 6; ++k) A.O.body[(long)k * n + p] = 0.0;
         }
     }
@@ -1834,10 +1833,11 @@ __global__ void __launch_bounds__(BLOCK) k_regime(const int* regime, const int* 
 }
 
 // Deterministic row sums: partial[row][block] = tree sum of rows[row][block
-// tile]; fixed order independent of scheduling.
+// tile]; fixed order independent of scheduling.  With `keep`, element e of
+// every row counts as +0.0 where keep[e] is 0 (rows not stored there).
 constexpr int RED_TILE = 4096;
 __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const double* rows, long n, int nrows,
-                                                       double* partial, int nblocks)
+                                                       double* partial, int nblocks, const unsigned char* keep)
 {
     const int row = blockIdx.y;
     const double* r = rows + (long)row * n;
@@ -1845,7 +1845,7 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const double* rows, long 
     double s = 0.0;
     for (int k = 0; k < RED_TILE / BLOCK; ++k) {
         const long e = base + (long)k * BLOCK + threadIdx.x;
-        if (e < n) s += r[e];
+        if (e < n) s += (!keep || keep[e]) ? r[e] : 0.0;
     }
     s = wave_sum(s);
     __shared__ double ws[BLOCK / 64];
@@ -2019,35 +2019,52 @@ __global__ void __launch_bounds__(BLOCK) k_transform(TriSet T)
     bb[0] = mnx; bb[1] = mny; bb[2] = mnz; bb[3] = mxx; bb[4] = mxy; bb[5] = mxz;
 }
 
-// countTriangleCellsKernel (contact:409-451)
+// countTriangleCellsKernel (contact:409-451): per triangle, the cells of its
+// padded bounding box; the box of all of them and the entry total are
+// reduced per wave, then one atomic per wave and quantity.
 __global__ void __launch_bounds__(BLOCK) k_tri_count(TriSet T, double inv, double pad, int* counts,
                                                      Scalars* sc)
 {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= T.nt) return;
-    const double* bb = T.aabb + (long)i * 6;
-    const double mnx = floor((bb[0] - pad) * inv), mny = floor((bb[1] - pad) * inv),
-                 mnz = floor((bb[2] - pad) * inv), mxx = floor((bb[3] + pad) * inv),
-                 mxy = floor((bb[4] + pad) * inv), mxz = floor((bb[5] + pad) * inv);
-    const double minCell = -(double)(1 << 20), maxCell = (double)((1 << 20) - 1);
-    if (!isfinite(mnx) || !isfinite(mny) || !isfinite(mnz) || !isfinite(mxx) || !isfinite(mxy) ||
-        !isfinite(mxz) || mnx < minCell || mny < minCell || mnz < minCell || mxx > maxCell ||
-        mxy > maxCell || mxz > maxCell) {
-        atomicAdd(&sc->tri_overflow, 1);
-        counts[i] = 0;
-        return;
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    unsigned long long cnt = 0;
+    int over = 0;
+    if (i < T.nt) {
+        const double* bb = T.aabb + (long)i * 6;
+        const double mnx = floor((bb[0] - pad) * inv), mny = floor((bb[1] - pad) * inv),
+                     mnz = floor((bb[2] - pad) * inv), mxx = floor((bb[3] + pad) * inv),
+                     mxy = floor((bb[4] + pad) * inv), mxz = floor((bb[5] + pad) * inv);
+        const double minCell = -(double)(1 << 20), maxCell = (double)((1 << 20) - 1);
+        long long nx = 0, ny = 0, nz = 0;
+        bool bad = !isfinite(mnx) || !isfinite(mny) || !isfinite(mnz) || !isfinite(mxx) || !isfinite(mxy) ||
+                   !isfinite(mxz) || mnx < minCell || mny < minCell || mnz < minCell || mxx > maxCell ||
+                   mxy > maxCell || mxz > maxCell;
+        if (!bad) {
+            nx = (long long)(int)mxx - (int)mnx + 1;
+            ny = (long long)(int)mxy - (int)mny + 1;
+            nz = (long long)(int)mxz - (int)mnz + 1;
+            bad = nx <= 0 || ny <= 0 || nz <= 0 || nx > LLONG_MAX / ny || nx * ny > LLONG_MAX / nz ||
+                  nx * ny * nz > INT_MAX;
+        }
+        if (bad) {
+            over = 1;
+            counts[i] = 0;
+        } else {
+            counts[i] = (int)(nx * ny * nz);
+            cnt = (unsigned long long)(nx * ny * nz);
+            lo[0] = (int)mnx; lo[1] = (int)mny; lo[2] = (int)mnz;
+            hi[0] = (int)mxx; hi[1] = (int)mxy; hi[2] = (int)mxz;
+        }
     }
-    const long long nx = (long long)(int)mxx - (int)mnx + 1, ny = (long long)(int)mxy - (int)mny + 1,
-                    nz = (long long)(int)mxz - (int)mnz + 1;
-    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > LLONG_MAX / ny || nx * ny > LLONG_MAX / nz ||
-        nx * ny * nz > INT_MAX) {
-        atomicAdd(&sc->tri_overflow, 1);
-        counts[i] = 0;
-        return;
+    atomic_add_wave(&sc->tri_overflow, over);
+    const bool lead = (threadIdx.x & 63) == 0;
+    for (int a = 0; a < 3; ++a) {
+        const int l = wave_min(lo[a]), h = wave_max(hi[a]);
+        if (lead && l != INT_MAX) atomicMin(&sc->tri_lo[a], l);
+        if (lead && h != INT_MIN) atomicMax(&sc->tri_hi[a], h);
     }
-    counts[i] = (int)(nx * ny * nz);
-    atomicMin(&sc->tri_lo[0], (int)mnx); atomicMin(&sc->tri_lo[1], (int)mny); atomicMin(&sc->tri_lo[2], (int)mnz);
-    atomicMax(&sc->tri_hi[0], (int)mxx); atomicMax(&sc->tri_hi[1], (int)mxy); atomicMax(&sc->tri_hi[2], (int)mxz);
+    cnt = wave_sum(cnt);
+    if (lead && cnt) atomicAdd(&sc->tri_total, cnt);
 }
 
 // fillTriangleCellsKernel (contact:453-476): x -> y -> z nested order.  The
@@ -2397,6 +2414,7 @@ struct dem_solver {
     unsigned long long* triHitMask = nullptr;  // [nr] per listed row: entries with a positive area scale
     int* triHitN = nullptr;                    // [nr] hits stored, or -1 (evaluated in the lane)
     double2* triHitVal = nullptr;              // [nr][TRI_HITS] (area scale, plane distance)
+    unsigned char* triListed = nullptr;        // [nr] by sorted position: listed by k_tri_sift
     double* tLocal = nullptr;
     double* tWorld = nullptr;
     double* tAabb = nullptr;
@@ -2463,7 +2481,7 @@ struct dem_solver {
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
                         H2.pwVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart,
-                        O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, triHitMask, triHitN, triHitVal, tLocal, tWorld, tAabb,
+                        O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, triHitMask, triHitN, triHitVal, triListed, tLocal, tWorld, tAabb,
                         pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
                         regimeInstant, regimeInterval, microInterval, bodyResult,
                         redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tStart, tCur, tScan, tBits};
@@ -2929,6 +2947,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         TRY(dalloc(s->triHitMask, nr, "hipMalloc(triangle work list)"));
         TRY(dalloc(s->triHitN, nr, "hipMalloc(triangle work list)"));
         TRY(dalloc(s->triHitVal, (size_t)TRI_HITS * nr, "hipMalloc(triangle work list)"));
+        TRY(dalloc(s->triListed, nr, "hipMalloc(triangle work list)"));
         s->redBlocks = (int)((nr + RED_TILE - 1) / RED_TILE);
         TRY(dalloc(s->redPartial, (size_t)29 * nb * s->redBlocks, "hipMalloc(reduction partials)"));
         if (s->keepBodyParticle) TRY(dalloc(s->O.bodyParticle, (size_t)nb * n * 6, "hipMalloc(body-particle forces)"));
@@ -3009,14 +3028,9 @@ int build_triangle_grid(dem_solver* s)
     hipLaunchKernelGGL(k_tri_count, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
                        s->tCounts, s->sc);
     TRY(launch_check("countTriangleCells"));
-    int* counts = s->hTri;
-    HIP_TRY(hipMemcpyAsync(counts, s->tCounts, sizeof(int) * s->nt, hipMemcpyDeviceToHost, st),
-            "copy triangle grid counts");
     HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
     HIP_TRY(hipStreamSynchronize(st), "triangle grid counts");
-    unsigned long long total = 0;
-    long long* off = s->hTriOff;
-    for (int i = 0; i < s->nt; ++i) { off[i] = (long long)total; total += (unsigned long long)counts[i]; }
+    const unsigned long long total = s->hScalars->tri_total;
     if (s->hScalars->tri_overflow != 0 || total > (unsigned long long)INT_MAX)
         return set_error(DEM_E_CAPACITY, "triangle grid entry count exceeds the supported 32-bit index range");
     if ((long long)total > s->tCap) {
@@ -3079,7 +3093,14 @@ int build_triangle_grid(dem_solver* s)
         s->tStartCap = tableN;
         return DEM_OK;
     }
-    // a box too large for a table: sorted (key, triangle) pairs, searched
+    // a box too large for a table: sorted (key, triangle) pairs, searched;
+    // each triangle writes its entries from its offset in triangle order
+    int* counts = s->hTri;
+    HIP_TRY(hipMemcpyAsync(counts, s->tCounts, sizeof(int) * s->nt, hipMemcpyDeviceToHost, st),
+            "copy triangle grid counts");
+    HIP_TRY(hipStreamSynchronize(st), "triangle grid counts");
+    long long* off = s->hTriOff;
+    for (long long i = 0, acc = 0; i < s->nt; ++i) { off[i] = acc; acc += counts[i]; }
     int bits = 1;
     while (bits < 63 && (1ull << bits) < cells) ++bits;
     int* hbits = s->hTri + s->nt;
@@ -3252,7 +3273,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         hipLaunchKernelGGL(k_law_clump, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     } else {
         if (nb > 0) {
-            hipLaunchKernelGGL(k_tri_sift, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A, s->triWork);
+            hipLaunchKernelGGL(k_tri_sift, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A, s->triWork, s->triListed);
             hipLaunchKernelGGL(k_tri_area, dim3(TRI_AREA_BLOCKS), dim3(BLOCK), 0, st, A, (const int*)s->triWork,
                                s->triHitMask, s->triHitN, s->triHitVal);
             hipLaunchKernelGGL(k_law_pt, dim3((unsigned)((n + TRI_BLOCK - 1) / TRI_BLOCK)), dim3(TRI_BLOCK), 0, st, A,
@@ -3273,7 +3294,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         // body loads [6 nb], Fz [nb] and microstate [22 nb] rows, laid out side
         // by side, in one reduction; the microstate sums also feed the interval
         hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 29 * nb), dim3(BLOCK), 0, st, s->O.bodyPart, nr,
-                           29 * nb, s->redPartial, s->redBlocks);
+                           29 * nb, s->redPartial, s->redBlocks, work ? (const unsigned char*)s->triListed : nullptr);
         hipLaunchKernelGGL(k_reduce_final, dim3((29 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 29 * nb,
                            s->redBlocks, s->bodyResult, interval ? s->microInterval : nullptr, 7 * nb);
         TRY(launch_check("body reductions"));

@@ -170,3 +170,34 @@ def test_dense_mixer_bed_pressed_onto_every_body(hip, oracle):
     assert o[0]["stats"]["body_contacts"] > 0.1 * sc.n
     compare_frames(g, o)
 
+
+
+def test_triangle_grid_box_beyond_the_cell_table(hip, oracle):
+    """A fixed plate and a dynamic lid 2 m apart: the triangle grid's cell
+    box (about 800^3 cells of 2.5 mm) is too large for the dense cell table,
+    so the grid is built as sorted (key, triangle) pairs and searched.
+    Spheres start pressed into the plate and the lid comes down onto a
+    sphere below it; 40 coupled steps, bit for bit, including the grid."""
+    from dem_mbd_cuda_amd import scene
+
+    rng = np.random.RandomState(13)
+    plate = scene.rectangle_mesh((-0.01, -0.01, 0.0), (0.02, 0, 0), (0, 0.02, 0), 4, 4)  # faces up
+    lid = scene.rectangle_mesh((-0.005, -0.005, 0.0), (0, 0.01, 0), (0.01, 0, 0), 2, 2)  # faces down
+    # 15 spheres pressed 0.05 mm into the first plate, one 0.05 mm below the lid
+    pos = np.column_stack([rng.uniform(-0.008, 0.008, 16), rng.uniform(-0.008, 0.008, 16), np.full(16, 0.95e-3)])
+    pos[15] = [2.0, 2.0, 2.0 - 1.05e-3]
+    P = scene.sphere_particles(pos, np.full(16, 1e-3), density=2500.0)
+    m = scene.mech(epsS=0.5, mu=0.4, epsN=0.5, cohesion=0.0, beta=0.5, muT=0.1, muR=0.1, kN=2.0e4)
+    ident = [1.0, 0.0, 0.0, 0.0]
+    pose = np.array([[0.0, 0.0, 0.0, *ident, 0, 0, 0, 0, 0, 0],
+                     [2.0, 2.0, 2.0, *ident, 0, 0, -0.2, 0, 0, 0]])
+    sc = scene.Scene(**P, pp=m, pt=dict(m), pw=dict(m), dt=1e-5,
+                     gravity=(0.0, 0.0, -9.81), gravity_enabled=1, steps=40,
+                     body_mass=np.array([100.0, 0.01]), body_pose=pose,
+                     body_tri_count=np.array([len(plate), len(lid)], np.int32),
+                     tri_local=np.concatenate([plate, lid]), body_state=np.array([1, 0], np.int32),
+                     body_inertia=np.stack([np.eye(3), np.eye(3) * 1e-6]))
+    g = run_frames(hip, sc, steps=40, interval=10, debug=True)
+    o = run_frames(oracle, sc, steps=40, interval=10, debug=True)
+    assert o[0]["stats"]["body_contacts"] > 0
+    compare_frames(g, o)
