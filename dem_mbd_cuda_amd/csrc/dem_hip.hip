@@ -72,6 +72,7 @@ int set_error(int code, const char* fmt, ...)
 constexpr int PP_SLOTS = DEM_PARTICLE_HISTORY_SLOTS;
 constexpr int PW_SLOTS = DEM_WALL_HISTORY_SLOTS;
 constexpr int PT_SLOTS = DEM_BODY_HISTORY_SLOTS;
+constexpr int BODY_ROW = 29;  // per (body, row): 6 load components, Fz, 22 microstate terms
 constexpr int MAX_LIST = PP_SLOTS;  // contact-list capacity per component
 constexpr int SLOT_BITS = 6;        // list entry = partner row << SLOT_BITS | slot
 static_assert((1 << SLOT_BITS) == PP_SLOTS, "slot field width");
@@ -185,12 +186,11 @@ struct Outputs {
     double* wall;     // [6][n]
     double* body;     // [6][n]
     double* field;    // [3][n]
-    double* bodyPart; // [nb][6][n] per-particle body load (sorted index)
-    double* fzPart;   // [n]
+    double* bodyRows;            // [nb][nr][BODY_ROW] per (body, component input row): load, Fz, microstate
+    unsigned char* bodyTouched;  // [nb][nr] the row holds values for the body (else they are zero)
     double* bodyParticle;  // [nb][n_orig][6] (optional)
     double* bodyDetail;    // [nb][n_orig][19] (optional)
     int* regime;           // [nb][n] by physical index
-    double* micro;         // [nb][22][n] by physical index
 };
 
 // ---------------------------------------------------------------- helpers
@@ -1424,16 +1424,16 @@ struct HitGeo {
 };
 
 // Particle-triangle part of computeContactsKernel (contact:1148-1208) for
-// contact row r at sorted position t (history row hr, owner input row o),
-// over the row's cell entries [b0, e0) with geometry from `geo`: per body,
-// the area-weighted manifold advances one history slot, then every triangle
+// contact row r (history row hr, owner input row o, component input row u),
+// over the row's cell entries with geometry from `geo`: per body, the
+// area-weighted manifold advances one history slot, then every triangle
 // contributes its read-only contact.  Adds the row's force and torque to
-// bF / bT and the owner's per-body outputs at o; writes the row's body load
-// and Fz at t (reduced over the rows later), its microstate terms at t and
-// its regime state at r; depthBits keeps the largest contact depth (as bits)
-// for one atomic per wave.
+// bF / bT and the owner's per-body outputs at o; writes the row's body load,
+// Fz and microstate terms at u (reduced over the component input rows later)
+// and its regime state at r; depthBits keeps the largest contact depth (as
+// bits) for one atomic per wave.
 template <class Geo>
-__device__ __forceinline__ void tri_pass_geo(const LawArgs& A, const Body& I, long r, long t, long hr, long o,
+__device__ __forceinline__ void tri_pass_geo(const LawArgs& A, const Body& I, long r, long u, long hr, long o,
                                              bool listed, Geo& geo, V3& bF, V3& bT, int& bodyHigh, int& ptOver,
                                              int& bcount, unsigned long long& depthBits)
 {
@@ -1456,7 +1456,7 @@ __device__ __forceinline__ void tri_pass_geo(const LawArgs& A, const Body& I, lo
             });
             double bl[6] = {0, 0, 0, 0, 0, 0};
             double fz = 0.0;
-            double* ms = A.O.micro + (long)body * 22 * nr + t;
+            double* br = A.O.bodyRows + ((long)body * nr + u) * BODY_ROW;
             bool observed = false;
             if (total > 0.0) {
                 const V3 mN = unit(wN);
@@ -1470,7 +1470,7 @@ __device__ __forceinline__ void tri_pass_geo(const LawArgs& A, const Body& I, lo
                     bool slide = false, twisty = false;
                     manifold_law(I, P, A.pt, mN, mC, mD, sv, &slide, &twisty);
                     A.O.regime[(long)body * nr + r] = 1 | (slide ? 2 : 0) | (twisty ? 4 : 0);
-                    micro_terms(I, P, A.pt, mN, mC, mD, total, sv, ms, nr);
+                    micro_terms(I, P, A.pt, mN, mC, mD, total, sv, br + 7, 1);
                     observed = true;
                     Material ro = A.pt;
                     ro.dt = 0.0;
@@ -1506,25 +1506,26 @@ __device__ __forceinline__ void tri_pass_geo(const LawArgs& A, const Body& I, lo
                     });
                 }
             }
-            for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * nr + t] = bl[k];
-            A.O.fzPart[(long)body * nr + t] = fz;
-            if (!observed)
-                for (int f = 0; f < 22; ++f) ms[(long)f * nr] = 0.0;
+            // a body with no manifold contributes zeros: its row is marked
+            // untouched instead of written
+            if (total > 0.0) {
+                for (int k = 0; k < 6; ++k) br[k] = bl[k];
+                br[6] = fz;
+                if (!observed)
+                    for (int f = 0; f < 22; ++f) br[7 + f] = 0.0;
+            }
+            A.O.bodyTouched[(long)body * nr + u] = total > 0.0 ? 1 : 0;
             geo.next();
         }
     } else {
-        for (int body = 0; body < A.nb; ++body) {
-            for (int k = 0; k < 6; ++k) A.O.bodyPart[((long)body * 6 + k) * nr + t] = 0.0;
-            A.O.fzPart[(long)body * nr + t] = 0.0;
-            for (int f = 0; f < 22; ++f) A.O.micro[((long)body * 22 + f) * nr + t] = 0.0;
-        }
+        for (int body = 0; body < A.nb; ++body) A.O.bodyTouched[(long)body * nr + u] = 0;
     }
     H.ptOcc[hr] = ptOcc;
     H.ptSeen[(long)(s & 1) * nr + hr] = ptCur;
 }
 
 // tri_pass_geo with the geometry evaluated in the lane (rows of any size).
-__device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r, long t, long hr, long o,
+__device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r, long u, long hr, long o,
                                          V3& bF, V3& bT, int& bodyHigh, int& ptOver, int& bcount,
                                          unsigned long long& depthBits)
 {
@@ -1533,7 +1534,7 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
     const bool listed = T.nt > 0 && T.entries > 0 && A.R.k(I_VALID, r);
     if (listed) tri_range(T, A.R.k(I_CX, r), A.R.k(I_CY, r), A.R.k(I_CZ, r), b0, e0);
     EntryGeo geo(T, I.pos, I.r, b0, e0);
-    tri_pass_geo(A, I, r, t, hr, o, listed, geo, bF, bT, bodyHigh, ptOver, bcount, depthBits);
+    tri_pass_geo(A, I, r, u, hr, o, listed, geo, bF, bT, bodyHigh, ptOver, bcount, depthBits);
 }
 
 // Force outputs of owner row p (contact:1211-1220) and its acceleration
@@ -1629,11 +1630,10 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
 }
 
 // Single-sphere owners whose cell holds no triangle entry: tri_pass's
-// no-contact path (triangle-slot expiry, zero body accumulator); their body
-// load, Fz and microstate rows are zero, which listed[t] = 0 tells the
-// reduction instead of storing them.  The others are listed in `work`
-// (sorted positions) for k_tri_area, k_law_pt and k_regime.
-__global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work, unsigned char* listed)
+// no-contact path (triangle-slot expiry, zero body accumulator, body rows
+// marked untouched).  The others are listed in `work` (sorted positions) for
+// k_tri_area, k_law_pt and k_regime.
+__global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const Soa& S = A.S;
@@ -1645,8 +1645,9 @@ __global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work, unsign
         if (A.T.nt > 0 && A.T.entries > 0 && S.k(I_VALID, p))
             tri_range(A.T, S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p), b0, e0);
         near = e0 > b0;
-        listed[t] = near ? 1 : 0;
         if (!near) {
+            const long u = S.k(I_ORIG, p);
+            for (int body = 0; body < A.nb; ++body) A.O.bodyTouched[(long)body * n + u] = 0;
             const History& H = A.H;
             const long hr = S.k(I_HIST, p);
             H.ptOcc[hr] = H.ptOcc[hr] & H.ptSeen[(long)((A.step + 1) & 1) * n + hr];
@@ -1729,9 +1730,9 @@ __global__ void __launch_bounds__(TRI_BLOCK) k_law_pt(LawArgs A, const int* work
             long b0, e0;
             tri_range(A.T, S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p), b0, e0);
             HitGeo geo(A.T, I.pos, b0, hitMask[i], hitVal + i * TRI_HITS);
-            tri_pass_geo(A, I, p, t, hr, o, true, geo, bF, bT, bodyHigh, ptOver, bcount, depthBits);
+            tri_pass_geo(A, I, p, o, hr, o, true, geo, bF, bT, bodyHigh, ptOver, bcount, depthBits);
         } else {
-            tri_pass(A, I, p, t, hr, o, bF, bT, bodyHigh, ptOver, bcount, depthBits);
+            tri_pass(A, I, p, o, hr, o, bF, bT, bodyHigh, ptOver, bcount, depthBits);
         }
         double* ob = A.O.body;
         ob[0 * n + p] = bF.x; ob[1 * n + p] = bF.y; ob[2 * n + p] = bF.z;
@@ -1774,7 +1775,7 @@ __global__ void __launch_bounds__(BLOCK) k_law_clump(LawArgs A)
                 cF = add(cF, mk(pf[0], pf[1], pf[2]));
                 cT = add(cT, mk(pf[3], pf[4], pf[5]));
             }
-            if (A.nb > 0) tri_pass(A, I, c, t, hr, i, bF, bT, bodyHigh, ptOver, bcount, depthBits);
+            if (A.nb > 0) tri_pass(A, I, c, c, hr, i, bF, bT, bodyHigh, ptOver, bcount, depthBits);
         }
         if (A.nb > 0) {
             double* ob = A.O.body;
@@ -1832,28 +1833,49 @@ __global__ void __launch_bounds__(BLOCK) k_regime(const int* regime, const int* 
     }
 }
 
-// Deterministic row sums: partial[row][block] = tree sum of rows[row][block
-// tile]; fixed order independent of scheduling.  With `keep`, element e of
-// every row counts as +0.0 where keep[e] is 0 (rows not stored there).
 constexpr int RED_TILE = 4096;
-__global__ void __launch_bounds__(BLOCK) k_reduce_rows(const double* rows, long n, int nrows,
-                                                       double* partial, int nblocks, const unsigned char* keep)
+
+// Deterministic sums of the body rows over the component input rows, in a
+// fixed tree independent of scheduling: tiles of RED_TILE rows; in a tile,
+// each thread sums rows tid, tid + BLOCK, ... in order, the wave folds its
+// partials with xor shuffles (32, 16, ..., 1), and the waves' results are
+// added in order.  One block per (tile, body) over all BODY_ROW quantities
+// of the body at once: a touched
+// element's values are one contiguous read; an untouched one adds nothing
+// (its values are zero, and a sum that starts at +0.0 never becomes -0.0, so
+// adding +0.0 would change no bit).  partial[row][tile] with rows in result
+// order: 6 nb loads, nb Fz, 22 nb microstate terms.
+__global__ void __launch_bounds__(BLOCK) k_reduce_body(const double* rows, const unsigned char* touched, long n,
+                                                       int nb, double* partial, int nblocks)
 {
-    const int row = blockIdx.y;
-    const double* r = rows + (long)row * n;
+    const int body = blockIdx.y;
     const long base = (long)blockIdx.x * RED_TILE;
-    double s = 0.0;
+    double acc[BODY_ROW];
+    for (int j = 0; j < BODY_ROW; ++j) acc[j] = 0.0;
+    // the thread's touched elements as bits (independent flag loads), then
+    // their values, lowest bit first: the order of the additions is kept
+    unsigned mask = 0;
+#pragma unroll
     for (int k = 0; k < RED_TILE / BLOCK; ++k) {
         const long e = base + (long)k * BLOCK + threadIdx.x;
-        if (e < n) s += (!keep || keep[e]) ? r[e] : 0.0;
+        if (e < n && touched[(long)body * n + e]) mask |= 1u << k;
     }
-    s = wave_sum(s);
-    __shared__ double ws[BLOCK / 64];
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    for (; mask; mask &= mask - 1u) {
+        const long e = base + (long)__builtin_ctz(mask) * BLOCK + threadIdx.x;
+        const double* v = rows + ((long)body * n + e) * BODY_ROW;
+        for (int j = 0; j < BODY_ROW; ++j) acc[j] += v[j];
+    }
+    __shared__ double ws[BODY_ROW][BLOCK / 64];
+    for (int j = 0; j < BODY_ROW; ++j) {
+        const double w = wave_sum(acc[j]);
+        if ((threadIdx.x & 63) == 0) ws[j][threadIdx.x >> 6] = w;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < BODY_ROW) {
+        const int j = threadIdx.x;
         double v = 0.0;
-        for (int w = 0; w < BLOCK / 64; ++w) v += ws[w];
+        for (int w = 0; w < BLOCK / 64; ++w) v += ws[j][w];
+        const int row = j < 6 ? body * 6 + j : (j == 6 ? 6 * nb + body : 7 * nb + body * 22 + (j - 7));
         partial[(long)row * nblocks + blockIdx.x] = v;
     }
 }
@@ -2414,7 +2436,6 @@ struct dem_solver {
     unsigned long long* triHitMask = nullptr;  // [nr] per listed row: entries with a positive area scale
     int* triHitN = nullptr;                    // [nr] hits stored, or -1 (evaluated in the lane)
     double2* triHitVal = nullptr;              // [nr][TRI_HITS] (area scale, plane distance)
-    unsigned char* triListed = nullptr;        // [nr] by sorted position: listed by k_tri_sift
     double* tLocal = nullptr;
     double* tWorld = nullptr;
     double* tAabb = nullptr;
@@ -2480,8 +2501,8 @@ struct dem_solver {
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
-                        H2.pwVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyPart,
-                        O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, triHitMask, triHitN, triHitVal, triListed, tLocal, tWorld, tAabb,
+                        H2.pwVal, H2.ptOcc, H2.ptSeen, H2.ptKey, H2.ptVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyRows, O.bodyTouched,
+                        O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, triHitMask, triHitN, triHitVal, tLocal, tWorld, tAabb,
                         pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
                         regimeInstant, regimeInterval, microInterval, bodyResult,
                         redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tStart, tCur, tScan, tBits};
@@ -2828,14 +2849,14 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     TRY(dalloc(s->pairs, (size_t)MAX_LIST * nr, "hipMalloc(compacted contacts)"));
     TRY(dalloc(s->pairF, (size_t)6 * MAX_LIST * nr, "hipMalloc(contact forces)"));
     {
-        // candidate lists (pure DEM): skin = DEM_SKIN (default 0.2) x the smallest
-        // scene's largest radius; the +-2-cell rebuild scan must cover r_i + r_j + skin
+        // candidate lists (single-sphere owners): skin = DEM_SKIN (default 0.2) x the
+        // smallest scene's largest radius; the +-2-cell rebuild scan must cover r_i + r_j + skin
         const char* e = getenv("DEM_SKIN");
         const double frac = e ? atof(e) : 0.2;
         double rmin = maxrScene[0];
         for (int k = 1; k < nsc; ++k) rmin = std::min(rmin, maxrScene[k]);
         s->skin = frac * rmin;
-        bool ok = frac > 0.0 && (!B || B->count == 0) && !clump;
+        bool ok = frac > 0.0 && !clump;
         for (int k = 0; k < nsc && ok; ++k) ok = 2.0 / s->hsp[k].inv >= 2.0 * maxrScene[k] + s->skin;
         if (ok) {
             TRY(dalloc(s->refC, (size_t)3 * n, "hipMalloc(candidate lists)"));
@@ -2908,6 +2929,12 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         HIP_TRY(hipMemset(s->H.ptOcc, 0, sizeof(unsigned long long) * nr), "clear history");
         HIP_TRY(hipMemset(s->H.ptSeen, 0, sizeof(unsigned long long) * 2 * nr), "clear history");
         HIP_TRY(hipMemset(s->H.ptKey, 0, sizeof(uint32_t) * PT_SLOTS * nr), "clear history");
+        if (s->verlet) {
+            TRY(dalloc(s->H2.ptOcc, nr, "hipMalloc(body history)"));
+            TRY(dalloc(s->H2.ptSeen, 2 * nr, "hipMalloc(body history)"));
+            TRY(dalloc(s->H2.ptKey, (size_t)PT_SLOTS * nr, "hipMalloc(body history keys)"));
+            TRY(dalloc(s->H2.ptVal, (size_t)PT_SLOTS * 9 * nr, "hipMalloc(body history values)"));
+        }
         TRY(dalloc(s->pose, (size_t)13 * nb, "hipMalloc(body poses)"));
         HIP_TRY(hipMemcpy(s->pose, B->pose13, sizeof(double) * 13 * nb, hipMemcpyHostToDevice), "upload body poses");
         std::vector<int> tb(s->nt);
@@ -2939,15 +2966,13 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         // loads, Fz and microstate sums side by side (one reduction over 29 nb rows)
         TRY(dalloc(s->bodyResult, (size_t)29 * nb, "hipMalloc(body resultant)"));
         s->microInstant = s->bodyResult + 7 * nb;
-        TRY(dalloc(s->O.bodyPart, (size_t)29 * nb * nr, "hipMalloc(body load partials)"));
-        s->O.fzPart = s->O.bodyPart + (size_t)6 * nb * nr;
-        s->O.micro = s->O.bodyPart + (size_t)7 * nb * nr;
+        TRY(dalloc(s->O.bodyRows, (size_t)BODY_ROW * nb * nr, "hipMalloc(body load partials)"));
+        TRY(dalloc(s->O.bodyTouched, (size_t)nb * nr, "hipMalloc(body load partials)"));
         TRY(dalloc(s->O.regime, (size_t)nb * nr, "hipMalloc(body regime state)"));
         TRY(dalloc(s->triWork, nr, "hipMalloc(triangle work list)"));
         TRY(dalloc(s->triHitMask, nr, "hipMalloc(triangle work list)"));
         TRY(dalloc(s->triHitN, nr, "hipMalloc(triangle work list)"));
         TRY(dalloc(s->triHitVal, (size_t)TRI_HITS * nr, "hipMalloc(triangle work list)"));
-        TRY(dalloc(s->triListed, nr, "hipMalloc(triangle work list)"));
         s->redBlocks = (int)((nr + RED_TILE - 1) / RED_TILE);
         TRY(dalloc(s->redPartial, (size_t)29 * nb * s->redBlocks, "hipMalloc(reduction partials)"));
         if (s->keepBodyParticle) TRY(dalloc(s->O.bodyParticle, (size_t)nb * n * 6, "hipMalloc(body-particle forces)"));
@@ -3146,8 +3171,9 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     Soa S = s->soa();   // owners
     Soa R = s->rows();  // contact rows (the owners, or their components with clumps)
     const long nr = R.n;
-    // candidate lists: only pure-DEM steps (bodies need the per-step cell order)
-    const bool verlet = s->verlet && nb == 0;
+    // candidate lists (single-sphere owners; body loads are reduced by input row,
+    // so coupled steps need no per-step sort either)
+    const bool verlet = s->verlet;
     const bool lists = verlet && s->vlistValid;
     if (s->clump) hipLaunchKernelGGL(k_expand, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, R, s->cOwner);
     hipLaunchKernelGGL(k_cells, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dSp, s->offs, s->sc,
@@ -3273,7 +3299,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         hipLaunchKernelGGL(k_law_clump, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     } else {
         if (nb > 0) {
-            hipLaunchKernelGGL(k_tri_sift, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A, s->triWork, s->triListed);
+            hipLaunchKernelGGL(k_tri_sift, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A, s->triWork);
             hipLaunchKernelGGL(k_tri_area, dim3(TRI_AREA_BLOCKS), dim3(BLOCK), 0, st, A, (const int*)s->triWork,
                                s->triHitMask, s->triHitN, s->triHitVal);
             hipLaunchKernelGGL(k_law_pt, dim3((unsigned)((n + TRI_BLOCK - 1) / TRI_BLOCK)), dim3(TRI_BLOCK), 0, st, A,
@@ -3291,10 +3317,10 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         const int* work = s->clump ? nullptr : s->triWork;
         hipLaunchKernelGGL(k_regime, dim3(REGIME_BLOCKS, (unsigned)nb), dim3(BLOCK), 0, st, s->O.regime, s->valA, nr, work,
                            (const Scalars*)s->sc, s->regimeInstant, s->regimeInterval, interval ? 1 : 0);
-        // body loads [6 nb], Fz [nb] and microstate [22 nb] rows, laid out side
-        // by side, in one reduction; the microstate sums also feed the interval
-        hipLaunchKernelGGL(k_reduce_rows, dim3(s->redBlocks, 29 * nb), dim3(BLOCK), 0, st, s->O.bodyPart, nr,
-                           29 * nb, s->redPartial, s->redBlocks, work ? (const unsigned char*)s->triListed : nullptr);
+        // body loads [6 nb], Fz [nb] and microstate [22 nb] sums in one
+        // reduction; the microstate sums also feed the interval
+        hipLaunchKernelGGL(k_reduce_body, dim3(s->redBlocks, nb), dim3(BLOCK), 0, st, (const double*)s->O.bodyRows,
+                           (const unsigned char*)s->O.bodyTouched, nr, nb, s->redPartial, s->redBlocks);
         hipLaunchKernelGGL(k_reduce_final, dim3((29 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 29 * nb,
                            s->redBlocks, s->bodyResult, interval ? s->microInterval : nullptr, 7 * nb);
         TRY(launch_check("body reductions"));

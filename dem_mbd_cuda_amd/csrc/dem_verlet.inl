@@ -189,5 +189,16 @@ __global__ void __launch_bounds__(BLOCK) k_hist_move(Soa S, History src, History
     const double* sw = src.pwVal + h * PW_SLOTS * 9;
     double* dw = dst.pwVal + t * PW_SLOTS * 9;
     for (int k = 0; k < wtop * 9; ++k) dw[k] = sw[k];
+    if (src.ptOcc) {  // triangle slots (runs with bodies)
+        const unsigned long long bocc = src.ptOcc[h];
+        dst.ptOcc[t] = bocc;
+        dst.ptSeen[t] = src.ptSeen[h];
+        dst.ptSeen[n + t] = src.ptSeen[n + h];
+        const int btop = bocc ? 64 - __clzll((long long)bocc) : 0;
+        for (int b = 0; b < btop; ++b) dst.ptKey[t * PT_SLOTS + b] = src.ptKey[h * PT_SLOTS + b];
+        const double* sb = src.ptVal + h * PT_SLOTS * 9;
+        double* db = dst.ptVal + t * PT_SLOTS * 9;
+        for (int k = 0; k < btop * 9; ++k) db[k] = sb[k];
+    }
     S.k(I_HIST, t) = (int)t;
 }

@@ -10,8 +10,8 @@
  * Deliberate choices (same in the HIP path):
  *   - Sums that the reference forms with device atomics (body loads, body
  *     contact counters, regime and microstate aggregates) are formed here per
- *     component and then over the components in cell order with a fixed tree
- *     (tree_sum); the reference's own order is nondeterministic.
+ *     component and then over the components in input order with a fixed
+ *     tree (tree_sum); the reference's own order is nondeterministic.
  *
  * Every function is pinned by a check independent of this restatement: the
  * reference's validation cases and known answers, its CPU binary at zero
@@ -165,7 +165,6 @@ struct ora_solver {
     int keepBodyParticle, keepBodyDetail;
     double *fcontact, *ffield, *fwall, *ftotal, *fbp, *fbd, *fbr;
     double *bodyRow, *fzRow; /* [nb][6][nc], [nb][nc]: per-component body load and Fz */
-    double* rowScratch;      /* [nc] rows in cell order for the tree sums */
     int pcount, bcount, pover, bover, phigh, bhigh;
     double bmaxdepth, bfz;
     int32_t* regimeState;
@@ -205,7 +204,7 @@ void ora_destroy(ora_solver* s)
                     s->gkey, s->gid, s->tkey, s->ttri, s->hpk, s->hwk, s->hbk,
                     s->hpl, s->hwl, s->hbl, s->hpv, s->hwv, s->hbv,
                     s->fcontact, s->ffield, s->fwall, s->ftotal, s->fbp, s->fbd, s->fbr,
-                    s->bodyRow, s->fzRow, s->rowScratch,
+                    s->bodyRow, s->fzRow,
                     s->regimeState, s->regimeInstant, s->regimeInterval, s->microState,
                     s->microInstant, s->microInterval, s->cachedRegimeInstant,
                     s->cachedRegimeInterval, s->cachedMicroInstant, s->cachedMicroInterval};
@@ -380,7 +379,6 @@ int ora_initialize(ora_solver* s, const dem_particles_view* P,
     ALLOC(s->fbr, 6 * s->nb);
     ALLOC(s->bodyRow, (size_t)6 * s->nb * nc);
     ALLOC(s->fzRow, (size_t)s->nb * nc);
-    ALLOC(s->rowScratch, nc);
     ALLOC(s->regimeState, (size_t)s->nb * nc);
     ALLOC(s->regimeInstant, NREG * s->nb); ALLOC(s->regimeInterval, NREG * s->nb);
     ALLOC(s->microState, (size_t)s->nb * nc * NMICRO);
@@ -1289,17 +1287,17 @@ static void reduce_body_rows(ora_solver* s, int interval)
     free(row);
 }
 
-/* Sum of one per-component row taken in cell order (the sorted grid order) with
- * a fixed tree: tiles of 4096 rows; in a tile, 256 partial sums over rows
- * tid, tid + 256, ...; each group of 64 partials folded pairwise with
- * partner lane ^ 32, ^ 16, ..., ^ 1; the 4 group results added in order; the
- * tile results added in order.  Any order is a faithful restatement of the
- * reference's atomic sums; this one is deterministic. */
+/* Sum of one per-component row taken in component input order (owner, then
+ * component id; independent of the grid) with a fixed tree: tiles of 4096
+ * rows; in a tile, 256 partial sums over rows tid, tid + 256, ...; each group
+ * of 64 partials folded pairwise with partner lane ^ 32, ^ 16, ..., ^ 1; the
+ * 4 group results added in order; the tile results added in order.  Any
+ * order is a faithful restatement of the reference's atomic sums; this one
+ * is deterministic. */
 static double tree_sum(ora_solver* s, const double* row)
 {
     const long n = s->nc;
-    double* v = s->rowScratch;
-    for (long t = 0; t < n; ++t) v[t] = row[s->gid[t]];
+    const double* v = row;
     double total = 0.0;
     for (long base = 0; base < n; base += 4096) {
         double part[256];

@@ -136,12 +136,17 @@ def test_single_contact_1000_steps(hip, oracle):
     compare_frames(g_, o, tol=1e-10)
 
 
-def test_mixer_with_all_three_body_states(hip, oracle):
+@pytest.mark.parametrize("skin", ["0.2", "0"])
+def test_mixer_with_all_three_body_states(hip, oracle, skin, monkeypatch):
     """A small mixer (3,000 spheres in a triangle-mesh bin, fixed; a blade in
     prescribed rotation; a dynamic plate pressed onto the bed), seeded
     velocities and spins, 60 coupled steps: particles, body kinematics,
-    loads, regime counters and microstate sums bit for bit."""
+    loads, regime counters and microstate sums bit for bit; with candidate
+    lists (their rebuilds move the triangle slots with the rows) and with
+    the full scan every step (DEM_SKIN=0)."""
     from dem_mbd_cuda_amd import scene
+
+    monkeypatch.setenv("DEM_SKIN", skin)
 
     sc = scene.mixer_scene(n=3000, side=0.05, seed=8, dt=1e-5, blade_clear=0.0, plate_gap=-0.3e-3)
     rng = np.random.RandomState(9)
