@@ -78,7 +78,7 @@ __device__ __forceinline__ void st4(double* a, int f, long n, long p, const doub
 // initialisation) and of the acceleration; rebuilds the contact records of
 // the step from the state before it.
 __global__ void __launch_bounds__(BLOCK) k_adj_prep(Soa S, const double* pre, const double* post, double dt,
-                                                    bool initial, Rec* rec, double* lam, double* z)
+                                                    bool initial, Recs rec, double* lam, double* z)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
@@ -106,9 +106,13 @@ __global__ void __launch_bounds__(BLOCK) k_adj_prep(Soa S, const double* pre, co
         const V3 ww = rot(q[0], q[1], q[2], q[3], w), wwh = rot(q[0], q[1], q[2], q[3], wh);
         r.wx = ww.x; r.wy = ww.y; r.wz = ww.z;
         r.ux = wwh.x; r.uy = wwh.y; r.uz = wwh.z;
-        r.mass = S.f(F_MASS, p); r.r = S.f(F_CR, p); r.status = status;
-        r.cid = (uint32_t)S.k(I_CID, p); r.scene = S.k(I_SCENE, p); r.pad0 = 0; r.pad1 = 0.0;
-        rec[p] = r;
+        r.mass = S.f(F_MASS, p);
+        rec.main[p] = r;
+        RecAux x;
+        x.r = S.f(F_CR, p);
+        x.status = status;
+        x.scene = S.k(I_SCENE, p);
+        rec.aux[p] = x;
     }
     st4(z, Z_Q, n, p, q);
     st3(z, Z_W, n, p, w);
@@ -236,7 +240,7 @@ __device__ __forceinline__ void scene_add(double* acc, int scene, bool active, c
 
 // Reverse of the particle-particle law over the step's contacts.
 __global__ void __launch_bounds__(BLOCK) k_adj_pairs(long count, const int2* pair, const double* prevTape,
-                                                     const unsigned long long* occTape, const Rec* rec,
+                                                     const unsigned long long* occTape, Recs rec,
                                                      const Material* mats, bool first, long n, double* z,
                                                      double* lamH, double* grads)
 {
@@ -251,7 +255,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pairs(long count, const int2* pai
         const long p = pr.x, q = pr.y >> SLOT_BITS;
         const int slot = pr.y & (PP_SLOTS - 1);
         const Body I = rec_body(rec, p), J = rec_body(rec, q);
-        scene = rec[p].scene;
+        scene = rec.aux[p].scene;
         Material m = mats[scene];
         if (first) m.dt = 0.0;
         double* hb = lamH + (p * PP_SLOTS + slot) * 9;
@@ -272,7 +276,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_pairs(long count, const int2* pai
 // Reverse of the particle-wall law over the step's wall contacts; the plate
 // load's adjoint enters as a seed on the plate wall's force.
 __global__ void __launch_bounds__(BLOCK) k_adj_walls(long count, const int4* wall, const double* prevTape,
-                                                     const unsigned* occTape, const Rec* rec, const Material* mats,
+                                                     const unsigned* occTape, Recs rec, const Material* mats,
                                                      const SceneParams* sp, Walls W, bool first, int plate,
                                                      const double* plateSeed, long n, double* z, double* lamHW,
                                                      double* grads)
@@ -286,7 +290,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_walls(long count, const int4* wal
         const long p = t.x;
         const int w = t.y, slot = t.z;
         const Body I = rec_body(rec, p);
-        scene = rec[p].scene;
+        scene = rec.aux[p].scene;
         Material m = mats[scene];
         if (first) m.dt = 0.0;
         const int gw = sp[scene].wall0 + w;
@@ -565,7 +569,7 @@ int diff_reorder(dem_solver* s)
 int diff_step(dem_solver* s, double dt)
 {
     const long n = s->n;
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, (Rec*)nullptr);
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, Recs{nullptr, nullptr});
     s->recFresh = false;
     TRY(dem_compute_forces(s, nullptr, nullptr, 0));
     hipLaunchKernelGGL(k_finish, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
@@ -733,16 +737,16 @@ int dem_diff_backward(dem_solver* s, const double* d_height, const double* d_pla
     auto reverse = [&](int k, int step, long long pairOff, long long wallOff, bool initial) {
         const double* pre = R.snaps + (size_t)k * NSNAP * n;
         const double* post = R.snaps + (size_t)(k + 1) * NSNAP * n;
-        hipLaunchKernelGGL(k_adj_prep, dim3(nb), dim3(BLOCK), 0, st, s->soa(), pre, post, dt, initial, s->rec,
+        hipLaunchKernelGGL(k_adj_prep, dim3(nb), dim3(BLOCK), 0, st, s->soa(), pre, post, dt, initial, s->recs(),
                            R.lam, R.z);
         const long long np = R.ppCount[step], nw = R.pwCount[step];
         if (np > 0)
             hipLaunchKernelGGL(k_adj_pairs, dim3((unsigned)((np + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, (long)np,
-                               R.pairT + pairOff, R.pairPrevT + pairOff * 9, R.occT + (size_t)k * n, s->rec,
+                               R.pairT + pairOff, R.pairPrevT + pairOff * 9, R.occT + (size_t)k * n, s->recs(),
                                s->dMatPP, initial, n, R.z, R.lamH, R.grads);
         if (nw > 0)
             hipLaunchKernelGGL(k_adj_walls, dim3((unsigned)((nw + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, (long)nw,
-                               R.wallT + wallOff, R.wallPrevT + wallOff * 9, R.pwOccT + (size_t)k * n, s->rec,
+                               R.wallT + wallOff, R.wallPrevT + wallOff * 9, R.pwOccT + (size_t)k * n, s->recs(),
                                s->dMatPW, s->dSp, W, initial, R.plate, plateSeed, n, R.z, R.lamHW, R.grads);
         if (!initial)
             hipLaunchKernelGGL(k_adj_body, dim3(nb), dim3(BLOCK), 0, st, s->soa(), pre, dt, R.lam, R.z);

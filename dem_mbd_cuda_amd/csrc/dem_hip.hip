@@ -705,23 +705,31 @@ __global__ void __launch_bounds__(BLOCK) k_cellstart(const unsigned long long* k
         start[c] = (unsigned)lower_bound_u64(key, n, c);
 }
 
-// Packed contact view of every particle, by physical index: the component
-// centre, endpoint and half-step velocities, world angular velocities, mass,
-// radius, status and component id (the fields particleContact reads for
-// both sides, contact:585-609), 160 B so a neighbour costs a few cache lines.
-struct alignas(32) Rec {
-    double px, py, pz, vx, vy, vz, hx, hy, hz, wx, wy, wz, ux, uy, uz, mass, r;
-    int status;
-    uint32_t cid;
-    int scene;
-    int pad0;
-    double pad1;
+// Packed contact view of every row, by physical index (the fields
+// particleContact reads for both sides, contact:585-609), in two parts: the
+// per-step kinematics (component centre, endpoint and half-step velocities,
+// world angular velocities) with the mass in one 128-byte line, and the
+// row's constants (radius, status, scene) in 16 bytes, so a neighbour costs
+// one line and a shared one.
+struct alignas(128) Rec {
+    double px, py, pz, vx, vy, vz, hx, hy, hz, wx, wy, wz, ux, uy, uz, mass;
 };
-static_assert(sizeof(Rec) == 160, "contact record layout");
+static_assert(sizeof(Rec) == 128, "contact record layout");
+struct alignas(16) RecAux {
+    double r;
+    int status;
+    int scene;
+};
+static_assert(sizeof(RecAux) == 16, "contact record layout");
+struct Recs {
+    Rec* main;
+    RecAux* aux;
+};
 
-__device__ __forceinline__ Body rec_body(const Rec* R, long p)
+__device__ __forceinline__ Body rec_body(const Recs& R, long p)
 {
-    const Rec& q = R[p];
+    const Rec& q = R.main[p];
+    const RecAux& x = R.aux[p];
     Body B;
     B.pos = mk(q.px, q.py, q.pz);
     B.vel = mk(q.vx, q.vy, q.vz);
@@ -729,18 +737,20 @@ __device__ __forceinline__ Body rec_body(const Rec* R, long p)
     B.w = mk(q.wx, q.wy, q.wz);
     B.wh = mk(q.ux, q.uy, q.uz);
     B.mass = q.mass;
-    B.r = q.r;
-    B.status = q.status;
-    B.cid = q.cid;
+    B.r = x.r;
+    B.status = x.status;
+    B.cid = 0u;  // not read by the laws (slots are found by the scans)
     B.off = zero3();
     return B;
 }
 
-// Contact record of a zero-offset row from its state values (what k_pack
-// writes for it; the advance kernels call it with the values they just
-// stored, saving k_pack's second pass over the state).
-__device__ __forceinline__ void put_rec(Rec* R, const Soa& S, long p, V3 pos, V3 vel, V3 vh, double qw, double qx,
-                                        double qy, double qz, V3 w, V3 wh)
+// Kinematic part of the contact record of a zero-offset row from its state
+// values (what k_pack writes for it; the advance kernels call it with the
+// values they just stored, saving k_pack's second pass over the state).  The
+// constant part stays as the last k_pack wrote it: rows only change place
+// in a reorder or a reload, and those clear the records' freshness.
+__device__ __forceinline__ void put_rec(const Recs& R, const Soa& S, long p, V3 pos, V3 vel, V3 vh, double qw,
+                                        double qx, double qy, double qz, V3 w, V3 wh)
 {
     Rec q;
     const V3 ww = rot(qw, qx, qy, qz, w), wwh = rot(qw, qx, qy, qz, wh);
@@ -749,12 +759,11 @@ __device__ __forceinline__ void put_rec(Rec* R, const Soa& S, long p, V3 pos, V3
     q.hx = vh.x; q.hy = vh.y; q.hz = vh.z;
     q.wx = ww.x; q.wy = ww.y; q.wz = ww.z;
     q.ux = wwh.x; q.uy = wwh.y; q.uz = wwh.z;
-    q.mass = S.f(F_MASS, p); q.r = S.f(F_CR, p); q.status = S.k(I_STATUS, p); q.cid = (uint32_t)S.k(I_CID, p);
-    q.scene = S.k(I_SCENE, p); q.pad0 = 0; q.pad1 = 0.0;
-    R[p] = q;
+    q.mass = S.f(F_MASS, p);
+    R.main[p] = q;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Rec* R)
+__global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Recs R)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (p >= S.n) return;
@@ -765,9 +774,13 @@ __global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Rec* R)
     q.hx = B.vh.x; q.hy = B.vh.y; q.hz = B.vh.z;
     q.wx = B.w.x; q.wy = B.w.y; q.wz = B.w.z;
     q.ux = B.wh.x; q.uy = B.wh.y; q.uz = B.wh.z;
-    q.mass = B.mass; q.r = B.r; q.status = B.status; q.cid = B.cid;
-    q.scene = S.k(I_SCENE, p); q.pad0 = 0; q.pad1 = 0.0;
-    R[p] = q;
+    q.mass = B.mass;
+    R.main[p] = q;
+    RecAux x;
+    x.r = B.r;
+    x.status = B.status;
+    x.scene = S.k(I_SCENE, p);
+    R.aux[p] = x;
 }
 
 // Contact centre, radius and component id in sorted order (what the
@@ -861,7 +874,7 @@ struct LawArgs {
     const int* rank;             // [rows] sorted position of each contact row (clumps)
     const int* ownerStart;       // [owners + 1] first component of each owner (clumps)
     const double4* posr;         // [rows] centre and radius in cell order
-    const Rec* rec;
+    Recs rec;
     const int* sorted;
     const int* ncount;
     const unsigned* off;
@@ -1131,7 +1144,7 @@ __global__ void __launch_bounds__(BLOCK) k_off_apply(const int* cnt, long n, con
 // particleContact (contact:591-706) for every listed contact, one thread per
 // contact (grid-stride over the compacted list), so uneven contact counts do
 // not idle lanes.  Writes the contact force on the owner and its torque term.
-__global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const Rec* rec,
+__global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs rec,
                                                  const int2* pairs, History H, const Material* mats,
                                                  bool first, bool offs, const Scalars* sc, double* pairF,
                                                  int2* tapePair, double* tapePrev)
@@ -1148,7 +1161,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, const
         const V3 delta = sub(J.pos, I.pos);
         const double d2 = dot(delta, delta);
         const long o = S.k(I_HIST, p);
-        Material m = mats[rec[p].scene];
+        Material m = mats[rec.aux[p].scene];
         if (first) m.dt = 0.0;  // the first evaluation advances no history
         double* hv = H.ppVal + (o * PP_SLOTS + slot) * 9;
         if (tapePair) {
@@ -1922,7 +1935,7 @@ __global__ void __launch_bounds__(BLOCK) k_init_half(Soa S, double h)
 
 // advancePositionKernel (particle:213-238); with R, also the row's contact
 // record for the force evaluation that follows.
-__global__ void __launch_bounds__(BLOCK) k_advance(Soa S, double dt, Rec* R)
+__global__ void __launch_bounds__(BLOCK) k_advance(Soa S, double dt, Recs R)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (p >= S.n) return;
@@ -1945,11 +1958,11 @@ __global__ void __launch_bounds__(BLOCK) k_advance(Soa S, double dt, Rec* R)
         S.f(F_VX, p) = vel.x; S.f(F_VY, p) = vel.y; S.f(F_VZ, p) = vel.z;
         S.f(F_WX, p) = w.x; S.f(F_WY, p) = w.y; S.f(F_WZ, p) = w.z;
     } else {
-        if (!R) return;
+        if (!R.main) return;
         vel = mk(S.f(F_VX, p), S.f(F_VY, p), S.f(F_VZ, p));
         w = mk(S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p));
     }
-    if (R) put_rec(R, S, p, pos, vel, vh, qw, qx, qy, qz, w, wh);
+    if (R.main) put_rec(R, S, p, pos, vel, vh, qw, qx, qy, qz, w, wh);
 }
 
 // advanceHalfVelocityKernel (particle:240-265)
@@ -1970,7 +1983,7 @@ __global__ void __launch_bounds__(BLOCK) k_finish(Soa S, double dt)
 // advancePositionKernel of the next one (particle:213-265), in one pass over
 // the state: the reference's loop calls them back to back with only host work
 // (the body solve) in between.
-__global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, double dt, Rec* R)
+__global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, double dt, Recs R)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (p >= S.n) return;
@@ -2003,11 +2016,11 @@ __global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, dou
         S.f(F_VX, p) = vel.x; S.f(F_VY, p) = vel.y; S.f(F_VZ, p) = vel.z;
         S.f(F_WX, p) = w.x; S.f(F_WY, p) = w.y; S.f(F_WZ, p) = w.z;
     } else {
-        if (!R) return;
+        if (!R.main) return;
         vel = mk(S.f(F_VX, p), S.f(F_VY, p), S.f(F_VZ, p));
         w = mk(S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p));
     }
-    if (R) put_rec(R, S, p, pos, vel, mk(vhx, vhy, vhz), qw, qx, qy, qz, w, mk(whx, why, whz));
+    if (R.main) put_rec(R, S, p, pos, vel, mk(vhx, vhy, vhz), qw, qx, qy, qz, w, mk(whx, why, whz));
 }
 
 // ------------------------------------------------------------ triangles
@@ -2414,7 +2427,9 @@ struct dem_solver {
     unsigned long long cellCap = 0;
     double4* posr = nullptr;
     uint32_t* cidS = nullptr;
-    Rec* rec = nullptr;
+    Rec* rec = nullptr;        // contact records, kinematic part
+    RecAux* recAux = nullptr;  // contact records, constant part
+    Recs recs() const { return Recs{rec, recAux}; }
     bool pendingFinish = false;
     bool recFresh = false;   // the contact records were written by the last advance (zero-offset rows)
     double pendingDt = 0.0;
@@ -2497,7 +2512,7 @@ struct dem_solver {
         diff_release(diff);
         graph_release(graphs);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
-                        rec, refC, cand, ncand, nkey, meta, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
+                        rec, recAux, refC, cand, ncand, nkey, meta, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
@@ -2830,6 +2845,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     TRY(dalloc(s->posr, nr, "hipMalloc(packed positions)"));
     TRY(dalloc(s->cidS, nr, "hipMalloc(sorted component ids)"));
     TRY(dalloc(s->rec, nr, "hipMalloc(contact records)"));
+    TRY(dalloc(s->recAux, nr, "hipMalloc(contact records)"));
     TRY(dalloc(s->boxPart, (size_t)6 * blocks_for(nr), "hipMalloc(grid box partials)"));
     TRY(dalloc(s->partScene, blocks_for(nr), "hipMalloc(grid box partials)"));
     TRY(dalloc(s->sceneAcc, (size_t)6 * nsc, "hipMalloc(scene boxes)"));
@@ -3227,7 +3243,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     }
     hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr, s->cidS,
                        verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank);
-    if (!s->recFresh) hipLaunchKernelGGL(k_pack, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->offs, s->rec);
+    if (!s->recFresh) hipLaunchKernelGGL(k_pack, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->offs, s->recs());
     TRY(launch_check("grid build"));
     bool useLists = verlet;
     if (verlet && rebuild) {
@@ -3273,12 +3289,12 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                        s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[2], st), "event");
-    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, s->valA, s->rec, s->pairs, s->H,
+    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, s->valA, s->recs(), s->pairs, s->H,
                        s->dMatPP, step == 0, s->offs, s->sc, s->pairF, s->tape.pair, s->tape.pairPrev);
     TRY(launch_check("particle contact law"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
-    A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.posr = s->posr; A.rec = s->rec; A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
+    A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.posr = s->posr; A.rec = s->recs(); A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
     A.pairF = s->pairF; A.H = s->H;
     A.W = Walls{s->nw, s->wallBuf, s->wallBuf + 3 * s->nw};
     A.T = triset(s); A.O = s->O; A.sc = s->sc;
@@ -3443,7 +3459,7 @@ int dem_advance_particles(dem_solver* s, double dt, double* vmax)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     // zero-offset single-sphere rows: the advance also writes the contact records
-    Rec* R = (!s->offs && !s->clump) ? s->rec : nullptr;
+    const Recs R = (!s->offs && !s->clump) ? s->recs() : Recs{nullptr, nullptr};
     if (s->pendingFinish) {
         s->pendingFinish = false;
         hipLaunchKernelGGL(k_finish_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(),
@@ -3451,7 +3467,7 @@ int dem_advance_particles(dem_solver* s, double dt, double* vmax)
     } else {
         hipLaunchKernelGGL(k_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, R);
     }
-    s->recFresh = R != nullptr;
+    s->recFresh = R.main != nullptr;
     if (vmax) *vmax = 0.0;
     return launch_check("advancePosition");
 }
