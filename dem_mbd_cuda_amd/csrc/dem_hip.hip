@@ -1231,13 +1231,15 @@ __device__ __forceinline__ void tri_range(const TriSet& T, int x, int y, int z, 
 }
 
 // Field force on owner row p: gravity, then the O(N^2) universal
-// gravitation over the owners in input-row order (contact:1083-1106).
+// gravitation over the owners in input-row order (contact:1083-1106), compiled
+// in only with UNIVERSAL (its loop costs the kernel registers otherwise).
+template <bool UNIVERSAL = true>
 __device__ __forceinline__ V3 field_force(const LawArgs& A, long p, double mass)
 {
     const Soa& S = A.S;
     V3 fF = zero3();
     if (A.gravity) fF = mk(mass * A.g[0], mass * A.g[1], mass * A.g[2]);
-    if (A.universal) {
+    if (UNIVERSAL && A.universal) {
         const V3 pi = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p));
         for (long oj = 0; oj < S.n; ++oj) {
             const long j = A.byorig[oj];
@@ -1585,7 +1587,9 @@ __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3
 }
 
 // computeContactsKernel (contact:1065-1221) for one single-sphere owner,
-// plus the acceleration (particle:71-101).
+// plus the acceleration (particle:71-101).  UNIVERSAL: runs with universal
+// gravitation (without it the kernel fits 3 waves per SIMD instead of 2).
+template <bool UNIVERSAL>
 __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
 {
     const long t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -1596,7 +1600,7 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
         const long p = A.sorted[t];
         const double mass = S.f(F_MASS, p);
         V3 cF = zero3(), cT = zero3(), wF = zero3(), wT = zero3(), bF = zero3(), bT = zero3();
-        const V3 fF = field_force(A, p, mass);
+        const V3 fF = field_force<UNIVERSAL>(A, p, mass);
         // walls: the depth test of wall_pass on the packed centre and radius;
         // the full contact record is read only by rows that touch a wall
         const int scene = S.k(I_SCENE, p);
@@ -3323,7 +3327,8 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                                (const int*)s->triHitN, (const double2*)s->triHitVal);
             TRY(launch_check("particle-triangle contact law"));
         }
-        hipLaunchKernelGGL(k_law, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+        if (s->universal) hipLaunchKernelGGL(k_law<true>, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+        else hipLaunchKernelGGL(k_law<false>, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     }
     TRY(launch_check("contact law"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[4], st), "event");

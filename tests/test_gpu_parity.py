@@ -206,3 +206,18 @@ def test_triangle_grid_box_beyond_the_cell_table(hip, oracle):
     o = run_frames(oracle, sc, steps=40, interval=10, debug=True)
     assert o[0]["stats"]["body_contacts"] > 0
     compare_frames(g, o)
+
+
+def test_universal_gravitation_single_spheres(hip, oracle):
+    """Universal gravitation on single-sphere owners (the O(N^2) field sum in
+    input-row order, compiled into the law kernel only for such runs) with
+    walls and contacts, 40 steps, bit for bit."""
+    from dem_mbd_cuda_amd import scene
+
+    sc = scene.box_scene(n=400, seed=17, box=(0.025, 0.025, 0.04), steps=40, dt=1e-6)
+    sc.universal_gravity = 1
+    sc.vel = np.random.RandomState(18).uniform(-2.0, 2.0, (sc.n, 3))
+    g = run_frames(hip, sc, interval=20, debug=True)
+    o = run_frames(oracle, sc, interval=20, debug=True)
+    assert o[40]["stats"]["particle_contacts"] > 0
+    compare_frames(g, o)
