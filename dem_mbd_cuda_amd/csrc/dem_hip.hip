@@ -3064,17 +3064,28 @@ TriSet triset(const dem_solver* s)
     return T;
 }
 
-int build_triangle_grid(dem_solver* s)
+// countTriangleCellsKernel for this step's triangles (box and total into the
+// scalars).
+int tri_grid_count(dem_solver* s)
 {
-    hipStream_t st = s->stream;
     s->tEntries = 0;
     s->tStartCap = 0;
+    hipLaunchKernelGGL(k_tri_count, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, s->stream, triset(s), s->inv,
+                       s->maxRadius, s->tCounts, s->sc);
+    return launch_check("countTriangleCells");
+}
+
+// The triangle grid of this step.  counted: tri_grid_count ran and the host
+// scalars already hold its box and total (read back with another value).
+int build_triangle_grid(dem_solver* s, bool counted = false)
+{
+    hipStream_t st = s->stream;
+    if (!counted) {
+        TRY(tri_grid_count(s));
+        HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
+        HIP_TRY(hipStreamSynchronize(st), "triangle grid counts");
+    }
     TriSet T = triset(s);
-    hipLaunchKernelGGL(k_tri_count, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
-                       s->tCounts, s->sc);
-    TRY(launch_check("countTriangleCells"));
-    HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
-    HIP_TRY(hipStreamSynchronize(st), "triangle grid counts");
     const unsigned long long total = s->hScalars->tri_total;
     if (s->hScalars->tri_overflow != 0 || total > (unsigned long long)INT_MAX)
         return set_error(DEM_E_CAPACITY, "triangle grid entry count exceeds the supported 32-bit index range");
@@ -3200,12 +3211,23 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                        s->boxPart, s->partScene, s->sceneAcc, lists ? s->refC : nullptr,
                        0.25 * s->skin * s->skin);
     bool rebuild = true;
+    bool triCounted = false;
     if (lists) {
         if (s->capturing) return set_error(DEM_E_INVALID, "candidate-list steps cannot be captured");
-        HIP_TRY(hipMemcpyAsync(s->hFlag, &s->sc->need_rebuild, sizeof(int), hipMemcpyDeviceToHost, st),
-                "read displacement flag");
-        HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
-        rebuild = s->hFlag[0] != 0;
+        if (s->nt > 0) {
+            // the triangle grid's box and total come back with the displacement flag
+            TRY(tri_grid_count(s));
+            HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st),
+                    "read displacement flag");
+            HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
+            rebuild = s->hScalars->need_rebuild != 0;
+            triCounted = true;
+        } else {
+            HIP_TRY(hipMemcpyAsync(s->hFlag, &s->sc->need_rebuild, sizeof(int), hipMemcpyDeviceToHost, st),
+                    "read displacement flag");
+            HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
+            rebuild = s->hFlag[0] != 0;
+        }
     }
     bool triDone = false;
     if (rebuild) {
@@ -3215,7 +3237,12 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         // also brings this step's key bits, so the sort launches only its passes
         int passes = 8;
         if (s->nt > 0) {
-            TRY(build_triangle_grid(s));
+            if (triCounted) {
+                HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st),
+                        "copy scalars");
+                HIP_TRY(hipStreamSynchronize(st), "sort key bits");
+            }
+            TRY(build_triangle_grid(s, triCounted));
             triDone = true;
             passes = (s->hScalars->nbits + 7) / 8;
         }
@@ -3269,7 +3296,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             s->vlistValid = true;
         }
     }
-    if (s->nt > 0 && !triDone) TRY(build_triangle_grid(s));
+    if (s->nt > 0 && !triDone) TRY(build_triangle_grid(s, triCounted));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
     const int step = s->stepIndex++;
     const int nblk = blocks_for(nr);
