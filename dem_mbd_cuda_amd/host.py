@@ -221,6 +221,7 @@ def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, 
     half_update(True)
     counter = 0
     cursor = 0
+    advanced = False  # the step's particle advance was launched at the end of the previous one
     for step in range(start_step, end):
         if prescribed:
             while cursor + 1 < len(prescribed) and prescribed[cursor + 1][0] <= step:
@@ -235,7 +236,9 @@ def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, 
             hdq[0] = gama(hq[0], hw[0])
         current = step * dt
         next_time = (step + 1) * dt
-        solver.advance(dt)
+        if not advanced:
+            solver.advance(dt)
+        advanced = False
         for b in range(nb):
             if B["state"][b] == BODY_FIXED:
                 continue
@@ -247,6 +250,11 @@ def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, 
         loads = solver.compute_forces(_pose13(B), loads=True, collect=due)
         impulses += loads * dt
         solver.finish(dt, half)
+        if not due and step + 1 < end:
+            # the particle finish and the next advance need no body state:
+            # launched now, they run on the device during the body solve below
+            solver.advance(dt)
+            advanced = True
         acc = dyn_eqn_bodyset(B, loads)
         half_update(False)
         counter += 1
