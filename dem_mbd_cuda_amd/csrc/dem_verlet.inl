@@ -83,33 +83,46 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
             const double4 me = posr[t];
             const int4 mine = meta[t];
             const int m = ncand[t];
-            for (int c = 0; c < m; ++c) {
-                const long k = cand[(long)c * n + t];
-                const double4 o = posr[k];
-                const double dx_ = o.x - me.x, dy_ = o.y - me.y, dz_ = o.z - me.z;
-                const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
-                const double rs = me.w + o.w;
-                if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
-                const int4 other = meta[k];
-                const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
-                if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
-                const uint32_t kk = ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) |
-                                    (uint32_t)other.w;
-                if (found >= MAX_LIST) {
+            // candidates in list order, their indices and centres fetched
+            // VSCAN_BATCH at a time so that the gathers overlap
+            constexpr int VSCAN_BATCH = 4;
+            for (int c0 = 0; c0 < m; c0 += VSCAN_BATCH) {
+                long kq[VSCAN_BATCH];
+                double4 oq[VSCAN_BATCH];
+#pragma unroll
+                for (int j = 0; j < VSCAN_BATCH; ++j) kq[j] = c0 + j < m ? cand[(long)(c0 + j) * n + t] : t;
+#pragma unroll
+                for (int j = 0; j < VSCAN_BATCH; ++j) oq[j] = posr[kq[j]];
+#pragma unroll
+                for (int j = 0; j < VSCAN_BATCH; ++j) {
+                    if (c0 + j >= m) break;
+                    const long k = kq[j];
+                    const double4 o = oq[j];
+                    const double dx_ = o.x - me.x, dy_ = o.y - me.y, dz_ = o.z - me.z;
+                    const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+                    const double rs = me.w + o.w;
+                    if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
+                    const int4 other = meta[k];
+                    const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
+                    if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
+                    const uint32_t kk = ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) |
+                                        (uint32_t)other.w;
+                    if (found >= MAX_LIST) {
+                        ++found;
+                        continue;
+                    }
+                    // insertion into the ordered list (nearly sorted: lists keep
+                    // their order from step to step)
+                    int pos = found;
+                    while (pos > 0 && nkey[(long)(pos - 1) * n + t] > kk) {
+                        nkey[(long)pos * n + t] = nkey[(long)(pos - 1) * n + t];
+                        nbr[(long)pos * n + t] = nbr[(long)(pos - 1) * n + t];
+                        --pos;
+                    }
+                    nkey[(long)pos * n + t] = kk;
+                    nbr[(long)pos * n + t] = (int)k;
                     ++found;
-                    continue;
                 }
-                // insertion into the ordered list (nearly sorted: lists keep
-                // their order from step to step)
-                int pos = found;
-                while (pos > 0 && nkey[(long)(pos - 1) * n + t] > kk) {
-                    nkey[(long)pos * n + t] = nkey[(long)(pos - 1) * n + t];
-                    nbr[(long)pos * n + t] = nbr[(long)(pos - 1) * n + t];
-                    --pos;
-                }
-                nkey[(long)pos * n + t] = kk;
-                nbr[(long)pos * n + t] = (int)k;
-                ++found;
             }
         }
         if (found > MAX_LIST) {
