@@ -537,6 +537,9 @@ __global__ void __launch_bounds__(BLOCK) k_rs_hist(const unsigned long long* key
 // Block-wide exclusive scan of one value per thread (wave prefix sums with
 // shuffles, then the four wave totals through LDS).  Returns the exclusive
 // prefix; *total receives the block sum.
+constexpr int SCAN_ITEMS = 8;  // values per thread in the one-block scans
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
+
 __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* total)
 {
     __shared__ unsigned wsum[BLOCK / 64];
@@ -1116,12 +1119,20 @@ __global__ void __launch_bounds__(BLOCK) k_off_blocks(const int* cnt, long n, un
 __global__ void __launch_bounds__(BLOCK) k_off_scan(unsigned* bsum, int nblocks, Scalars* sc)
 {
     unsigned carry = 0;
-    for (int b0 = 0; b0 < nblocks; b0 += BLOCK) {
-        const int b = b0 + threadIdx.x;
-        const unsigned v = b < nblocks ? bsum[b] : 0u;
+    for (long base = 0; base < nblocks; base += SCAN_TILE) {
+        const long b = base + (long)threadIdx.x * SCAN_ITEMS;
+        unsigned v[SCAN_ITEMS];
+        unsigned sum = 0;
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            v[j] = b + j < nblocks ? bsum[b + j] : 0u;
+            sum += v[j];
+        }
         unsigned tot;
-        const unsigned ex = block_exclusive_scan(v, &tot);
-        if (b < nblocks) bsum[b] = carry + ex;
+        unsigned run = carry + block_exclusive_scan(sum, &tot);
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            if (b + j < nblocks) bsum[b + j] = run;
+            run += v[j];
+        }
         carry += tot;
     }
     if (threadIdx.x == 0) sc->total_pairs = carry;
@@ -2219,28 +2230,6 @@ __global__ void __launch_bounds__(64) k_tri_cell_order_big(const unsigned* start
 // Exclusive scan of n unsigned values: tiles of SCAN_TILE per block (block
 // scan of per-thread sums), a one-block scan of the tile totals, then the
 // tile offsets added (and, with `copy`, the result duplicated there).
-constexpr int SCAN_ITEMS = 8;
-constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
-__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned& total)
-{
-    __shared__ unsigned wsum[BLOCK / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned inc = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const unsigned u = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += u;
-    }
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    unsigned before = 0;
-    total = 0;
-    for (int k = 0; k < BLOCK / 64; ++k) {
-        if (k < w) before += wsum[k];
-        total += wsum[k];
-    }
-    __syncthreads();
-    return before + inc - v;
-}
 __global__ void __launch_bounds__(BLOCK) k_scan_tiles(const unsigned* in, long n, unsigned* out, unsigned* tileSum)
 {
     const long base = (long)blockIdx.x * SCAN_TILE + (long)threadIdx.x * SCAN_ITEMS;
@@ -2251,7 +2240,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan_tiles(const unsigned* in, long n
         sum += v[j];
     }
     unsigned total;
-    unsigned run = block_exclusive_scan(sum, total);
+    unsigned run = block_exclusive_scan(sum, &total);
     for (int j = 0; j < SCAN_ITEMS; ++j) {
         if (base + j < n) out[base + j] = run;
         run += v[j];
@@ -2270,7 +2259,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan_top(unsigned* tileSum, long tile
             sum += v[j];
         }
         unsigned total;
-        unsigned run = carry + block_exclusive_scan(sum, total);
+        unsigned run = carry + block_exclusive_scan(sum, &total);
         for (int j = 0; j < SCAN_ITEMS; ++j) {
             if (b + j < tiles) tileSum[b + j] = run;
             run += v[j];
