@@ -561,6 +561,15 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* t
     return before + x - v;
 }
 
+// The block's sum of the rows' contact counts, for the list offsets
+// (bsum[b] covers rows b * BLOCK ...); every thread of the block calls it.
+__device__ __forceinline__ void block_count(int stored, unsigned* bsum, long b)
+{
+    unsigned tot;
+    block_exclusive_scan((unsigned)stored, &tot);
+    if (threadIdx.x == 0) bsum[b] = tot;
+}
+
 // Digit-major counts [256][nblocks]: block d turns row d into exclusive
 // offsets within the digit and records the digit total.
 __global__ void __launch_bounds__(BLOCK) k_rs_scan_rows(unsigned* counts, int nblocks, int shift,
@@ -1072,7 +1081,8 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
                                                 const uint32_t* cidS, const unsigned long long* key,
                                                 const unsigned* start, const SceneGrid* grid, Scalars* sc,
                                                 History H, int step, int* nbr, int* ncount,
-                                                unsigned long long* occTape, bool counters, const int* ownS)
+                                                unsigned long long* occTape, bool counters, const int* ownS,
+                                                unsigned* bsum)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
@@ -1095,6 +1105,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
             stored = assign_slots(S, t, p, found, sorted, cidS, H, step, nbr, occTape, high, over);
         ncount[t] = stored;
     }
+    block_count(stored, bsum, blockIdx.x);
     // per-step statistics only when they are read; overflow counts are
     // cumulative and (almost always) zero
     if (counters) {
@@ -1108,13 +1119,6 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
 // exclusive prefix sum of the list lengths in three passes (block sums, one
 // block scanning the sums, per-block offsets), the last also writing each
 // contact's (owner, entry) at its position.
-__global__ void __launch_bounds__(BLOCK) k_off_blocks(const int* cnt, long n, unsigned* bsum)
-{
-    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
-    unsigned tot;
-    block_exclusive_scan(t < n ? (unsigned)cnt[t] : 0u, &tot);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
-}
 
 __global__ void __launch_bounds__(BLOCK) k_off_scan(unsigned* bsum, int nblocks, Scalars* sc)
 {
@@ -3294,16 +3298,15 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (useLists)
         hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta, s->cidS,
                            s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->nkey, s->ncount, s->tape.ppOcc,
-                           counters);
+                           counters, s->listBlk);
     else if (s->clump)
         hipLaunchKernelGGL(k_scan<true>, dim3(nblk), dim3(BLOCK), 0, st, R, s->valA, s->posr, s->cidS, s->keyA,
                            s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
-                           counters, (const int*)s->ownS);
+                           counters, (const int*)s->ownS, s->listBlk);
     else
         hipLaunchKernelGGL(k_scan<false>, dim3(nblk), dim3(BLOCK), 0, st, R, s->valA, s->posr, s->cidS, s->keyA,
                            s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
-                           counters, (const int*)nullptr);
-    hipLaunchKernelGGL(k_off_blocks, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, nr, s->listBlk);
+                           counters, (const int*)nullptr, s->listBlk);
     hipLaunchKernelGGL(k_off_scan, dim3(1), dim3(BLOCK), 0, st, s->listBlk, nblk, s->sc);
     hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, nr, s->listBlk, s->nbr,
                        s->listOff, s->pairs);

@@ -73,9 +73,11 @@ __global__ void __launch_bounds__(BLOCK) k_vbuild(Soa S, const double4* posr, co
 __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const double4* posr, const int4* meta,
                                                  const uint32_t* cidS, const int* cand, const int* ncand,
                                                  Scalars* sc, History H, int step, int* nbr, uint32_t* nkey,
-                                                 int* ncount, unsigned long long* occTape, bool counters)
+                                                 int* ncount, unsigned long long* occTape, bool counters,
+                                                 unsigned* bsum)
 {
-    const long t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const long b = xcd_block(blockIdx.x, gridDim.x);
+    const long t = b * BLOCK + threadIdx.x;
     const long n = S.n;
     int found = 0, stored = 0, high = 0, over = 0;
     if (t < n) {
@@ -166,6 +168,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
         }
         ncount[t] = stored;
     }
+    block_count(stored, bsum, b);
     if (counters) {
         atomic_add_wave(&sc->particle_contacts, stored);
         atomic_max_wave(&sc->pp_high, high);
