@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_prep(Soa S, const double* pre, co
     st4(z, Z_Q, n, p, q);
     st3(z, Z_W, n, p, w);
     // post-step values
-    const V3 ap = ld3(post, P_A, n, p), alp = ld3(post, P_AL, n, p);
+    const V3 alp = ld3(post, P_AL, n, p);
     const V3 lvh = ld3(lam, L_VH, n, p), lwh = ld3(lam, L_WH, n, p), la = ld3(lam, L_A, n, p),
              lal = ld3(lam, L_AL, n, p);
     double lhq[4];

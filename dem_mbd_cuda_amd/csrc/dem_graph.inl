@@ -66,18 +66,6 @@ int capture_steps(dem_solver* s, double dt, int period, hipGraphExec_t* out)
     return DEM_OK;
 }
 
-int timed_steps(dem_solver* s, double dt, int steps, float* ms)
-{
-    TRY(eager_step(s, dt));  // settle the pending finish
-    HIP_TRY(hipEventRecord(s->ev[0], s->stream), "event");
-    for (int k = 0; k < steps; ++k) TRY(eager_step(s, dt));
-    TRY(flush_finish(s));
-    HIP_TRY(hipEventRecord(s->ev[5], s->stream), "event");
-    HIP_TRY(hipEventSynchronize(s->ev[5]), "event");
-    HIP_TRY(hipEventElapsedTime(ms, s->ev[0], s->ev[5]), "event");
-    return DEM_OK;
-}
-
 }  // namespace
 
 namespace {

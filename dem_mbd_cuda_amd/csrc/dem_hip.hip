@@ -2299,13 +2299,6 @@ __global__ void __launch_bounds__(BLOCK) k_gather_rows(Soa S, const double* src,
     for (int r = 0; r < nrows; ++r) out[(long)r * S.n + o] = src[(long)r * S.n + p];
 }
 
-__global__ void __launch_bounds__(BLOCK) k_gather_ints(Soa S, int field, int* out)
-{
-    const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (p >= S.n) return;
-    out[S.k(I_ORIG, p)] = S.k(field, p);
-}
-
 __global__ void __launch_bounds__(BLOCK) k_byorig(Soa S, int* byorig)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
