@@ -221,3 +221,30 @@ def test_universal_gravitation_single_spheres(hip, oracle):
     o = run_frames(oracle, sc, interval=20, debug=True)
     assert o[40]["stats"]["particle_contacts"] > 0
     compare_frames(g, o)
+
+
+def test_fine_mesh_rows_beyond_the_stored_hits(hip, oracle):
+    """Spheres pressed into a fine triangle floor (0.25 mm triangles): every
+    cell holds hundreds of triangle entries and a sphere touches about 20
+    triangles, more than the per-row hit store keeps, so the contacts take
+    the in-lane path; 30 coupled steps, bit for bit."""
+    from dem_mbd_cuda_amd import scene
+
+    rng = np.random.RandomState(21)
+    floor = scene.rectangle_mesh((-0.01, -0.01, 0.0), (0.02, 0, 0), (0, 0.02, 0), 80, 80)  # faces up
+    pos = np.column_stack([rng.uniform(-0.008, 0.008, 40), rng.uniform(-0.008, 0.008, 40),
+                           rng.uniform(0.85e-3, 0.95e-3, 40)])
+    P = scene.sphere_particles(pos, np.full(40, 1e-3), density=2500.0)
+    P["vel"] = rng.uniform(-0.2, 0.2, (40, 3))
+    m = scene.mech(epsS=0.5, mu=0.4, epsN=0.5, cohesion=0.0, beta=0.5, muT=0.1, muR=0.1, kN=2.0e4)
+    pose = np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0, 0]])
+    sc = scene.Scene(**P, pp=m, pt=dict(m), pw=dict(m), dt=1e-5, gravity=(0.0, 0.0, -9.81), gravity_enabled=1,
+                     steps=30, body_mass=np.array([1.0]), body_pose=pose,
+                     body_tri_count=np.array([len(floor)], np.int32), tri_local=floor,
+                     body_state=np.array([1], np.int32), body_inertia=np.eye(3)[None])
+    sc.keep_body_particle = 1
+    sc.keep_body_detail = 1
+    g = run_frames(hip, sc, steps=30, interval=10, debug=True)
+    o = run_frames(oracle, sc, steps=30, interval=10, debug=True)
+    assert o[0]["stats"]["body_contacts"] > 16 * 20
+    compare_frames(g, o)
