@@ -1690,50 +1690,64 @@ __global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work)
     wave_append(near, (int)t, &A.sc->tri_work, work);
 }
 
-// Triangle geometry of the rows listed by k_tri_sift, one wave per row and
-// one lane per entry of the row's cell (waves stride over the list): the
-// bounding-box test and area scale of EntryGeo for all of the row's
-// triangles at once.  Stores the row's hit mask (entries with a positive
-// area scale) and (scale, plane distance) per hit in entry order; a row with
-// more than 64 entries or TRI_HITS hits gets hitN = -1 and is evaluated in
-// the lane by k_law_pt.
+// Triangle geometry of the rows listed by k_tri_sift, 16 lanes per row (four
+// rows per wave; waves stride over the list), one lane per entry of the
+// row's cell in chunks of 16: the bounding-box test and area scale of
+// EntryGeo for all of the row's triangles at once.  Stores the row's hit
+// mask (entries with a positive area scale) and (scale, plane distance) per
+// hit in entry order; a row with more than 64 entries or TRI_HITS hits gets
+// hitN = -1 and is evaluated in the lane by k_law_pt.
 constexpr int TRI_HITS = 16;
 constexpr int TRI_AREA_BLOCKS = 2048;
+constexpr int TRI_GROUP = 16;
 __global__ void __launch_bounds__(BLOCK) k_tri_area(LawArgs A, const int* work, unsigned long long* hitMask,
                                                     int* hitN, double2* hitVal)
 {
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, g = lane / TRI_GROUP, l = lane % TRI_GROUP;
     const long count = A.sc->tri_work;
-    for (long i = (long)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); i < count;
-         i += (long)gridDim.x * (BLOCK / 64)) {
-        const long t = work[i];
-        const long p = A.sorted[t];
-        long b0, e0;
-        tri_range(A.T, A.S.k(I_CX, p), A.S.k(I_CY, p), A.S.k(I_CZ, p), b0, e0);
-        if (e0 - b0 > 64) {
-            if (lane == 0) hitN[i] = -1;
-            continue;
+    const long rowsPerPass = (long)gridDim.x * (BLOCK / 64) * (64 / TRI_GROUP);
+    for (long i0 = ((long)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * (64 / TRI_GROUP); i0 < count;
+         i0 += rowsPerPass) {
+        const long i = i0 + g;
+        const bool active = i < count;
+        long b0 = 0, e0 = 0;
+        double4 cr = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (active) {
+            const long t = work[i];
+            const long p = A.sorted[t];
+            tri_range(A.T, A.S.k(I_CX, p), A.S.k(I_CY, p), A.S.k(I_CZ, p), b0, e0);
+            cr = A.posr[t];
         }
-        const double4 cr = A.posr[t];
+        const long m = e0 - b0;
+        const int chunks = active && m <= 64 ? (int)((m + TRI_GROUP - 1) / TRI_GROUP) : 0;
         const V3 pos = mk(cr.x, cr.y, cr.z);
-        double scv = 0.0, distance = 0.0;
-        if (lane < e0 - b0) {
-            const int tr = A.T.ttri[b0 + lane];
-            const double* box = A.T.aabb + (long)tr * 6;
-            if (ball_meets_box(pos, cr.w, box)) {
-                const Tri tri = load_tri(A.T, tr);
-                scv = tri_area_scale(pos, cr.w, tri, box);
-                if (scv > 0.0) distance = dot(tri.n, sub(pos, tri.a));
+        unsigned long long rowHits = 0ull;
+        int nh = 0;
+        const int passes = wave_max(chunks);  // uniform over the wave: the ballots stay converged
+        for (int c = 0; c < passes; ++c) {
+            const long j = (long)c * TRI_GROUP + l;
+            double scv = 0.0, distance = 0.0;
+            if (c < chunks && j < m) {
+                const int tr = A.T.ttri[b0 + j];
+                const double* box = A.T.aabb + (long)tr * 6;
+                if (ball_meets_box(pos, cr.w, box)) {
+                    const Tri tri = load_tri(A.T, tr);
+                    scv = tri_area_scale(pos, cr.w, tri, box);
+                    if (scv > 0.0) distance = dot(tri.n, sub(pos, tri.a));
+                }
             }
+            const unsigned bits = (unsigned)((__ballot(scv > 0.0) >> (g * TRI_GROUP)) & 0xFFFFull);
+            if (scv > 0.0) {
+                const int rank = nh + __popc(bits & ((1u << l) - 1u));
+                if (rank < TRI_HITS) hitVal[i * TRI_HITS + rank] = make_double2(scv, distance);
+            }
+            rowHits |= (unsigned long long)bits << (c * TRI_GROUP);
+            nh += __popc(bits);
         }
-        const unsigned long long hits = __ballot(scv > 0.0);
-        const int nh = __popcll(hits);
-        if (lane == 0) {
-            hitMask[i] = hits;
-            hitN[i] = nh <= TRI_HITS ? nh : -1;
+        if (active && l == 0) {
+            hitMask[i] = rowHits;
+            hitN[i] = m <= 64 && nh <= TRI_HITS ? nh : -1;
         }
-        if (scv > 0.0 && nh <= TRI_HITS)
-            hitVal[i * TRI_HITS + __popcll(hits & ((1ull << lane) - 1ull))] = make_double2(scv, distance);
     }
 }
 
