@@ -242,7 +242,26 @@ __device__ __forceinline__ void atomic_max_wave_u64(unsigned long long* dst, uns
     if ((threadIdx.x & 63) == 0 && v) atomicMax(dst, v);
 }
 // Appends `item` for the lanes with `on` set to list[*count ...], one atomic
-// per wave; the order within the list is unspecified.
+// per wave; the order within the list is unspecified.  block_append does the
+// same with one atomic per block; every thread of the block calls it.
+__device__ __forceinline__ void block_append(bool on, int item, int* count, int* list)
+{
+    __shared__ int wcount[BLOCK / 64], wbase[BLOCK / 64 + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(on);
+    if (lane == 0) wcount[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) {
+            wbase[w] = total;
+            total += wcount[w];
+        }
+        wbase[BLOCK / 64] = total ? atomicAdd(count, total) : 0;
+    }
+    __syncthreads();
+    if (on) list[wbase[BLOCK / 64] + wbase[wave] + __popcll(m & ((1ull << lane) - 1ull))] = item;
+}
 __device__ __forceinline__ void wave_append(bool on, int item, int* count, int* list)
 {
     const unsigned long long m = __ballot(on);
@@ -1687,7 +1706,7 @@ __global__ void __launch_bounds__(BLOCK) k_tri_sift(LawArgs A, int* work)
             for (int k = 0; k < 6; ++k) A.O.body[(long)k * n + p] = 0.0;
         }
     }
-    wave_append(near, (int)t, &A.sc->tri_work, work);
+    block_append(near, (int)t, &A.sc->tri_work, work);
 }
 
 // Triangle geometry of the rows listed by k_tri_sift, 16 lanes per row (four
