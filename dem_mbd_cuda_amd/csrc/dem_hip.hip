@@ -1947,15 +1947,24 @@ __global__ void __launch_bounds__(BLOCK) k_reduce_body(const double* rows, const
 
 // out[row] = sum of the row's partials in block order; rows from accFrom on
 // are also added to acc[row - accFrom] when acc is given.
-__global__ void k_reduce_final(const double* partial, int nrows, int nblocks, double* out, double* acc,
-                               int accFrom)
+// One wave per row: the lanes load 64 partials at a time, lane 0 adds them in
+// block order from the lanes' registers.
+__global__ void __launch_bounds__(64) k_reduce_final(const double* partial, int nrows, int nblocks, double* out,
+                                                     double* acc, int accFrom)
 {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= nrows) return;
+    const int row = blockIdx.x;
+    if (row >= nrows) return;  // whole block
+    const int lane = threadIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += partial[(long)row * nblocks + b];
-    out[row] = s;
-    if (acc && row >= accFrom) acc[row - accFrom] += s;
+    for (int b0 = 0; b0 < nblocks; b0 += 64) {
+        const double v = b0 + lane < nblocks ? partial[(long)row * nblocks + b0 + lane] : 0.0;
+        const int k = nblocks - b0 < 64 ? nblocks - b0 : 64;
+        for (int j = 0; j < k; ++j) s += __shfl(v, j, 64);
+    }
+    if (lane == 0) {
+        out[row] = s;
+        if (acc && row >= accFrom) acc[row - accFrom] += s;
+    }
 }
 
 // ------------------------------------------------------------ leapfrog
@@ -2243,23 +2252,37 @@ __global__ void __launch_bounds__(BLOCK) k_tri_cell_order(const unsigned* start,
 }
 
 // One 64-lane block per listed cell (blocks stride over the list): each
-// entry goes to its rank among the cell's (distinct) triangle indices,
-// through `tmp`, then back.
+// entry goes to its rank among the cell's (distinct) triangle indices, the
+// cell staged in LDS (through `tmp` in global memory for a cell beyond the
+// LDS stage).
 __global__ void __launch_bounds__(64) k_tri_cell_order_big(const unsigned* start, const int* bigCount,
                                                            const int* big, int* tri, int* tmp)
 {
+    constexpr unsigned kLds = 4096;  // cells up to this size are ranked from LDS
+    __shared__ int sh[kLds];
     const int count = *bigCount;
     for (int k = blockIdx.x; k < count; k += gridDim.x) {
         const long long c = big[k];
         const unsigned b = start[c], m = start[c + 1] - b;
-        for (unsigned i = threadIdx.x; i < m; i += 64) {
-            const int v = tri[b + i];
-            unsigned rank = 0;
-            for (unsigned j = 0; j < m; ++j) rank += tri[b + j] < v;
-            tmp[b + rank] = v;
+        if (m <= kLds) {
+            for (unsigned i = threadIdx.x; i < m; i += 64) sh[i] = tri[b + i];
+            __syncthreads();
+            for (unsigned i = threadIdx.x; i < m; i += 64) {
+                const int v = sh[i];
+                unsigned rank = 0;
+                for (unsigned j = 0; j < m; ++j) rank += sh[j] < v;
+                tri[b + rank] = v;
+            }
+        } else {
+            for (unsigned i = threadIdx.x; i < m; i += 64) {
+                const int v = tri[b + i];
+                unsigned rank = 0;
+                for (unsigned j = 0; j < m; ++j) rank += tri[b + j] < v;
+                tmp[b + rank] = v;
+            }
+            __syncthreads();
+            for (unsigned i = threadIdx.x; i < m; i += 64) tri[b + i] = tmp[b + i];
         }
-        __syncthreads();
-        for (unsigned i = threadIdx.x; i < m; i += 64) tri[b + i] = tmp[b + i];
         __syncthreads();
     }
 }
@@ -3387,7 +3410,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         // reduction; the microstate sums also feed the interval
         hipLaunchKernelGGL(k_reduce_body, dim3(s->redBlocks, nb), dim3(BLOCK), 0, st, (const double*)s->O.bodyRows,
                            (const unsigned char*)s->O.bodyTouched, nr, nb, s->redPartial, s->redBlocks);
-        hipLaunchKernelGGL(k_reduce_final, dim3((29 * nb + 255) / 256), dim3(256), 0, st, s->redPartial, 29 * nb,
+        hipLaunchKernelGGL(k_reduce_final, dim3(29 * nb), dim3(64), 0, st, s->redPartial, 29 * nb,
                            s->redBlocks, s->bodyResult, interval ? s->microInterval : nullptr, 7 * nb);
         TRY(launch_check("body reductions"));
         ++s->intervalSteps;
