@@ -910,6 +910,7 @@ struct LawArgs {
     const int* ncount;
     const unsigned* off;
     const double* pairF;
+    const double* rowF;          // [rows][6] contact sums of rows summed in k_pairs (single spheres)
     History H;
     Walls W;
     TriSet T;
@@ -1175,16 +1176,43 @@ __global__ void __launch_bounds__(BLOCK) k_off_apply(const int* cnt, long n, con
     }
 }
 
+// Orders a wave's LDS accesses across its lanes (no block barrier).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A row's contacts are summed inside k_pairs when its list [off, off + cnt)
+// lies in one aligned 64-entry chunk, i.e. in one wave's iteration.
+__device__ __forceinline__ bool row_in_chunk(long base, int cnt)
+{
+    return cnt > 0 && (base >> 6) == ((base + cnt - 1) >> 6);
+}
+
 // particleContact (contact:591-706) for every listed contact, one thread per
 // contact (grid-stride over the compacted list), so uneven contact counts do
-// not idle lanes.  Writes the contact force on the owner and its torque term.
+// not idle lanes.  With rowF, a row whose list lies in one wave's chunk is
+// summed here by its first contact's lane, from +0.0 in list order (the order
+// and start of the owner's own sum), into rowF[row]; the other rows' contacts
+// write their force and torque term to pairF for the owner to add.
 __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs rec,
-                                                 const int2* pairs, History H, const Material* mats,
+                                                 const int2* pairs, const unsigned* off, const int* ncount,
+                                                 History H, const Material* mats,
                                                  bool first, bool offs, const Scalars* sc, double* pairF,
-                                                 int2* tapePair, double* tapePrev)
+                                                 double* rowF, int2* tapePair, double* tapePrev)
 {
+    __shared__ double stage[BLOCK * 6];
     const long total = sc->total_pairs;
-    for (long e = (long)blockIdx.x * BLOCK + threadIdx.x; e < total; e += (long)gridDim.x * BLOCK) {
+    const int lane = threadIdx.x & 63;
+    double* ws = stage + (threadIdx.x - lane) * 6;  // this wave's 64 results
+    for (long e0 = (long)blockIdx.x * BLOCK; e0 < total; e0 += (long)gridDim.x * BLOCK) {
+        const long e = e0 + threadIdx.x;
+        V3 cf = zero3(), tq = zero3();
+        long row = -1;
+        int len = 0;
+        if (e < total) {
         const int2 pr = pairs[e];
         const long p = sorted[pr.x];
         const long q = pr.y >> SLOT_BITS;
@@ -1202,11 +1230,41 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs 
             tapePair[e] = make_int2((int)p, pr.y);
             for (int k = 0; k < 9; ++k) tapePrev[e * 9 + k] = hv[k];
         }
-        V3 cf, tq;
         pp_law(I, J, delta, d2, m, hv, cf, tq);
-        double* out = pairF + e * 6;
-        out[0] = cf.x; out[1] = cf.y; out[2] = cf.z;
-        out[3] = tq.x; out[4] = tq.y; out[5] = tq.z;
+        bool whole = false;
+        if (rowF) {
+            const long base = off[pr.x];
+            const int cnt = ncount[pr.x];
+            whole = row_in_chunk(base, cnt);
+            if (whole && e == base) {
+                row = pr.x;
+                len = cnt;
+            }
+        }
+        if (!whole) {
+            double* out = pairF + e * 6;
+            out[0] = cf.x; out[1] = cf.y; out[2] = cf.z;
+            out[3] = tq.x; out[4] = tq.y; out[5] = tq.z;
+        }
+        }
+        if (rowF) {  // block-uniform
+            wave_sync();
+            double* mine = ws + lane * 6;
+            mine[0] = cf.x; mine[1] = cf.y; mine[2] = cf.z;
+            mine[3] = tq.x; mine[4] = tq.y; mine[5] = tq.z;
+            wave_sync();
+            if (row >= 0) {
+                double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0, f5 = 0.0;
+                for (int j = 0; j < len; ++j) {
+                    const double* v = ws + (lane + j) * 6;
+                    f0 = f0 + v[0]; f1 = f1 + v[1]; f2 = f2 + v[2];
+                    f3 = f3 + v[3]; f4 = f4 + v[4]; f5 = f5 + v[5];
+                }
+                double* out = rowF + row * 6;
+                out[0] = f0; out[1] = f1; out[2] = f2;
+                out[3] = f3; out[4] = f4; out[5] = f5;
+            }
+        }
     }
 }
 
@@ -1663,10 +1721,16 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
         // particle pairs, summed in candidate order from the per-contact results
         const int cnt = A.ncount[t];
         const long base = A.off[t];
-        for (int c = 0; c < cnt; ++c) {
-            const double* pf = A.pairF + (base + c) * 6;
-            cF = add(cF, mk(pf[0], pf[1], pf[2]));
-            cT = add(cT, mk(pf[3], pf[4], pf[5]));
+        if (A.rowF && row_in_chunk(base, cnt)) {
+            const double* rf = A.rowF + t * 6;
+            cF = mk(rf[0], rf[1], rf[2]);
+            cT = mk(rf[3], rf[4], rf[5]);
+        } else {
+            for (int c = 0; c < cnt; ++c) {
+                const double* pf = A.pairF + (base + c) * 6;
+                cF = add(cF, mk(pf[0], pf[1], pf[2]));
+                cT = add(cT, mk(pf[3], pf[4], pf[5]));
+            }
         }
         // triangle contacts were evaluated by k_law_pt (separate accumulator)
         if (A.nb > 0) {
@@ -2485,6 +2549,7 @@ struct dem_solver {
     unsigned* listBlk = nullptr;  // [blocks] per-block offsets
     int2* pairs = nullptr;   // compacted (owner, entry) [MAX_LIST * n]
     double* pairF = nullptr; // per-contact force and torque term [MAX_LIST * n][6]
+    double* rowF = nullptr;  // [n][6] contact sums of rows summed in k_pairs (single-sphere runs)
     int* byorig = nullptr;
     Tape tape{};  // set by a differentiable run while it records a step
     DiffRun* diff = nullptr;
@@ -2559,7 +2624,7 @@ struct dem_solver {
         graph_release(graphs);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
                         rec, recAux, refC, cand, ncand, nkey, meta, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
-                        dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
+                        dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, rowF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
                         H2.pwVal, H2.ptOcc, H2.ptSeen, H2.ptKey, H2.ptVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyRows, O.bodyTouched,
@@ -2910,6 +2975,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     TRY(dalloc(s->listBlk, blocks_for(nr), "hipMalloc(contact list offsets)"));
     TRY(dalloc(s->pairs, (size_t)MAX_LIST * nr, "hipMalloc(compacted contacts)"));
     TRY(dalloc(s->pairF, (size_t)6 * MAX_LIST * nr, "hipMalloc(contact forces)"));
+    if (!clump) TRY(dalloc(s->rowF, (size_t)6 * nr, "hipMalloc(contact sums)"));
     {
         // candidate lists (single-sphere owners): skin = DEM_SKIN (default 0.2) x the
         // smallest scene's largest radius; the +-2-cell rebuild scan must cover r_i + r_j + skin
@@ -3361,13 +3427,14 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                        s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[2], st), "event");
-    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, s->valA, s->recs(), s->pairs, s->H,
-                       s->dMatPP, step == 0, s->offs, s->sc, s->pairF, s->tape.pair, s->tape.pairPrev);
+    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, s->valA, s->recs(), s->pairs,
+                       (const unsigned*)s->listOff, (const int*)s->ncount, s->H, s->dMatPP, step == 0, s->offs,
+                       s->sc, s->pairF, s->rowF, s->tape.pair, s->tape.pairPrev);
     TRY(launch_check("particle contact law"));
     if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
     A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.posr = s->posr; A.rec = s->recs(); A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
-    A.pairF = s->pairF; A.H = s->H;
+    A.pairF = s->pairF; A.rowF = s->rowF; A.H = s->H;
     A.W = Walls{s->nw, s->wallBuf, s->wallBuf + 3 * s->nw};
     A.T = triset(s); A.O = s->O; A.sc = s->sc;
     A.pt = s->pt; A.matPW = s->dMatPW; A.sp = s->dSp; A.first = step == 0;
