@@ -156,6 +156,7 @@ def measure_hopper(dm, scene, args, packing, steps, warmup, timing_steps):
     s.synchronize()
     elapsed = time.perf_counter() - t1
     tms = []
+    s.enable_timings(True)
     for _ in range(timing_steps):
         s.advance(dt)
         s.compute_forces(collect=False)

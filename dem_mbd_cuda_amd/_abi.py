@@ -153,6 +153,8 @@ class Library:
             bind("diff_backward", [vp, _dp, _dp, _dp])
         if hasattr(L, p + "debug_last_timings"):
             bind("debug_last_timings", [vp, _dp, C.c_int, C.POINTER(C.c_int)])
+        if hasattr(L, p + "debug_enable_timings"):
+            bind("debug_enable_timings", [vp, C.c_int])
 
     def call(self, name, *args):
         rc = self._fn[name](*args)
@@ -388,6 +390,9 @@ class Solver:
         self.lib.call("diff_backward", self.h, _ptr(dh, C.c_double), _ptr(dl, C.c_double),
                       _ptr(out, C.c_double))
         return out
+
+    def enable_timings(self, on=True):
+        self.lib.call("debug_enable_timings", self.h, 1 if on else 0)
 
     def last_timings(self):
         buf = np.zeros(16)

@@ -820,8 +820,9 @@ __global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Recs R)
 // the t-th sorted row (the scan skips partners of the same owner).
 __global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool offs, double4* posr,
                                                 uint32_t* cidS, int4* meta, const int* owner, int* ownS,
-                                                int* rank)
+                                                int* rank, const int* skip)
 {
+    if (skip && *skip) return;  // launched ahead of a decision that came out as a rebuild
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= S.n) return;
     const long p = sorted[t];
@@ -2607,7 +2608,9 @@ struct dem_solver {
     bool keepBodyParticle = false, keepBodyDetail = false;
     // timing
     hipEvent_t ev[NEV] = {};
+    hipEvent_t flagEv = nullptr;  // the displacement flag has reached host memory
     bool evReady = false;
+    bool timing = false;  // record the stage events of dem_debug_last_timings
     // metadata for checkpoints
     std::vector<int> compIds, ownerIds;  // per component: id, owner id
     std::vector<int> pids;               // per owner: id
@@ -2642,6 +2645,7 @@ struct dem_solver {
         if (hTriOff) hipHostFree(hTriOff);
         for (auto& e : ev)
             if (e) hipEventDestroy(e);
+        if (flagEv) hipEventDestroy(flagEv);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -2832,6 +2836,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipMemGetInfo(&s->freeBefore, &total), "query HIP memory before allocation");
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+    HIP_TRY(hipEventCreateWithFlags(&s->flagEv, hipEventDisableTiming), "hipEventCreate");
 
     s->n = n;
     s->nc = nc;
@@ -3305,7 +3310,9 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             TRY(launch_check("transform triangles"));
         }
     }
-    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[0], st), "event");
+    // stage events only when asked for: each adds a gap between kernels
+    const bool timed = s->timing && !s->capturing;
+    if (timed) HIP_TRY(hipEventRecord(s->ev[0], st), "event");
     hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
     Soa S = s->soa();   // owners
     Soa R = s->rows();  // contact rows (the owners, or their components with clumps)
@@ -3320,6 +3327,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                        0.25 * s->skin * s->skin);
     bool rebuild = true;
     bool triCounted = false;
+    bool scanned = false;  // k_posr and the list scan already queued
     if (lists) {
         if (s->capturing) return set_error(DEM_E_INVALID, "candidate-list steps cannot be captured");
         if (s->nt > 0) {
@@ -3330,6 +3338,25 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
             rebuild = s->hScalars->need_rebuild != 0;
             triCounted = true;
+        } else if (s->recFresh) {
+            // the step's scan is queued before the flag is read, each kernel
+            // doing nothing if the flag asks for a rebuild, so the device does
+            // not wait for the host's decision; the scalars (with the range
+            // flag of k_cells) come back with the flag
+            HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st),
+                    "read displacement flag");
+            HIP_TRY(hipEventRecord(s->flagEv, st), "event");
+            const int* skip = &s->sc->need_rebuild;
+            hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr,
+                               s->cidS, s->meta, s->cOwner, s->ownS, s->cRank, skip);
+            if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
+            hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta,
+                               s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr, s->nkey, s->ncount,
+                               s->tape.ppOcc, collect || s->countAlways, s->listBlk, skip);
+            TRY(launch_check("contact scan"));
+            HIP_TRY(hipEventSynchronize(s->flagEv), "read displacement flag");
+            rebuild = s->hScalars->need_rebuild != 0;
+            scanned = !rebuild;
         } else {
             HIP_TRY(hipMemcpyAsync(s->hFlag, &s->sc->need_rebuild, sizeof(int), hipMemcpyDeviceToHost, st),
                     "read displacement flag");
@@ -3380,8 +3407,9 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, nr, s->sc, s->cellStart,
                            s->cellCap);
     }
-    hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr, s->cidS,
-                       verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank);
+    if (!scanned)
+        hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr, s->cidS,
+                           verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank, (const int*)nullptr);
     if (!s->recFresh) hipLaunchKernelGGL(k_pack, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->offs, s->recs());
     TRY(launch_check("grid build"));
     bool useLists = verlet;
@@ -3405,15 +3433,17 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         }
     }
     if (s->nt > 0 && !triDone) TRY(build_triangle_grid(s, triCounted));
-    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
+    if (timed && !scanned) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
     const int step = s->stepIndex++;
     const int nblk = blocks_for(nr);
     // the reference reads its per-step counters only on diagnostics steps
     const bool counters = collect || s->countAlways;
-    if (useLists)
+    if (scanned)
+        ;
+    else if (useLists)
         hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta, s->cidS,
                            s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->nkey, s->ncount, s->tape.ppOcc,
-                           counters, s->listBlk);
+                           counters, s->listBlk, (const int*)nullptr);
     else if (s->clump)
         hipLaunchKernelGGL(k_scan<true>, dim3(nblk), dim3(BLOCK), 0, st, R, s->valA, s->posr, s->cidS, s->keyA,
                            s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
@@ -3426,12 +3456,12 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     hipLaunchKernelGGL(k_off_apply, dim3(nblk), dim3(BLOCK), 0, st, s->ncount, nr, s->listBlk, s->nbr,
                        s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
-    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[2], st), "event");
+    if (timed) HIP_TRY(hipEventRecord(s->ev[2], st), "event");
     hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, s->valA, s->recs(), s->pairs,
                        (const unsigned*)s->listOff, (const int*)s->ncount, s->H, s->dMatPP, step == 0, s->offs,
                        s->sc, s->pairF, s->rowF, s->tape.pair, s->tape.pairPrev);
     TRY(launch_check("particle contact law"));
-    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
+    if (timed) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
     A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.posr = s->posr; A.rec = s->recs(); A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
     A.pairF = s->pairF; A.rowF = s->rowF; A.H = s->H;
@@ -3466,7 +3496,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         else hipLaunchKernelGGL(k_law<false>, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
     }
     TRY(launch_check("contact law"));
-    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[4], st), "event");
+    if (timed) HIP_TRY(hipEventRecord(s->ev[4], st), "event");
     const bool interval = A.pt.dt > 0.0;
     if (nb > 0) {
         // single-sphere owners: only the rows k_tri_sift listed; clumps: every row
@@ -3483,9 +3513,12 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         ++s->intervalSteps;
         if (!interval) --s->intervalSteps;
     }
-    if (!s->capturing) HIP_TRY(hipEventRecord(s->ev[5], st), "event");
-    s->evReady = true;
-    HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
+    if (timed) HIP_TRY(hipEventRecord(s->ev[5], st), "event");
+    s->evReady = timed;
+    // a list step without rebuild has brought the scalars back with its
+    // displacement flag (the only later writer of the range flag is the rebuild)
+    if (!scanned || collect)
+        HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st), "copy scalars");
     const bool needLoads = loads6 && nb > 0;
     if (needLoads)
         HIP_TRY(hipMemcpyAsync(s->hLoads, s->bodyResult, sizeof(double) * 7 * nb, hipMemcpyDeviceToHost, st),
@@ -3838,6 +3871,14 @@ int dem_synchronize(const dem_solver* s)
     if (!s || !s->stream) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     TRY(flush_finish(s));
     HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    return DEM_OK;
+}
+
+int dem_debug_enable_timings(dem_solver* s, int enabled)
+{
+    if (!s) return set_error(DEM_E_INVALID, "null argument");
+    s->timing = enabled != 0;
+    if (!s->timing) s->evReady = false;
     return DEM_OK;
 }
 

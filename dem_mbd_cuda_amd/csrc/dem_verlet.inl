@@ -74,8 +74,9 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
                                                  const uint32_t* cidS, const int* cand, const int* ncand,
                                                  Scalars* sc, History H, int step, int* nbr, uint32_t* nkey,
                                                  int* ncount, unsigned long long* occTape, bool counters,
-                                                 unsigned* bsum)
+                                                 unsigned* bsum, const int* skip)
 {
+    if (skip && *skip) return;  // launched ahead of a decision that came out as a rebuild
     const long b = xcd_block(blockIdx.x, gridDim.x);
     const long t = b * BLOCK + threadIdx.x;
     const long n = S.n;

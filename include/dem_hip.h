@@ -246,7 +246,12 @@ int dem_debug_triangle_grid(const dem_solver* solver, uint64_t* keys,
  * reference counterpart: the reference's calls are synchronous, these are
  * ordered on one stream and complete at the next download). */
 int dem_synchronize(const dem_solver* solver);
-/* Device kernel timing of the last dem_compute_forces, in milliseconds:
+/* Turns on (enabled != 0) or off the stage events behind
+ * dem_debug_last_timings; off by default, since each event adds a short gap
+ * between kernels (no reference counterpart). */
+int dem_debug_enable_timings(dem_solver* solver, int enabled);
+/* Device kernel timing of the last dem_compute_forces made while timings
+ * were enabled (count 0 otherwise), in milliseconds:
  * [0] grid build, [1] contact scan + list compaction, [2] per-contact
  * particle law, [3] per-particle sums (triangles, walls, field,
  * acceleration), [4] body reductions.  For measurement only. */
