@@ -348,10 +348,12 @@ __global__ void __launch_bounds__(BLOCK) k_expand(Soa S, Soa R, const int* owner
 // the particle's scene.  The cell bounding box of each scene is reduced per
 // block (wave shuffles, then LDS) into part[block][6] when the block lies in
 // one scene; a block that straddles scenes adds its lanes to the scenes'
-// accumulators with atomics.  k_grid_setup folds both.
+// accumulators with atomics.  k_grid_setup folds both.  With posr (rows in
+// cell order: sorted position = row) it also writes k_posr's outputs, which
+// a step without rebuild then uses as they are.
 __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, bool offs, Scalars* sc,
                                                  int* part, int* partScene, int* sceneAcc, const double* refC,
-                                                 double halfSkin2)
+                                                 double halfSkin2, double4* posr, uint32_t* cidS, int4* meta)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
@@ -385,6 +387,12 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
             atomicAdd(&sc->grid_overflow, 1);
         }
         S.k(I_VALID, p) = ok ? 1 : 0;
+        if (posr) {
+            posr[p] = make_double4(cx, cy, cz, S.f(F_CR, p));
+            cidS[p] = (uint32_t)S.k(I_CID, p);
+            meta[p] = ok ? make_int4(lo[0], lo[1], lo[2], S.k(I_ORIG, p))
+                         : make_int4(INT_MIN / 2, INT_MIN / 2, INT_MIN / 2, S.k(I_ORIG, p));
+        }
     }
     if (refC) atomic_max_wave(&sc->need_rebuild, moved);
     __shared__ int red[BLOCK / 64][8];
@@ -820,9 +828,8 @@ __global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Recs R)
 // the t-th sorted row (the scan skips partners of the same owner).
 __global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool offs, double4* posr,
                                                 uint32_t* cidS, int4* meta, const int* owner, int* ownS,
-                                                int* rank, const int* skip)
+                                                int* rank)
 {
-    if (skip && *skip) return;  // launched ahead of a decision that came out as a rebuild
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= S.n) return;
     const long p = sorted[t];
@@ -3322,12 +3329,14 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     const bool verlet = s->verlet;
     const bool lists = verlet && s->vlistValid;
     if (s->clump) hipLaunchKernelGGL(k_expand, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, R, s->cOwner);
+    // a list step on current records gets k_posr's outputs from k_cells
+    const bool fusedPosr = lists && s->nt == 0 && s->recFresh;
     hipLaunchKernelGGL(k_cells, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dSp, s->offs, s->sc,
                        s->boxPart, s->partScene, s->sceneAcc, lists ? s->refC : nullptr,
-                       0.25 * s->skin * s->skin);
+                       0.25 * s->skin * s->skin, fusedPosr ? s->posr : nullptr, s->cidS, s->meta);
     bool rebuild = true;
     bool triCounted = false;
-    bool scanned = false;  // k_posr and the list scan already queued
+    bool scanned = false;  // k_cells wrote k_posr's outputs and the list scan is queued
     if (lists) {
         if (s->capturing) return set_error(DEM_E_INVALID, "candidate-list steps cannot be captured");
         if (s->nt > 0) {
@@ -3338,17 +3347,15 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
             rebuild = s->hScalars->need_rebuild != 0;
             triCounted = true;
-        } else if (s->recFresh) {
-            // the step's scan is queued before the flag is read, each kernel
-            // doing nothing if the flag asks for a rebuild, so the device does
-            // not wait for the host's decision; the scalars (with the range
-            // flag of k_cells) come back with the flag
+        } else if (fusedPosr) {
+            // the step's scan is queued before the flag is read (doing nothing
+            // if the flag asks for a rebuild), so the device does not wait for
+            // the host's decision; the scalars (with the range flag of k_cells)
+            // come back with the flag
             HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st),
                     "read displacement flag");
             HIP_TRY(hipEventRecord(s->flagEv, st), "event");
             const int* skip = &s->sc->need_rebuild;
-            hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr,
-                               s->cidS, s->meta, s->cOwner, s->ownS, s->cRank, skip);
             if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
             hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta,
                                s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr, s->nkey, s->ncount,
@@ -3409,7 +3416,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     }
     if (!scanned)
         hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr, s->cidS,
-                           verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank, (const int*)nullptr);
+                           verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank);
     if (!s->recFresh) hipLaunchKernelGGL(k_pack, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->offs, s->recs());
     TRY(launch_check("grid build"));
     bool useLists = verlet;
