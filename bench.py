@@ -32,11 +32,13 @@ HBM_PEAK_GBS = 8000.0
 
 def pairs_bytes(contacts: float, n: int) -> float:
     """Algorithmic HBM bytes of one k_pairs launch: per stored contact the
-    list entry (8 B), the history slot's 9 f64 read and written (144 B) and
-    the 6 f64 result (48 B); per sphere its 144-byte contact record (a
-    128-byte kinematic line and 16 bytes of constants, read once, as owner
-    and as partner), its sorted-order and history-row indices (8 B)."""
-    return 200.0 * contacts + 152.0 * n
+    list entry (8 B) and the history slot's 9 f64 read and written (144 B);
+    per sphere its 144-byte contact record (a 128-byte kinematic line and 16
+    bytes of constants, read once, as owner and as partner), its sorted-order
+    and history-row indices (8 B), its list offset and count (8 B) and its
+    6 f64 contact sum (48 B).  (Contacts of a row whose list straddles a
+    64-entry chunk write their own 6 f64 instead; the model leaves that out.)"""
+    return 152.0 * contacts + 208.0 * n
 
 
 def step_bytes_per_particle(cbar: float, passes: int = 3) -> float:

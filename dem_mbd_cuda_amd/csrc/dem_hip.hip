@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -2497,6 +2498,9 @@ struct dem_solver {
     uint32_t* nkey = nullptr;  // [MAX_LIST][n] order keys of the current contacts
     int4* meta = nullptr;      // [n] cell and input row in cell order
     int* hFlag = nullptr;      // pinned: displacement / overflow flags
+    int* hMirror = nullptr;    // pinned, coherent: [seq, need_rebuild, grid_overflow] written by k_vscan
+    int* dMirror = nullptr;    // device address of hMirror
+    int mirrorSeq = 0;
     History H2{};              // second history buffer (candidate-list rebuilds move rows)
     long rebuilds = 0;
     bool verletBroken = false;  // a rebuild overflowed CAND_MAX: lists stay off
@@ -2615,7 +2619,6 @@ struct dem_solver {
     bool keepBodyParticle = false, keepBodyDetail = false;
     // timing
     hipEvent_t ev[NEV] = {};
-    hipEvent_t flagEv = nullptr;  // the displacement flag has reached host memory
     bool evReady = false;
     bool timing = false;  // record the stage events of dem_debug_last_timings
     // metadata for checkpoints
@@ -2648,11 +2651,11 @@ struct dem_solver {
         if (hLoads) hipHostFree(hLoads);
         if (hScalars) hipHostFree(hScalars);
         if (hFlag) hipHostFree(hFlag);
+        if (hMirror) hipHostFree(hMirror);
         if (hTri) hipHostFree(hTri);
         if (hTriOff) hipHostFree(hTriOff);
         for (auto& e : ev)
             if (e) hipEventDestroy(e);
-        if (flagEv) hipEventDestroy(flagEv);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -2843,7 +2846,6 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipMemGetInfo(&s->freeBefore, &total), "query HIP memory before allocation");
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
-    HIP_TRY(hipEventCreateWithFlags(&s->flagEv, hipEventDisableTiming), "hipEventCreate");
 
     s->n = n;
     s->nc = nc;
@@ -3006,6 +3008,10 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
             TRY(dalloc(s->meta, n, "hipMalloc(candidate lists)"));
             HIP_TRY(hipHostMalloc((void**)&s->hFlag, sizeof(int) * 2, hipHostMallocDefault), "hipHostMalloc");
             s->hFlag[0] = s->hFlag[1] = 0;
+            HIP_TRY(hipHostMalloc((void**)&s->hMirror, sizeof(int) * 4, hipHostMallocMapped | hipHostMallocCoherent),
+                    "hipHostMalloc");
+            std::memset(s->hMirror, 0, sizeof(int) * 4);
+            HIP_TRY(hipHostGetDevicePointer((void**)&s->dMirror, s->hMirror, 0), "hipHostGetDevicePointer");
             s->verlet = true;
         }
     }
@@ -3296,6 +3302,23 @@ int build_triangle_grid(dem_solver* s, bool counted = false)
 
 }  // namespace
 
+// Waits until k_vscan has written sequence number `seq` into the mirror (a
+// few microseconds after k_cells ends); falls back to a stream
+// synchronisation after a second.
+static int await_mirror(dem_solver* s, int seq)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    while (__atomic_load_n(&s->hMirror[0], __ATOMIC_ACQUIRE) != seq) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
+            HIP_TRY(hipStreamSynchronize(s->stream), "read displacement flag");
+            if (__atomic_load_n(&s->hMirror[0], __ATOMIC_ACQUIRE) != seq)
+                return set_error(DEM_E_DEVICE, "displacement flag not received");
+            break;
+        }
+    }
+    return DEM_OK;
+}
+
 int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int collect)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
@@ -3350,19 +3373,18 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         } else if (fusedPosr) {
             // the step's scan is queued before the flag is read (doing nothing
             // if the flag asks for a rebuild), so the device does not wait for
-            // the host's decision; the scalars (with the range flag of k_cells)
-            // come back with the flag
-            HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st),
-                    "read displacement flag");
-            HIP_TRY(hipEventRecord(s->flagEv, st), "event");
+            // the host's decision; its first block hands the flag and the range
+            // count of k_cells to host memory under a sequence number
             const int* skip = &s->sc->need_rebuild;
+            const int seq = ++s->mirrorSeq;
             if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
             hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta,
                                s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr, s->nkey, s->ncount,
-                               s->tape.ppOcc, collect || s->countAlways, s->listBlk, skip);
+                               s->tape.ppOcc, collect || s->countAlways, s->listBlk, skip, s->dMirror, seq);
             TRY(launch_check("contact scan"));
-            HIP_TRY(hipEventSynchronize(s->flagEv), "read displacement flag");
-            rebuild = s->hScalars->need_rebuild != 0;
+            TRY(await_mirror(s, seq));
+            rebuild = s->hMirror[1] != 0;
+            s->hScalars->grid_overflow = s->hMirror[2];
             scanned = !rebuild;
         } else {
             HIP_TRY(hipMemcpyAsync(s->hFlag, &s->sc->need_rebuild, sizeof(int), hipMemcpyDeviceToHost, st),
@@ -3450,7 +3472,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     else if (useLists)
         hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta, s->cidS,
                            s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->nkey, s->ncount, s->tape.ppOcc,
-                           counters, s->listBlk, (const int*)nullptr);
+                           counters, s->listBlk, (const int*)nullptr, (int*)nullptr, 0);
     else if (s->clump)
         hipLaunchKernelGGL(k_scan<true>, dim3(nblk), dim3(BLOCK), 0, st, R, s->valA, s->posr, s->cidS, s->keyA,
                            s->cellStart, s->dGrid, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,

@@ -74,8 +74,15 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
                                                  const uint32_t* cidS, const int* cand, const int* ncand,
                                                  Scalars* sc, History H, int step, int* nbr, uint32_t* nkey,
                                                  int* ncount, unsigned long long* occTape, bool counters,
-                                                 unsigned* bsum, const int* skip)
+                                                 unsigned* bsum, const int* skip, int* mirror, int seq)
 {
+    if (mirror && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the rebuild flag and range count of k_cells, for the host (which
+        // waits for `seq`)
+        mirror[1] = sc->need_rebuild;
+        mirror[2] = sc->grid_overflow;
+        __hip_atomic_store(mirror, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     if (skip && *skip) return;  // launched ahead of a decision that came out as a rebuild
     const long b = xcd_block(blockIdx.x, gridDim.x);
     const long t = b * BLOCK + threadIdx.x;

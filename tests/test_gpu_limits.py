@@ -117,3 +117,22 @@ def test_cell_range_error(hip, oracle):
         msgs.append(str(e.value))
         s.close()
     assert msgs[0] == msgs[1] and "21-bit" in msgs[0]
+
+
+def test_stage_timings_only_when_enabled(hip):
+    """Stage events are recorded only while enabled: the last evaluation made
+    with timings on reports five non-negative stage times, and nothing is
+    reported once they are turned off."""
+    sc = crowded(40, -2e-5)
+    s = hip.solver()
+    s.initialize(sc)
+    s.compute_forces(collect=False)
+    assert len(s.last_timings()) == 0
+    s.enable_timings(True)
+    s.compute_forces(collect=False)
+    t = s.last_timings()
+    assert len(t) == 5 and np.all(t >= 0.0) and t.sum() > 0.0
+    s.enable_timings(False)
+    s.compute_forces(collect=False)
+    assert len(s.last_timings()) == 0
+    s.close()
