@@ -2619,6 +2619,7 @@ struct dem_solver {
     bool keepBodyParticle = false, keepBodyDetail = false;
     // timing
     hipEvent_t ev[NEV] = {};
+    hipEvent_t flagEv = nullptr;  // the scalars copy with the displacement flag has landed
     bool evReady = false;
     bool timing = false;  // record the stage events of dem_debug_last_timings
     // metadata for checkpoints
@@ -2656,6 +2657,7 @@ struct dem_solver {
         if (hTriOff) hipHostFree(hTriOff);
         for (auto& e : ev)
             if (e) hipEventDestroy(e);
+        if (flagEv) hipEventDestroy(flagEv);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -2846,6 +2848,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipMemGetInfo(&s->freeBefore, &total), "query HIP memory before allocation");
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+    HIP_TRY(hipEventCreateWithFlags(&s->flagEv, hipEventDisableTiming), "hipEventCreate");
 
     s->n = n;
     s->nc = nc;
@@ -3353,7 +3356,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     const bool lists = verlet && s->vlistValid;
     if (s->clump) hipLaunchKernelGGL(k_expand, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, R, s->cOwner);
     // a list step on current records gets k_posr's outputs from k_cells
-    const bool fusedPosr = lists && s->nt == 0 && s->recFresh;
+    const bool fusedPosr = lists && s->recFresh;
     hipLaunchKernelGGL(k_cells, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dSp, s->offs, s->sc,
                        s->boxPart, s->partScene, s->sceneAcc, lists ? s->refC : nullptr,
                        0.25 * s->skin * s->skin, fusedPosr ? s->posr : nullptr, s->cidS, s->meta);
@@ -3363,12 +3366,27 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (lists) {
         if (s->capturing) return set_error(DEM_E_INVALID, "candidate-list steps cannot be captured");
         if (s->nt > 0) {
-            // the triangle grid's box and total come back with the displacement flag
+            // the triangle grid's box and total come back with the displacement
+            // flag; the particle scan (doing nothing if the flag asks for a
+            // rebuild) is queued before they are read, so the device works
+            // while the host sizes the triangle grid
             TRY(tri_grid_count(s));
             HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st),
                     "read displacement flag");
-            HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
+            if (fusedPosr) {
+                HIP_TRY(hipEventRecord(s->flagEv, st), "event");
+                if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
+                hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->valA, s->posr,
+                                   s->meta, s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr,
+                                   s->nkey, s->ncount, s->tape.ppOcc, collect || s->countAlways, s->listBlk,
+                                   (const int*)&s->sc->need_rebuild, (int*)nullptr, 0);
+                TRY(launch_check("contact scan"));
+                HIP_TRY(hipEventSynchronize(s->flagEv), "read displacement flag");
+            } else {
+                HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
+            }
             rebuild = s->hScalars->need_rebuild != 0;
+            scanned = fusedPosr && !rebuild;
             triCounted = true;
         } else if (fusedPosr) {
             // the step's scan is queued before the flag is read (doing nothing
