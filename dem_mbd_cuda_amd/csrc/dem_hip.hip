@@ -2579,6 +2579,7 @@ struct dem_solver {
     double* tAabb = nullptr;
     double* pose = nullptr;
     double* bodyMass = nullptr;
+    std::vector<double> hBodyMass;  // host copy (fixed at initialisation)
     int* tCounts = nullptr;
     long long* tOffsets = nullptr;
     long long tCap = 0;
@@ -3131,6 +3132,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         std::vector<double> mass(B->mass, B->mass + nb);
         TRY(dalloc(s->bodyMass, nb, "hipMalloc(body mass)"));
         HIP_TRY(hipMemcpy(s->bodyMass, mass.data(), sizeof(double) * nb, hipMemcpyHostToDevice), "upload");
+        s->hBodyMass = mass;
         s->cRegInst.assign(7 * nb, 0); s->cRegInt.assign(7 * nb, 0);
         s->cMicInst.assign(22 * nb, 0.0); s->cMicInt.assign(22 * nb, 0.0);
         HIP_TRY(hipDeviceSynchronize(), "initialize");
@@ -3616,8 +3618,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     }
     if (needLoads) {
         HIP_TRY(hipStreamSynchronize(st), "download body resultant forces");
-        std::vector<double> mass(nb);
-        HIP_TRY(hipMemcpy(mass.data(), s->bodyMass, sizeof(double) * nb, hipMemcpyDeviceToHost), "download mass");
+        const std::vector<double>& mass = s->hBodyMass;
         for (int b = 0; b < nb; ++b) {
             for (int k = 0; k < 6; ++k) loads6[6 * b + k] = s->hLoads[6 * b + k];
             if (s->gravity) {
