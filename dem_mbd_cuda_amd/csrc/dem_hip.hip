@@ -2225,15 +2225,35 @@ __global__ void __launch_bounds__(BLOCK) k_tri_count(TriSet T, double inv, doubl
             hi[0] = (int)mxx; hi[1] = (int)mxy; hi[2] = (int)mxz;
         }
     }
-    atomic_add_wave(&sc->tri_overflow, over);
+    // reduced per wave, then per block: one atomic per block and quantity
+    __shared__ int red[BLOCK / 64][7];
+    __shared__ unsigned long long redCnt[BLOCK / 64];
+    const int wave = threadIdx.x >> 6;
     const bool lead = (threadIdx.x & 63) == 0;
     for (int a = 0; a < 3; ++a) {
         const int l = wave_min(lo[a]), h = wave_max(hi[a]);
-        if (lead && l != INT_MAX) atomicMin(&sc->tri_lo[a], l);
-        if (lead && h != INT_MIN) atomicMax(&sc->tri_hi[a], h);
+        if (lead) { red[wave][a] = l; red[wave][3 + a] = h; }
     }
+    const int o = wave_sum(over);
     cnt = wave_sum(cnt);
-    if (lead && cnt) atomicAdd(&sc->tri_total, cnt);
+    if (lead) { red[wave][6] = o; redCnt[wave] = cnt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int l[3], h[3], ov = 0;
+        unsigned long long total = 0;
+        for (int a = 0; a < 3; ++a) { l[a] = INT_MAX; h[a] = INT_MIN; }
+        for (int w = 0; w < BLOCK / 64; ++w) {
+            for (int a = 0; a < 3; ++a) { l[a] = min(l[a], red[w][a]); h[a] = max(h[a], red[w][3 + a]); }
+            ov += red[w][6];
+            total += redCnt[w];
+        }
+        for (int a = 0; a < 3; ++a) {
+            if (l[a] != INT_MAX) atomicMin(&sc->tri_lo[a], l[a]);
+            if (h[a] != INT_MIN) atomicMax(&sc->tri_hi[a], h[a]);
+        }
+        if (ov) atomicAdd(&sc->tri_overflow, ov);
+        if (total) atomicAdd(&sc->tri_total, total);
+    }
 }
 
 // fillTriangleCellsKernel (contact:453-476): x -> y -> z nested order.  The
