@@ -102,6 +102,25 @@ def test_reorder_interval_invariance(hip, oracle, reorder, monkeypatch):
     compare_frames(g, o)
 
 
+def test_candidate_lists_rebuilt_every_few_steps(hip, oracle, monkeypatch):
+    """A skin of 0.001 x the largest radius (the rebuild threshold, half the
+    skin, is 0.75 um) with spheres moving about 0.5 um per step: candidate
+    lists are rebuilt every few steps, so list steps whose queued scan runs
+    alternate with steps whose queued scan finds the rebuild flag and does
+    nothing, and the rebuild path takes over."""
+    from dem_mbd_cuda_amd import scene
+
+    monkeypatch.setenv("DEM_SKIN", "0.001")
+    sc = scene.box_scene(n=1500, seed=13, box=(0.04, 0.04, 0.04), steps=60, dt=1e-6)
+    rng = np.random.RandomState(17)
+    sc.vel = rng.uniform(-0.5, 0.5, (sc.n, 3))
+    sc.angvel = rng.uniform(-30, 30, (sc.n, 3))
+    g = run_frames(hip, sc, interval=20, debug=True)
+    o = run_frames(oracle, sc, interval=20, debug=True)
+    assert o[60]["stats"]["particle_contacts"] > 0
+    compare_frames(g, o)
+
+
 def test_data1_coupled(hip, oracle):
     """DATA1 (1,704 spheres, 5 walls, 2 dynamic bodies x 1,416 triangles, host
     Newton-Euler between steps), 5 steps, bit for bit."""
