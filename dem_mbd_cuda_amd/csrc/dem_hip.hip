@@ -127,6 +127,7 @@ struct SceneGrid {
 struct Scalars {
     unsigned long long ncells;  // all scenes
     int nbits;
+    int mbits;          // key bits of the Morton row order (candidate-list rebuilds)
     int dense;
     int grid_overflow;
     int list_overflow;
@@ -720,6 +721,57 @@ __global__ void __launch_bounds__(BLOCK) k_identity(int* sorted, long n)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (t < n) sorted[t] = (int)t;
+}
+
+// Row layout between candidate-list rebuilds: the rows are put in Morton
+// order of their cells (16 bits per axis, x y z interleaved, under 4 bits of
+// scene), rows of one cell in the grid's order.  Neighbouring cells in all
+// three directions then lie close in memory, so the partner gathers of the
+// list steps stay in L2.  Any row order gives the same results: the contact
+// order comes from the reference's key, the history moves with the rows, and
+// the sorted order of the grid is kept in key/sorted (sorted position ->
+// row) for the rebuild scan and the debug output.
+__device__ __forceinline__ unsigned long long spread3(unsigned v)
+{
+    unsigned long long x = v & 0xFFFFu;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_morton_keys(Soa S, const int* sorted, const SceneGrid* grid,
+                                                       unsigned long long* key, int* val, Scalars* sc)
+{
+    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (t == 0) sc->mbits = 52;
+    if (t >= S.n) return;
+    const long p = sorted[t];
+    unsigned long long k = (1ull << 52) - 1ull;  // rows outside the hash range last
+    if (S.k(I_VALID, p)) {
+        const int scene = S.k(I_SCENE, p);
+        const SceneGrid& g = grid[scene];
+        const unsigned x = (unsigned)(S.k(I_CX, p) - g.lo[0]), y = (unsigned)(S.k(I_CY, p) - g.lo[1]),
+                       z = (unsigned)(S.k(I_CZ, p) - g.lo[2]);
+        k = ((unsigned long long)(scene & 15) << 48) | (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+    }
+    key[t] = k;
+    val[t] = (int)t;
+}
+
+// perm[j] = the old row that goes to row j; sorted[t] = the new row of the
+// row at sorted position t.
+__global__ void __launch_bounds__(BLOCK) k_morton_perm(const int* sorted, const int* order, int* perm, long n)
+{
+    const long j = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n) perm[j] = sorted[order[j]];
+}
+__global__ void __launch_bounds__(BLOCK) k_morton_rank(const int* order, int* sorted, long n)
+{
+    const long j = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (j < n) sorted[order[j]] = (int)j;
 }
 
 __device__ __forceinline__ long lower_bound_u64(const unsigned long long* k, long n,
@@ -2552,6 +2604,9 @@ struct dem_solver {
     unsigned long long* keyB = nullptr;
     int* valA = nullptr;
     int* valB = nullptr;
+    int* ident = nullptr;                 // [n] identity (list steps index rows directly)
+    unsigned long long* mKey = nullptr;   // [n] Morton keys of a candidate-list rebuild
+    int* mVal = nullptr;                  // [n] sorted positions in Morton order
     unsigned* rsCounts = nullptr;
     unsigned* rsDigits = nullptr;  // [256] digit totals, [256] digit bases
     int rsBlocks = 0;
@@ -2657,7 +2712,7 @@ struct dem_solver {
         if (stream) hipStreamSynchronize(stream);
         diff_release(diff);
         graph_release(graphs);
-        void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB,
+        void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB, ident, mKey, mVal,
                         rec, recAux, refC, cand, ncand, nkey, meta, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, rowF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
@@ -3030,6 +3085,10 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
             TRY(dalloc(s->ncand, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->nkey, (size_t)MAX_LIST * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->meta, n, "hipMalloc(candidate lists)"));
+            TRY(dalloc(s->ident, n, "hipMalloc(candidate lists)"));
+            TRY(dalloc(s->mKey, n, "hipMalloc(candidate lists)"));
+            TRY(dalloc(s->mVal, n, "hipMalloc(candidate lists)"));
+            hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->ident, n);
             HIP_TRY(hipHostMalloc((void**)&s->hFlag, sizeof(int) * 2, hipHostMallocDefault), "hipHostMalloc");
             s->hFlag[0] = s->hFlag[1] = 0;
             HIP_TRY(hipHostMalloc((void**)&s->hMirror, sizeof(int) * 4, hipHostMallocMapped | hipHostMallocCoherent),
@@ -3398,7 +3457,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             if (fusedPosr) {
                 HIP_TRY(hipEventRecord(s->flagEv, st), "event");
                 if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
-                hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->valA, s->posr,
+                hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ident, s->posr,
                                    s->meta, s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr,
                                    s->nkey, s->ncount, s->tape.ppOcc, collect || s->countAlways, s->listBlk,
                                    (const int*)&s->sc->need_rebuild, (int*)nullptr, 0);
@@ -3418,7 +3477,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             const int* skip = &s->sc->need_rebuild;
             const int seq = ++s->mirrorSeq;
             if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
-            hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta,
+            hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ident, s->posr, s->meta,
                                s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr, s->nkey, s->ncount,
                                s->tape.ppOcc, collect || s->countAlways, s->listBlk, skip, s->dMirror, seq);
             TRY(launch_check("contact scan"));
@@ -3461,8 +3520,21 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                                                       (s->stepIndex % s->reorderInterval) == 0));
         if (reorder) {
             Soa D{s->dState[1 - s->cur], s->iState[1 - s->cur], n};
-            hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valA);
-            hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->valA, n);
+            if (verlet) {
+                // rows in Morton order of their cells; valA becomes sorted
+                // position -> new row
+                hipLaunchKernelGGL(k_morton_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->dGrid,
+                                   s->mKey, s->mVal, s->sc);
+                TRY(radix_sort(st, s->mKey, s->mVal, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->mbits,
+                               s->rsDigits, 7));
+                hipLaunchKernelGGL(k_morton_perm, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->valA, s->mVal,
+                                   s->valB, n);
+                hipLaunchKernelGGL(k_morton_rank, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->mVal, s->valA, n);
+                hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valB);
+            } else {
+                hipLaunchKernelGGL(k_permute, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, D, s->valA);
+                hipLaunchKernelGGL(k_identity, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->valA, n);
+            }
             s->cur = 1 - s->cur;
             S = s->soa();
             R = S;
@@ -3476,15 +3548,17 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, nr, s->sc, s->cellStart,
                            s->cellCap);
     }
+    // with candidate lists the packed arrays are indexed by row, else by
+    // sorted position
     if (!scanned)
-        hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr, s->cidS,
-                           verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank);
+        hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, verlet ? s->ident : s->valA, s->offs,
+                           s->posr, s->cidS, verlet ? s->meta : nullptr, s->cOwner, s->ownS, s->cRank);
     if (!s->recFresh) hipLaunchKernelGGL(k_pack, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->offs, s->recs());
     TRY(launch_check("grid build"));
     bool useLists = verlet;
     if (verlet && rebuild) {
-        hipLaunchKernelGGL(k_vbuild, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->posr, s->keyA, s->cellStart,
-                           s->dGrid, s->sc, s->skin, s->cand, s->ncand, s->refC);
+        hipLaunchKernelGGL(k_vbuild, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->posr, s->keyA, s->valA,
+                           s->cellStart, s->dGrid, s->sc, s->skin, s->cand, s->ncand, s->refC);
         TRY(launch_check("candidate lists"));
         if (s->capturing) return set_error(DEM_E_INVALID, "candidate-list steps cannot be captured");
         HIP_TRY(hipMemcpyAsync(s->hFlag + 1, &s->sc->cand_overflow, sizeof(int), hipMemcpyDeviceToHost, st),
@@ -3497,6 +3571,9 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             s->verletBroken = true;
             s->vlistValid = false;
             useLists = false;
+            // the full scan reads the packed arrays by sorted position
+            hipLaunchKernelGGL(k_posr, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->valA, s->offs, s->posr,
+                               s->cidS, (int4*)nullptr, s->cOwner, s->ownS, s->cRank);
         } else {
             s->vlistValid = true;
         }
@@ -3505,12 +3582,14 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (timed && !scanned) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
     const int step = s->stepIndex++;
     const int nblk = blocks_for(nr);
+    // list steps index rows directly (rows in Morton order, see k_morton_keys)
+    const int* ord = useLists ? s->ident : s->valA;
     // the reference reads its per-step counters only on diagnostics steps
     const bool counters = collect || s->countAlways;
     if (scanned)
         ;
     else if (useLists)
-        hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->valA, s->posr, s->meta, s->cidS,
+        hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->ident, s->posr, s->meta, s->cidS,
                            s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->nkey, s->ncount, s->tape.ppOcc,
                            counters, s->listBlk, (const int*)nullptr, (int*)nullptr, 0);
     else if (s->clump)
@@ -3526,13 +3605,13 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                        s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
     if (timed) HIP_TRY(hipEventRecord(s->ev[2], st), "event");
-    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, s->valA, s->recs(), s->pairs,
+    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, ord, s->recs(), s->pairs,
                        (const unsigned*)s->listOff, (const int*)s->ncount, s->H, s->dMatPP, step == 0, s->offs,
                        s->sc, s->pairF, s->rowF, s->tape.pair, s->tape.pairPrev);
     TRY(launch_check("particle contact law"));
     if (timed) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
-    A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.posr = s->posr; A.rec = s->recs(); A.sorted = s->valA; A.ncount = s->ncount; A.off = s->listOff;
+    A.S = S; A.R = R; A.rank = s->cRank; A.ownerStart = s->ownerStart; A.posr = s->posr; A.rec = s->recs(); A.sorted = ord; A.ncount = s->ncount; A.off = s->listOff;
     A.pairF = s->pairF; A.rowF = s->rowF; A.H = s->H;
     A.W = Walls{s->nw, s->wallBuf, s->wallBuf + 3 * s->nw};
     A.T = triset(s); A.O = s->O; A.sc = s->sc;
@@ -3570,7 +3649,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     if (nb > 0) {
         // single-sphere owners: only the rows k_tri_sift listed; clumps: every row
         const int* work = s->clump ? nullptr : s->triWork;
-        hipLaunchKernelGGL(k_regime, dim3(REGIME_BLOCKS, (unsigned)nb), dim3(BLOCK), 0, st, s->O.regime, s->valA, nr, work,
+        hipLaunchKernelGGL(k_regime, dim3(REGIME_BLOCKS, (unsigned)nb), dim3(BLOCK), 0, st, s->O.regime, ord, nr, work,
                            (const Scalars*)s->sc, s->regimeInstant, s->regimeInterval, interval ? 1 : 0);
         // body loads [6 nb], Fz [nb] and microstate [22 nb] sums in one
         // reduction; the microstate sums also feed the interval

@@ -16,12 +16,12 @@
 
 constexpr int CAND_MAX = 96;  // listed partners per component
 
-// Rebuild: partners within r_i + r_j + skin, rows in cell order (the
-// arrays were just reordered, so sorted index = row), and the centres the
+// Rebuild: partners within r_i + r_j + skin (rows; the grid's cell ranges
+// are sorted positions, mapped to rows by `sorted`), and the centres the
 // displacement test compares against.
 __global__ void __launch_bounds__(BLOCK) k_vbuild(Soa S, const double4* posr, const unsigned long long* key,
-                                                  const unsigned* start, const SceneGrid* grid, Scalars* sc,
-                                                  double skin, int* cand, int* ncand, double* refC)
+                                                  const int* sorted, const unsigned* start, const SceneGrid* grid,
+                                                  Scalars* sc, double skin, int* cand, int* ncand, double* refC)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
@@ -53,12 +53,13 @@ __global__ void __launch_bounds__(BLOCK) k_vbuild(Soa S, const double4* posr, co
                     e = lower_bound_u64(key, n, col + zhi + 1);
                 }
                 for (long k = b; k < e; ++k) {
-                    if (k == t) continue;
-                    const double4 c = posr[k];
+                    const long q = sorted[k];
+                    if (q == t) continue;
+                    const double4 c = posr[q];
                     const double ex = c.x - me.x, ey = c.y - me.y, ez = c.z - me.z;
                     const double reach = me.w + c.w + skin;
                     if (ex * ex + ey * ey + ez * ez >= reach * reach) continue;
-                    if (count < CAND_MAX) cand[(long)count * n + t] = (int)k;
+                    if (count < CAND_MAX) cand[(long)count * n + t] = (int)q;
                     ++count;
                 }
             }
