@@ -484,8 +484,8 @@ int save_checkpoint(dem_solver* s, Checkpoint& c, int ppW, int pwW)
     if (ppW > 0) {
         HIP_TRY(hipMemcpy2DAsync(c.ppKey, sizeof(uint32_t) * ppW, s->H.ppKey, sizeof(uint32_t) * PP_SLOTS,
                                  sizeof(uint32_t) * ppW, n, hipMemcpyDeviceToDevice, st), "checkpoint");
-        HIP_TRY(hipMemcpy2DAsync(c.ppVal, sizeof(double) * 9 * ppW, s->H.ppVal, sizeof(double) * 9 * PP_SLOTS,
-                                 sizeof(double) * 9 * ppW, n, hipMemcpyDeviceToDevice, st), "checkpoint");
+        // values are slot-major: the first ppW slots are one contiguous block
+        HIP_TRY(cp(c.ppVal, s->H.ppVal, sizeof(double) * 9 * n * ppW), "checkpoint");
     }
     if (pwW > 0) {
         HIP_TRY(hipMemcpy2DAsync(c.pwKey, sizeof(uint32_t) * pwW, s->H.pwKey, sizeof(uint32_t) * PW_SLOTS,
@@ -513,8 +513,7 @@ int restore_checkpoint(dem_solver* s, const Checkpoint& c)
     if (c.ppW > 0) {
         HIP_TRY(hipMemcpy2DAsync(s->H.ppKey, sizeof(uint32_t) * PP_SLOTS, c.ppKey, sizeof(uint32_t) * c.ppW,
                                  sizeof(uint32_t) * c.ppW, n, hipMemcpyDeviceToDevice, st), "restore checkpoint");
-        HIP_TRY(hipMemcpy2DAsync(s->H.ppVal, sizeof(double) * 9 * PP_SLOTS, c.ppVal, sizeof(double) * 9 * c.ppW,
-                                 sizeof(double) * 9 * c.ppW, n, hipMemcpyDeviceToDevice, st), "restore checkpoint");
+        HIP_TRY(cp(s->H.ppVal, c.ppVal, sizeof(double) * 9 * n * c.ppW), "restore checkpoint");
     }
     if (c.pwW > 0) {
         HIP_TRY(hipMemcpy2DAsync(s->H.pwKey, sizeof(uint32_t) * PW_SLOTS, c.pwKey, sizeof(uint32_t) * c.pwW,

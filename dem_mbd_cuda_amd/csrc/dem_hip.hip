@@ -149,7 +149,7 @@ struct History {
     unsigned long long* ppOcc;  // [n]
     unsigned long long* ppSeen; // [2][n]
     uint32_t* ppKey;            // [n][64]
-    double* ppVal;              // [n][64][9]
+    double* ppVal;              // [64][n][9] (slot-major: a slot of consecutive rows is contiguous)
     unsigned* pwOcc;            // [n]
     unsigned* pwSeen;           // [2][n]
     uint32_t* pwKey;            // [n][16]
@@ -158,7 +158,16 @@ struct History {
     unsigned long long* ptSeen; // [2][n]
     uint32_t* ptKey;            // [n][64]
     double* ptVal;              // [n][64][9]
+    long rows;                  // n
 };
+
+// Particle-pair history values of slot `slot` of history row `o`.  Slot-major,
+// so the contacts of a wave (consecutive rows, mostly the low slots) read and
+// write a few contiguous runs instead of a partial line per row.
+__device__ __forceinline__ double* pp_slot(const History& H, long o, int slot)
+{
+    return H.ppVal + ((long)slot * H.rows + o) * 9;
+}
 
 struct Walls {
     int n;
@@ -917,7 +926,7 @@ __device__ __forceinline__ long xcd_block(long b, long nb)
 // History slot acquisition (acquireHistorySlot, contact:190-215) on an
 // occupancy mask.  Returns the slot or -1 (overflow).
 template <typename M>
-__device__ __forceinline__ int acquire(M& occ, M& cur, uint32_t* keys, double* vals, int slots,
+__device__ __forceinline__ int acquire(M& occ, M& cur, uint32_t* keys, double* vals, long vstride, int slots,
                                        uint32_t key, int& high, int& overflow)
 {
     M m = occ;
@@ -940,7 +949,7 @@ __device__ __forceinline__ int acquire(M& occ, M& cur, uint32_t* keys, double* v
     occ |= (M)1 << b;
     cur |= (M)1 << b;
     keys[b] = key;
-    double* v = vals + (long)b * 9;
+    double* v = vals + (long)b * vstride;
     for (int q = 0; q < 9; ++q) v[q] = 0.0;
     high = max(high, b + 1);
     return b;
@@ -996,7 +1005,7 @@ struct LawArgs {
 // in step with memory when a slot is reassigned): a hit among them needs no
 // dependent load.  Same result as acquire().
 __device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long long& cur,
-                                          uint32_t (&k16)[16], uint32_t* keys, double* vals,
+                                          uint32_t (&k16)[16], uint32_t* keys, double* vals, long vstride,
                                           uint32_t key, int& high, int& overflow)
 {
     unsigned match = 0;
@@ -1031,7 +1040,7 @@ __device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long
 #pragma unroll
     for (int i = 0; i < 16; ++i)
         if (i == b) k16[i] = key;  // keep the register copy current (a reused slot)
-    double* v = vals + (long)b * 9;
+    double* v = vals + (long)b * vstride;
     for (int q = 0; q < 9; ++q) v[q] = 0.0;
     high = max(high, b + 1);
     return b;
@@ -1052,7 +1061,7 @@ __device__ __forceinline__ int assign_slots(const Soa& S, long t, long p, int fo
     if (occTape) occTape[o] = occ;
     unsigned long long cur = 0;
     uint32_t* keys = H.ppKey + o * PP_SLOTS;
-    double* vals = H.ppVal + o * PP_SLOTS * 9;
+    double* vals = pp_slot(H, o, 0);
     uint32_t k16[16];
     {
         const uint4* kv = reinterpret_cast<const uint4*>(keys);
@@ -1065,7 +1074,7 @@ __device__ __forceinline__ int assign_slots(const Soa& S, long t, long p, int fo
     int stored = 0;
     for (int c = 0; c < found; ++c) {
         const long k = nbr[(long)c * n + t];
-        const int slot = acquire_pp(occ, cur, k16, keys, vals, cidS[k] + 1u, high, over);
+        const int slot = acquire_pp(occ, cur, k16, keys, vals, H.rows * 9, cidS[k] + 1u, high, over);
         if (slot < 0) continue;  // slot overflow: counted, no force
         nbr[(long)stored * n + t] = (sorted[k] << SLOT_BITS) | slot;
         ++stored;
@@ -1092,10 +1101,10 @@ __device__ __forceinline__ int assign_inline(const Soa& S, long t, long p, const
     if (occTape) occTape[o] = occ;
     unsigned long long cur = 0;
     uint32_t* keys = H.ppKey + o * PP_SLOTS;
-    double* vals = H.ppVal + o * PP_SLOTS * 9;
+    double* vals = pp_slot(H, o, 0);
     int stored = 0;
     visit([&](long k) {
-        const int slot = acquire(occ, cur, keys, vals, PP_SLOTS, cidS[k] + 1u, high, over);
+        const int slot = acquire(occ, cur, keys, vals, H.rows * 9, PP_SLOTS, cidS[k] + 1u, high, over);
         if (slot < 0) return;
         nbr[(long)stored * n + t] = (sorted[k] << SLOT_BITS) | slot;
         ++stored;
@@ -1286,7 +1295,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs 
         const long o = S.k(I_HIST, p);
         Material m = mats[rec.aux[p].scene];
         if (first) m.dt = 0.0;  // the first evaluation advances no history
-        double* hv = H.ppVal + (o * PP_SLOTS + slot) * 9;
+        double* hv = pp_slot(H, o, slot);
         if (tapePair) {
             tapePair[e] = make_int2((int)p, pr.y);
             for (int k = 0; k < 9; ++k) tapePrev[e * 9 + k] = hv[k];
@@ -1429,7 +1438,7 @@ __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long 
         const double distance = dot(sub(I.pos, origin), outward);
         const double depth = I.r - distance;
         if (depth <= 0.0) continue;
-        const int slot = acquire(pwOcc, pwCur, H.pwKey + hr * PW_SLOTS, H.pwVal + hr * PW_SLOTS * 9,
+        const int slot = acquire(pwOcc, pwCur, H.pwKey + hr * PW_SLOTS, H.pwVal + hr * PW_SLOTS * 9, 9L,
                                  PW_SLOTS, (uint32_t)(w + 1), bodyHigh, pwOver);
         if (slot < 0) continue;
         double* hv = H.pwVal + (hr * PW_SLOTS + slot) * 9;
@@ -1630,7 +1639,7 @@ __device__ __forceinline__ void tri_pass_geo(const LawArgs& A, const Body& I, lo
                 const V3 mN = unit(wN);
                 const V3 mC = scl(wC, 1.0 / total);
                 const double mD = wD / total;
-                const int slot = acquire(ptOcc, ptCur, H.ptKey + hr * PT_SLOTS, H.ptVal + hr * PT_SLOTS * 9,
+                const int slot = acquire(ptOcc, ptCur, H.ptKey + hr * PT_SLOTS, H.ptVal + hr * PT_SLOTS * 9, 9L,
                                          PT_SLOTS, (uint32_t)(body + 1), bodyHigh, ptOver);
                 if (slot >= 0) {
                     double* sv = H.ptVal + (hr * PT_SLOTS + slot) * 9;
@@ -3100,6 +3109,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     }
     TRY(dalloc(s->byorig, n, "hipMalloc(row index)"));
     // history
+    s->H.rows = nr;
     TRY(dalloc(s->H.ppOcc, nr, "hipMalloc(particle history)"));
     TRY(dalloc(s->H.ppSeen, 2 * nr, "hipMalloc(particle history)"));
     TRY(dalloc(s->H.ppKey, (size_t)PP_SLOTS * nr, "hipMalloc(particle history keys)"));
@@ -3115,6 +3125,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipMemset(s->H.ppKey, 0, sizeof(uint32_t) * PP_SLOTS * nr), "clear history");
     HIP_TRY(hipMemset(s->H.pwKey, 0, sizeof(uint32_t) * PW_SLOTS * nr), "clear history");
     if (s->verlet) {
+        s->H2.rows = nr;
         TRY(dalloc(s->H2.ppOcc, nr, "hipMalloc(particle history)"));
         TRY(dalloc(s->H2.ppSeen, 2 * nr, "hipMalloc(particle history)"));
         TRY(dalloc(s->H2.ppKey, (size_t)PP_SLOTS * nr, "hipMalloc(particle history keys)"));
@@ -3934,6 +3945,25 @@ int dem_debug_grid(const dem_solver* s, uint64_t* keys, int32_t* ids)
     return DEM_OK;
 }
 
+// The particle-pair value table is slot-major on the device ([slot][row][9]);
+// the host tables are row-major ([row][slot][9]).
+static void pp_to_row_major(std::vector<double>& v, size_t rows)
+{
+    std::vector<double> r(v.size());
+    for (size_t b = 0; b < (size_t)PP_SLOTS; ++b)
+        for (size_t h = 0; h < rows; ++h)
+            std::memcpy(&r[(h * PP_SLOTS + b) * 9], &v[(b * rows + h) * 9], 9 * sizeof(double));
+    v.swap(r);
+}
+static void pp_to_slot_major(std::vector<double>& v, size_t rows)
+{
+    std::vector<double> r(v.size());
+    for (size_t h = 0; h < rows; ++h)
+        for (size_t b = 0; b < (size_t)PP_SLOTS; ++b)
+            std::memcpy(&r[(b * rows + h) * 9], &v[(h * PP_SLOTS + b) * 9], 9 * sizeof(double));
+    v.swap(r);
+}
+
 int dem_debug_history(const dem_solver* s, uint64_t* pk, double* pv, uint64_t* wk, double* wv, uint64_t* bk,
                       double* bv)
 {
@@ -3967,6 +3997,7 @@ int dem_debug_history(const dem_solver* s, uint64_t* pk, double* pv, uint64_t* w
         if (T.ov) {
             val.resize((size_t)T.slots * 9 * n);
             HIP_TRY(hipMemcpy(val.data(), T.val, sizeof(double) * val.size(), hipMemcpyDeviceToHost), "download");
+            if (T.val == s->H.ppVal) pp_to_row_major(val, (size_t)n);
         }
         for (long i = 0; i < n; ++i) {
             const long h = hrow[i];
@@ -4186,6 +4217,7 @@ int dem_load_contact_history(dem_solver* s, const char* path)
     HIP_TRY(hipMemcpy(s->H.ppOcc, pocc.data(), 8 * nc, hipMemcpyHostToDevice), "upload particle history");
     HIP_TRY(hipMemcpy(s->H.ppSeen, pseen.data(), 16 * nc, hipMemcpyHostToDevice), "upload particle history");
     HIP_TRY(hipMemcpy(s->H.ppKey, pk32.data(), 4 * pk32.size(), hipMemcpyHostToDevice), "upload particle history keys");
+    pp_to_slot_major(pv, nc);
     HIP_TRY(hipMemcpy(s->H.ppVal, pv.data(), 8 * pv.size(), hipMemcpyHostToDevice), "upload particle history values");
     HIP_TRY(hipMemcpy(s->H.pwOcc, wocc32.data(), 4 * nc, hipMemcpyHostToDevice), "upload wall history");
     HIP_TRY(hipMemcpy(s->H.pwSeen, wseen.data(), 8 * nc, hipMemcpyHostToDevice), "upload wall history");
@@ -4247,6 +4279,7 @@ int dem_save_contact_history(const dem_solver* cs, const char* path)
         std::vector<double> v(nc * slots_ * 9);
         HIP_TRY(hipMemcpy(k32.data(), keyDev, 4 * k32.size(), hipMemcpyDeviceToHost), "download history keys");
         HIP_TRY(hipMemcpy(v.data(), valDev, 8 * v.size(), hipMemcpyDeviceToHost), "download history values");
+        if (valDev == cs->H.ppVal) pp_to_row_major(v, nc);
         for (size_t i = 0; i < nc; ++i) {
             const size_t h = (size_t)hrow[i];
             for (int b = 0; b < slots_; ++b)

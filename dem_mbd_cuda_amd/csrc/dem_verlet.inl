@@ -202,9 +202,11 @@ __global__ void __launch_bounds__(BLOCK) k_hist_move(Soa S, History src, History
     dst.ppSeen[n + t] = src.ppSeen[n + h];
     const int top = occ ? 64 - __clzll((long long)occ) : 0;
     for (int b = 0; b < top; ++b) dst.ppKey[t * PP_SLOTS + b] = src.ppKey[h * PP_SLOTS + b];
-    const double* sv = src.ppVal + h * PP_SLOTS * 9;
-    double* dv = dst.ppVal + t * PP_SLOTS * 9;
-    for (int k = 0; k < top * 9; ++k) dv[k] = sv[k];
+    for (int b = 0; b < top; ++b) {
+        const double* sv = pp_slot(src, h, b);
+        double* dv = pp_slot(dst, t, b);
+        for (int k = 0; k < 9; ++k) dv[k] = sv[k];
+    }
     const unsigned wocc = src.pwOcc[h];
     dst.pwOcc[t] = wocc;
     dst.pwSeen[t] = src.pwSeen[h];
