@@ -733,8 +733,9 @@ __global__ void __launch_bounds__(BLOCK) k_identity(int* sorted, long n)
 }
 
 // Row layout between candidate-list rebuilds: the rows are put in Morton
-// order of their cells (16 bits per axis, x y z interleaved, under 4 bits of
-// scene), rows of one cell in the grid's order.  Neighbouring cells in all
+// order of their cells (16 bits per axis, x y z interleaved) under their
+// scene index (so a scene's rows stay contiguous, as the per-scene block
+// reductions want), rows of one cell in the grid's order.  Neighbouring cells in all
 // three directions then lie close in memory, so the partner gathers of the
 // list steps stay in L2.  Any row order gives the same results: the contact
 // order comes from the reference's key, the history moves with the rows, and
@@ -752,19 +753,20 @@ __device__ __forceinline__ unsigned long long spread3(unsigned v)
 }
 
 __global__ void __launch_bounds__(BLOCK) k_morton_keys(Soa S, const int* sorted, const SceneGrid* grid,
-                                                       unsigned long long* key, int* val, Scalars* sc)
+                                                       unsigned long long* key, int* val, Scalars* sc, int bits)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
-    if (t == 0) sc->mbits = 52;
+    if (t == 0) sc->mbits = bits;
     if (t >= S.n) return;
     const long p = sorted[t];
-    unsigned long long k = (1ull << 52) - 1ull;  // rows outside the hash range last
+    // rows outside the hash range last within their scene
+    unsigned long long k = ((unsigned long long)S.k(I_SCENE, p) << 48) | ((1ull << 48) - 1ull);
     if (S.k(I_VALID, p)) {
         const int scene = S.k(I_SCENE, p);
         const SceneGrid& g = grid[scene];
         const unsigned x = (unsigned)(S.k(I_CX, p) - g.lo[0]), y = (unsigned)(S.k(I_CY, p) - g.lo[1]),
                        z = (unsigned)(S.k(I_CZ, p) - g.lo[2]);
-        k = ((unsigned long long)(scene & 15) << 48) | (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+        k = ((unsigned long long)scene << 48) | (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
     }
     key[t] = k;
     val[t] = (int)t;
@@ -3534,10 +3536,12 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             if (verlet) {
                 // rows in Morton order of their cells; valA becomes sorted
                 // position -> new row
+                int sbits = 0;
+                while (sbits < 16 && (1 << sbits) < s->nscenes) ++sbits;
                 hipLaunchKernelGGL(k_morton_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->valA, s->dGrid,
-                                   s->mKey, s->mVal, s->sc);
+                                   s->mKey, s->mVal, s->sc, 48 + sbits);
                 TRY(radix_sort(st, s->mKey, s->mVal, s->keyB, s->valB, n, s->rsCounts, s->rsBlocks, &s->sc->mbits,
-                               s->rsDigits, 7));
+                               s->rsDigits, 8));
                 hipLaunchKernelGGL(k_morton_perm, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->valA, s->mVal,
                                    s->valB, n);
                 hipLaunchKernelGGL(k_morton_rank, dim3(blocks_for(n)), dim3(BLOCK), 0, st, s->mVal, s->valA, n);
