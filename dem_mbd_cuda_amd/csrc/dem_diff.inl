@@ -603,6 +603,7 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
                      double* plate_load)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    s->scalarsReset = false;  // the run's own kernels write the step counters
     if (steps < 1 || checkpoint_interval < 1) return set_error(DEM_E_INVALID, "steps and checkpoint interval must be positive");
     if (s->stepIndex != 0) return set_error(DEM_E_INVALID, "a differentiable run starts from a freshly initialized solver");
     if (s->nb > 0 || s->offs || s->universal || s->clump)
@@ -689,6 +690,7 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
 int dem_diff_backward(dem_solver* s, const double* d_height, const double* d_plate_load, double* d_mech)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    s->scalarsReset = false;
     if (!s->diff || !s->diff->ready) return set_error(DEM_E_INVALID, "dem_diff_backward needs a completed dem_diff_forward");
     if (!d_height || !d_plate_load || !d_mech) return set_error(DEM_E_INVALID, "null argument");
     DiffRun& R = *s->diff;

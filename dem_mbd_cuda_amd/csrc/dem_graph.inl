@@ -99,10 +99,12 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
                 // the captured steps reorder an even number of times: the buffer is unchanged
                 HIP_TRY(hipGraphLaunch(hit->exec, s->stream), "launch step graph");
                 s->stepIndex += P;
+                s->scalarsReset = false;  // the replayed evaluations wrote the counters
             } else {
                 GraphCache::Entry e{phase, s->cur, P, s->pairBlocks, s->reorderInterval, nullptr};
                 TRY(capture_steps(s, dt, P, &e.exec));
                 HIP_TRY(hipGraphLaunch(e.exec, s->stream), "launch step graph");
+                s->scalarsReset = false;
                 G.entries.push_back(e);
             }
             k += P;

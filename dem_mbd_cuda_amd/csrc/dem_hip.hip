@@ -316,7 +316,10 @@ __device__ __forceinline__ Body load_body(const Soa& S, long p, bool offs)
 }
 
 // ------------------------------------------------------------ step kernels
-__global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc, int nscenes)
+// The per-step counters and the per-scene cell box accumulators, reset by one
+// block before a force evaluation: by k_step_init, or at the end of the
+// advance that precedes it (which then saves a launch).
+__device__ __forceinline__ void reset_step_scalars(Scalars* sc, int* sceneAcc, int nscenes)
 {
     for (int i = threadIdx.x; i < 6 * nscenes; i += BLOCK) sceneAcc[i] = (i % 6) < 3 ? INT_MAX : INT_MIN;
     if (threadIdx.x == 0) {
@@ -337,6 +340,11 @@ __global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc,
             sc->tri_hi[a] = INT_MIN;
         }
     }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_step_init(Scalars* sc, int* sceneAcc, int nscenes)
+{
+    reset_step_scalars(sc, sceneAcc, nscenes);
 }
 
 // Owner state copied to each of its component rows (clumps), so that the
@@ -2180,8 +2188,10 @@ __global__ void __launch_bounds__(BLOCK) k_finish(Soa S, double dt)
 // advancePositionKernel of the next one (particle:213-265), in one pass over
 // the state: the reference's loop calls them back to back with only host work
 // (the body solve) in between.
-__global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, double dt, Recs R)
+__global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, double dt, Recs R, Scalars* sc,
+                                                          int* sceneAcc, int nscenes)
 {
+    if (sc && blockIdx.x == 0) reset_step_scalars(sc, sceneAcc, nscenes);
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (p >= S.n) return;
     double vhx = S.f(F_VHX, p) + S.f(F_AX, p) * dtf, vhy = S.f(F_VHY, p) + S.f(F_AY, p) * dtf,
@@ -2709,6 +2719,7 @@ struct dem_solver {
     hipEvent_t flagEv = nullptr;  // the scalars copy with the displacement flag has landed
     bool evReady = false;
     bool timing = false;  // record the stage events of dem_debug_last_timings
+    bool scalarsReset = false;  // the last advance reset the step counters (no k_step_init needed)
     // metadata for checkpoints
     std::vector<int> compIds, ownerIds;  // per component: id, owner id
     std::vector<int> pids;               // per owner: id
@@ -3440,7 +3451,9 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
     // stage events only when asked for: each adds a gap between kernels
     const bool timed = s->timing && !s->capturing;
     if (timed) HIP_TRY(hipEventRecord(s->ev[0], st), "event");
-    hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
+    if (!s->scalarsReset || s->capturing)
+        hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
+    s->scalarsReset = false;
     Soa S = s->soa();   // owners
     Soa R = s->rows();  // contact rows (the owners, or their components with clumps)
     const long nr = R.n;
@@ -3797,8 +3810,11 @@ int dem_advance_particles(dem_solver* s, double dt, double* vmax)
     const Recs R = (!s->offs && !s->clump) ? s->recs() : Recs{nullptr, nullptr};
     if (s->pendingFinish) {
         s->pendingFinish = false;
+        // the next force evaluation's counters are reset here (the host has
+        // taken this step's already)
         hipLaunchKernelGGL(k_finish_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(),
-                           s->pendingDt, dt, R);
+                           s->pendingDt, dt, R, s->sc, s->sceneAcc, s->nscenes);
+        s->scalarsReset = true;
     } else {
         hipLaunchKernelGGL(k_advance, dim3(blocks_for(s->n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, R);
     }
