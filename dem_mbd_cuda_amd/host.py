@@ -131,8 +131,8 @@ def load_prescribed_motion(path, start_step, end_step):
             if not tokens:
                 continue
             m = re.match(r"[+-]?\d+", tokens[0])
-            if not m:
-                continue
+            if not m or not -2 ** 31 <= int(m.group(0)) < 2 ** 31:
+                continue  # not an int: the reference's `row >> key.step` fails and skips the row
             rest = tokens[0][m.end():]
             tokens = ([rest] if rest else []) + tokens[1:]
             try:
@@ -150,6 +150,28 @@ def load_prescribed_motion(path, start_step, end_step):
     if not keys or keys[0][0] != start_step:
         raise RuntimeError("prescribed motion must begin at ODE_StartStep")
     return keys
+
+
+def check_prescribed_bodies(scene):
+    """The one-prescribed-body rule of applyPrescribedMotion (reference
+    cuda_simulation_integrator.cpp:229-232), for a scene about to run a
+    schedule."""
+    if len(scene.body_mass) != 1 or int(scene.body_state[0]) != BODY_PRESCRIBED:
+        raise RuntimeError("prescribed motion schedule currently requires one prescribed body")
+
+
+def _apply_prescribed(keys, cursor, step, B):
+    """applyPrescribedMotion (reference cuda_simulation_integrator.cpp:223-243)
+    without the half-step part: advances the cursor to the last key at or
+    before `step` and sets the one prescribed body's velocities from it.
+    Returns the cursor."""
+    while cursor + 1 < len(keys) and keys[cursor + 1][0] <= step:
+        cursor += 1
+    if len(B["mass"]) != 1 or B["state"][0] != BODY_PRESCRIBED:
+        raise RuntimeError("prescribed motion schedule currently requires one prescribed body")
+    B["vel"][0] = keys[cursor][1]
+    B["angvel"][0] = keys[cursor][2]
+    return cursor
 
 
 def body_state_from_scene(scene):
@@ -178,6 +200,10 @@ def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, 
     B = body_state_from_scene(scene)
     nb = len(B["mass"])
     impulses = np.zeros((nb, 6))
+    cursor = 0
+    if prescribed:
+        # the schedule's first key sets the body before the first evaluation (:292-294)
+        cursor = _apply_prescribed(prescribed, cursor, start_step, B)
     loads = solver.compute_forces(_pose13(B), loads=True, collect=True)
     if on_frame:
         on_frame(-1, 0.0, dict(bodies={k: v.copy() for k, v in B.items()}, loads=loads.copy(),
@@ -220,16 +246,10 @@ def run_coupled(solver, scene, steps=None, output_interval=None, on_frame=None, 
 
     half_update(True)
     counter = 0
-    cursor = 0
     advanced = False  # the step's particle advance was launched at the end of the previous one
     for step in range(start_step, end):
         if prescribed:
-            while cursor + 1 < len(prescribed) and prescribed[cursor + 1][0] <= step:
-                cursor += 1
-            if nb != 1 or B["state"][0] != BODY_PRESCRIBED:
-                raise RuntimeError("prescribed motion schedule currently requires one prescribed body")
-            B["vel"][0] = prescribed[cursor][1]
-            B["angvel"][0] = prescribed[cursor][2]
+            cursor = _apply_prescribed(prescribed, cursor, step, B)
             hv[0] = B["vel"][0]
             hw[0] = B["angvel"][0]
             hq[0] = qnormalize(B["orien"][0] + gama(B["orien"][0], B["angvel"][0]) * half)

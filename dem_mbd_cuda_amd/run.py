@@ -12,7 +12,6 @@ CONTROL_OUTPUT_* under <case>/OutputFile/ and saves
 from __future__ import annotations
 
 import os
-import re
 import sys
 
 from . import host, library, output, scene
@@ -23,6 +22,15 @@ def run_case(case_dir, lib=None, log=print):
     sc = scene.read_case(case_dir)
     flag = lambda key: int(par.get(key, "1")) != 0
     start = int(float(par.get("ODE_StartStep", "0")))
+    keys = []
+    if len(sc.body_mass):
+        # read before the solver starts, as the reference does; the path is
+        # opened as written, relative to the working directory
+        # (cuda_simulation_integrator.cpp:182-185, 289-294)
+        keys = host.load_prescribed_motion(par.get("CONTROL_PRESCRIBED_MOTION_FILE", ""), start,
+                                           start + sc.steps)
+        if keys:
+            host.check_prescribed_bodies(sc)
     solver = (lib or library()).solver()
     solver.initialize(sc)
     out = output.CaseOutput(case_dir, sc, particle_state=flag("CONTROL_OUTPUT_PARTICLE_STATE"),
@@ -35,11 +43,6 @@ def run_case(case_dir, lib=None, log=print):
     dt = sc.dt
     try:
         if len(sc.body_mass):
-            path = par.get("CONTROL_PRESCRIBED_MOTION_FILE", "")
-            if path:
-                path = os.path.join(case_dir, re.sub(r"^Data/DATA/", "", path.replace("\\", "/")))
-            keys = host.load_prescribed_motion(path, start, start + sc.steps)
-
             def frame(step, t, extra):
                 out.frame(solver, step, t, extra)
                 if step < 0:

@@ -137,3 +137,31 @@ def test_clump_case_reference_host_writes_the_same_files(hip, tmp_path):
     assert sorted(tm) == sorted(tt)
     for rel in tt:
         assert tm[rel] == tt[rel], rel
+
+
+def test_prescribed_motion_case(hip, oracle, tmp_path):
+    """The triangle validation case with its body prescribed and a schedule
+    of three keys (CONTROL_PRESCRIBED_MOTION_FILE): every file the case
+    runner writes on the HIP backend equals the oracle's and, where built,
+    the reference's own host's through the same backend
+    (cuda_simulation_integrator.cpp:172-243, 288-294, 372-373)."""
+    from test_prescribed_motion import prescribed_case
+
+    text = "0 0.5 0 0 0 0 1.5\n7 0 -0.25 0 0 0 0\n13 0.1 0.1 0 0 0 -2\n"
+    g, _ = prescribed_case(tmp_path, "hip", text)
+    o, _ = prescribed_case(tmp_path, "ora", text)
+    run.run_case(g, lib=hip, log=lambda *a: None)
+    run.run_case(o, lib=oracle, log=lambda *a: None)
+    tg, to = tree(g), tree(o)
+    assert sorted(tg) == sorted(to) and os.path.join("state_bodys", "At.00000000.bt") in tg
+    for rel in to:
+        assert tg[rel] == to[rel], rel
+    if os.path.exists(REF_HOST):
+        r, _ = prescribed_case(tmp_path, "ref", text)
+        res = subprocess.run([REF_HOST, r], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                             timeout=300)
+        assert res.returncode == 0, res.stdout[-3000:]
+        tr = tree(r)
+        assert sorted(tr) == sorted(tg)
+        for rel in tr:
+            assert tg[rel] == tr[rel], rel
