@@ -121,6 +121,8 @@ class Library:
             bind("initialize_batch", [vp, C.POINTER(ParticlesView), C.POINTER(ComponentsView),
                                       C.POINTER(WallsView), C.POINTER(BatchView), C.POINTER(Control)])
         bind("compute_forces", [vp, _dp, _dp, C.c_int])
+        if hasattr(L, p + "compute_forces_noalloc"):
+            bind("compute_forces_noalloc", [vp, _dp, _dp, C.c_int])
         bind("init_half_step", [vp, C.c_double])
         bind("advance_particles", [vp, C.c_double, _dp])
         bind("finish_particle_step", [vp, C.c_double, C.c_double])
@@ -182,6 +184,8 @@ class Solver:
         self.nc = 0
         self.nb = 0
         self._keep = []
+        # compute_forces goes through dem_compute_forces_noalloc when set
+        self.noalloc = False
 
     def close(self):
         if self.h:
@@ -256,8 +260,8 @@ class Solver:
     def compute_forces(self, body_pose=None, loads=False, collect=False):
         pose = None if body_pose is None else _f64(body_pose, (self.nb, 13))
         out = np.zeros((self.nb, 6)) if loads else None
-        self.lib.call("compute_forces", self.h, _ptr(pose, C.c_double), _ptr(out, C.c_double),
-                      1 if collect else 0)
+        self.lib.call("compute_forces_noalloc" if self.noalloc else "compute_forces", self.h,
+                      _ptr(pose, C.c_double), _ptr(out, C.c_double), 1 if collect else 0)
         return out
 
     def init_half_step(self, half_dt):

@@ -45,6 +45,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -69,6 +70,39 @@ int set_error(int code, const char* fmt, ...)
         hipError_t e_ = (expr);                                                                   \
         if (e_ != hipSuccess) return set_error(DEM_E_DEVICE, "%s: %s", what, hipGetErrorString(e_)); \
     } while (0)
+
+}  // namespace
+
+// Scratch memory of dem_compute_forces_noalloc: the per-step triangle-grid
+// buffers of a call that may not allocate.  Every device the library runs on
+// holds one copy for the life of the process; host code reaches it only
+// through the address scratch_on_current_device returns for that device.
+static __device__ __align__(256) unsigned char scratch[DEM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchLock;
+void* g_scratchPtr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int scratch_on_current_device(void** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return set_error(DEM_E_DEVICE, "device %d is beyond the no-allocation scratch table", dev);
+    std::lock_guard<std::mutex> guard(g_scratchLock);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) return set_error(DEM_E_DEVICE, "hipGetSymbolAddress(scratch): null address");
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return DEM_OK;
+}
 
 constexpr int PP_SLOTS = DEM_PARTICLE_HISTORY_SLOTS;
 constexpr int PW_SLOTS = DEM_WALL_HISTORY_SLOTS;
@@ -2567,6 +2601,23 @@ void graph_release(GraphCache* g);
 // owner sums (triangles, walls, field, acceleration) | body reductions
 constexpr int NEV = 6;
 
+// Per-step buffers of the triangle grid: sorted (key, triangle) pairs and
+// their radix-sort counts, or the dense cell table with its fill cursors and
+// scan partials.
+struct TriBufs {
+    unsigned long long* keyA = nullptr;
+    unsigned long long* keyB = nullptr;
+    int* triA = nullptr;
+    int* triB = nullptr;
+    unsigned* counts = nullptr;  // radix-sort digit counts [256][countBlocks]
+    int countBlocks = 0;
+    long long cap = 0;           // entries keyA..triB hold
+    unsigned* start = nullptr;   // dense cell -> first entry table of the grid's box
+    unsigned* cur = nullptr;     // per-cell counts, then fill cursors (startCap entries)
+    unsigned* scan = nullptr;    // tile totals of the cell-table scan
+    long long startCap = 0;      // entries start and cur hold
+};
+
 struct dem_solver {
     bool initialized = false;
     int device = 0;
@@ -2678,21 +2729,16 @@ struct dem_solver {
     std::vector<double> hBodyMass;  // host copy (fixed at initialisation)
     int* tCounts = nullptr;
     long long* tOffsets = nullptr;
-    long long tCap = 0;
-    unsigned long long* tKeyA = nullptr;
-    unsigned long long* tKeyB = nullptr;
-    int* tTriA = nullptr;
-    int* tTriB = nullptr;
-    unsigned* tsCounts = nullptr;
-    int tsBlocks = 0;
+    // the triangle grid's per-step buffers: the solver's own, grown on demand
+    // (tOwn), and the set this step uses (tUse): tOwn, or one carved out of the
+    // device's static scratch array by dem_compute_forces_noalloc
+    TriBufs tOwn{};
+    TriBufs tUse{};
+    bool tInScratch = false;       // tUse points into the scratch array
     int tEntries = 0;
     int tLo[3] = {0, 0, 0};        // cell box of the triangle grid entries
     long long tDims[3] = {0, 0, 0};
-    unsigned* tStart = nullptr;    // dense cell -> first entry table of that box
-    unsigned* tCur = nullptr;      // per-cell counts, then fill cursors (same capacity)
-    unsigned* tScan = nullptr;     // tile totals of the cell-table scan
     long long tStartCap = 0;       // entries of the table valid this step (0: binary search)
-    long long tStartCapAlloc = 0;
     int* tBits = nullptr;          // device: key bits of the triangle sort
     int* hTri = nullptr;           // pinned: [nt] counts, then the sort bits
     long long* hTriOff = nullptr;  // pinned: [nt] entry offsets
@@ -2741,9 +2787,9 @@ struct dem_solver {
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
                         H2.pwVal, H2.ptOcc, H2.ptSeen, H2.ptKey, H2.ptVal, wallBuf, O.contact, O.wall, O.body, O.field, O.bodyRows, O.bodyTouched,
                         O.bodyParticle, O.bodyDetail, O.regime, tBody, triWork, triHitMask, triHitN, triHitVal, tLocal, tWorld, tAabb,
-                        pose, bodyMass, tCounts, tOffsets, tKeyA, tKeyB, tTriA, tTriB, tsCounts, d64,
+                        pose, bodyMass, tCounts, tOffsets, tOwn.keyA, tOwn.keyB, tOwn.triA, tOwn.triB, tOwn.counts, d64,
                         regimeInstant, regimeInterval, microInterval, bodyResult,
-                        redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tStart, tCur, tScan, tBits};
+                        redPartial, dl, cState, ciState, cOwner, ownerStart, cRank, ownS, tOwn.start, tOwn.cur, tOwn.scan, tBits};
         for (void* p : ptrs)
             if (p) hipFree(p);
         if (hPose) hipHostFree(hPose);
@@ -3289,12 +3335,12 @@ TriSet triset(const dem_solver* s)
 {
     TriSet T{};
     T.nt = s->nt; T.nb = s->nb; T.body = s->tBody; T.local = s->tLocal; T.world = s->tWorld;
-    T.aabb = s->tAabb; T.pose = s->pose; T.entries = s->tEntries; T.tkey = s->tKeyA; T.ttri = s->tTriA;
+    T.aabb = s->tAabb; T.pose = s->pose; T.entries = s->tEntries; T.tkey = s->tUse.keyA; T.ttri = s->tUse.triA;
     for (int a = 0; a < 3; ++a) {
         T.lo[a] = s->tLo[a];
         T.dims[a] = s->tDims[a];
     }
-    T.start = s->tStartCap > 0 ? s->tStart : nullptr;
+    T.start = s->tStartCap > 0 ? s->tUse.start : nullptr;
     return T;
 }
 
@@ -3309,9 +3355,94 @@ int tri_grid_count(dem_solver* s)
     return launch_check("countTriangleCells");
 }
 
+// The solver's own triangle-grid buffers, grown to `total` entries and, for
+// the dense-table path (cells > 0), a table of cells + 1 entries.
+int tri_bufs_own(dem_solver* s, long long total, long long cells)
+{
+    TriBufs& b = s->tOwn;
+    if (total > b.cap) {
+        for (void* p : {(void*)b.keyA, (void*)b.keyB, (void*)b.triA, (void*)b.triB, (void*)b.counts})
+            if (p) hipFree(p);
+        b.keyA = b.keyB = nullptr;
+        b.triA = b.triB = nullptr;
+        b.counts = nullptr;
+        b.cap = 0;
+        const long long cap = total + total / 4 + 64;
+        TRY(dalloc(b.keyA, cap, "hipMalloc(dynamic triangle grid keys)"));
+        TRY(dalloc(b.keyB, cap, "hipMalloc(dynamic triangle grid keys)"));
+        TRY(dalloc(b.triA, cap, "hipMalloc(dynamic triangle grid ids)"));
+        TRY(dalloc(b.triB, cap, "hipMalloc(dynamic triangle grid ids)"));
+        b.countBlocks = (int)((cap + RS_TILE - 1) / RS_TILE);
+        TRY(dalloc(b.counts, (size_t)256 * b.countBlocks, "hipMalloc(triangle sort counts)"));
+        b.cap = cap;
+    }
+    if (cells > 0 && cells + 1 > b.startCap) {
+        for (void* p : {(void*)b.start, (void*)b.cur, (void*)b.scan})
+            if (p) hipFree(p);
+        b.start = b.cur = b.scan = nullptr;
+        b.startCap = 0;
+        const long long cap = cells + 1 + cells / 4;
+        TRY(dalloc(b.start, (size_t)cap, "hipMalloc(triangle cell table)"));
+        TRY(dalloc(b.cur, (size_t)cap, "hipMalloc(triangle cell table)"));
+        TRY(dalloc(b.scan, (size_t)((cap + SCAN_TILE - 1) / SCAN_TILE), "hipMalloc(triangle cell table)"));
+        b.startCap = cap;
+    }
+    s->tUse = b;
+    s->tInScratch = false;
+    return DEM_OK;
+}
+
+// The same buffers carved out of the device's scratch array (no allocation):
+// the dense-table path needs the two triangle-id arrays, the table, its
+// cursors (one slot more, for the count of large cells) and the scan
+// partials; the sorted-pairs path the two key and two id arrays and the
+// radix counts.  Steps whose grid does not fit fail with DEM_E_CAPACITY.
+int tri_bufs_scratch(dem_solver* s, long long total, long long cells)
+{
+    void* base = nullptr;
+    TRY(scratch_on_current_device(&base));
+    auto al = [](unsigned long long bytes) { return (bytes + 255ull) & ~255ull; };
+    const unsigned long long e = (unsigned long long)total;
+    const long long blocks = (total + RS_TILE - 1) / RS_TILE;
+    const long long tiles = (cells + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    const unsigned long long need =
+        2 * al(4 * e) + (cells > 0 ? 2 * al(4ull * (unsigned long long)(cells + 2)) + al(4ull * (unsigned long long)tiles)
+                                   : 2 * al(8 * e) + al(4ull * 256ull * (unsigned long long)blocks));
+    if (need > (unsigned long long)DEM_NOALLOC_SCRATCH_BYTES)
+        return set_error(DEM_E_CAPACITY,
+                         "no-allocation scratch too small for this step's triangle grid: %llu entries over %lld "
+                         "table cells need %llu bytes, the limit is %llu bytes (DEM_NOALLOC_SCRATCH_BYTES)",
+                         e, cells, need, (unsigned long long)DEM_NOALLOC_SCRATCH_BYTES);
+    unsigned char* p = (unsigned char*)base;
+    auto take = [&](unsigned long long bytes) {
+        void* q = p;
+        p += al(bytes);
+        return q;
+    };
+    TriBufs b{};
+    b.triA = (int*)take(4 * e);
+    b.triB = (int*)take(4 * e);
+    b.cap = total;
+    if (cells > 0) {
+        b.start = (unsigned*)take(4ull * (unsigned long long)(cells + 2));
+        b.cur = (unsigned*)take(4ull * (unsigned long long)(cells + 2));
+        b.scan = (unsigned*)take(4ull * (unsigned long long)tiles);
+        b.startCap = cells + 2;
+    } else {
+        b.keyA = (unsigned long long*)take(8 * e);
+        b.keyB = (unsigned long long*)take(8 * e);
+        b.counts = (unsigned*)take(4ull * 256ull * (unsigned long long)blocks);
+        b.countBlocks = (int)blocks;
+    }
+    s->tUse = b;
+    s->tInScratch = true;
+    return DEM_OK;
+}
+
 // The triangle grid of this step.  counted: tri_grid_count ran and the host
 // scalars already hold its box and total (read back with another value).
-int build_triangle_grid(dem_solver* s, bool counted = false)
+// noalloc: the per-step buffers come from the device's scratch array.
+int build_triangle_grid(dem_solver* s, bool counted = false, bool noalloc = false)
 {
     hipStream_t st = s->stream;
     if (!counted) {
@@ -3323,20 +3454,7 @@ int build_triangle_grid(dem_solver* s, bool counted = false)
     const unsigned long long total = s->hScalars->tri_total;
     if (s->hScalars->tri_overflow != 0 || total > (unsigned long long)INT_MAX)
         return set_error(DEM_E_CAPACITY, "triangle grid entry count exceeds the supported 32-bit index range");
-    if ((long long)total > s->tCap) {
-        for (void* p : {(void*)s->tKeyA, (void*)s->tKeyB, (void*)s->tTriA, (void*)s->tTriB, (void*)s->tsCounts})
-            if (p) hipFree(p);
-        const long long cap = (long long)total + (long long)total / 4 + 64;
-        TRY(dalloc(s->tKeyA, cap, "hipMalloc(dynamic triangle grid keys)"));
-        TRY(dalloc(s->tKeyB, cap, "hipMalloc(dynamic triangle grid keys)"));
-        TRY(dalloc(s->tTriA, cap, "hipMalloc(dynamic triangle grid ids)"));
-        TRY(dalloc(s->tTriB, cap, "hipMalloc(dynamic triangle grid ids)"));
-        s->tsBlocks = (int)((cap + RS_TILE - 1) / RS_TILE);
-        TRY(dalloc(s->tsCounts, (size_t)256 * s->tsBlocks, "hipMalloc(triangle sort counts)"));
-        s->tCap = cap;
-    }
     if (total == 0) return DEM_OK;
-    s->tEntries = (int)total;
     // cell box of all entries: compact x-major keys of ceil(log2 cells) bits
     unsigned long long cells = 1;
     for (int a = 0; a < 3; ++a) {
@@ -3345,40 +3463,34 @@ int build_triangle_grid(dem_solver* s, bool counted = false)
         cells *= (unsigned long long)s->tDims[a];  // each dim < 2^21: no overflow
     }
     constexpr unsigned long long kMaxTableCells = 1ull << 24;
-    if (cells <= kMaxTableCells) {
+    const long long tableCells = cells <= kMaxTableCells ? (long long)cells : 0;
+    TRY(noalloc ? tri_bufs_scratch(s, (long long)total, tableCells) : tri_bufs_own(s, (long long)total, tableCells));
+    s->tEntries = (int)total;
+    if (tableCells > 0) {
         // counting sort over the box; its scan is the dense cell table
-        if ((long long)cells + 1 > s->tStartCapAlloc) {
-            for (void* p : {(void*)s->tStart, (void*)s->tCur, (void*)s->tScan})
-                if (p) hipFree(p);
-            s->tStart = s->tCur = s->tScan = nullptr;
-            s->tStartCapAlloc = (long long)cells + 1 + (long long)cells / 4;
-            TRY(dalloc(s->tStart, (size_t)s->tStartCapAlloc, "hipMalloc(triangle cell table)"));
-            TRY(dalloc(s->tCur, (size_t)s->tStartCapAlloc, "hipMalloc(triangle cell table)"));
-            TRY(dalloc(s->tScan, (size_t)((s->tStartCapAlloc + SCAN_TILE - 1) / SCAN_TILE), "hipMalloc(triangle cell table)"));
-        }
         const long tableN = (long)cells + 1;
         const int fillBlocks = (int)std::min<long>((tableN + BLOCK - 1) / BLOCK, 4096L);
         const unsigned triBlocks = (unsigned)((s->nt + BLOCK / 64 - 1) / (BLOCK / 64));
         T = triset(s);
-        hipLaunchKernelGGL(k_fill_u32, dim3(fillBlocks), dim3(BLOCK), 0, st, s->tCur, tableN);
+        hipLaunchKernelGGL(k_fill_u32, dim3(fillBlocks), dim3(BLOCK), 0, st, s->tUse.cur, tableN);
         hipLaunchKernelGGL(k_tri_cells<true>, dim3(triBlocks), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
-                           (const int*)s->tCounts, s->tCur, (int*)nullptr);
+                           (const int*)s->tCounts, s->tUse.cur, (int*)nullptr);
         const long tiles = (tableN + SCAN_TILE - 1) / SCAN_TILE;
-        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(BLOCK), 0, st, (const unsigned*)s->tCur, tableN,
-                           s->tStart, s->tScan);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, st, s->tScan, tiles);
-        hipLaunchKernelGGL(k_scan_add, dim3(fillBlocks), dim3(BLOCK), 0, st, s->tStart, tableN,
-                           (const unsigned*)s->tScan, s->tCur);
+        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(BLOCK), 0, st, (const unsigned*)s->tUse.cur, tableN,
+                           s->tUse.start, s->tUse.scan);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BLOCK), 0, st, s->tUse.scan, tiles);
+        hipLaunchKernelGGL(k_scan_add, dim3(fillBlocks), dim3(BLOCK), 0, st, s->tUse.start, tableN,
+                           (const unsigned*)s->tUse.scan, s->tUse.cur);
         hipLaunchKernelGGL(k_tri_cells<false>, dim3(triBlocks), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
-                           (const int*)s->tCounts, s->tCur, s->tTriA);
+                           (const int*)s->tCounts, s->tUse.cur, s->tUse.triA);
         // the cursors are spent: tCur holds the list of large cells, its
         // count sits in the last slot (cells <= 2^24 leaves it unused)
-        int* bigCount = (int*)s->tCur + s->tStartCapAlloc - 1;
+        int* bigCount = (int*)s->tUse.cur + s->tUse.startCap - 1;
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (unsigned*)bigCount, 1L);
         hipLaunchKernelGGL(k_tri_cell_order, dim3((unsigned)((cells + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st,
-                           (const unsigned*)s->tStart, (long long)cells, s->tTriA, bigCount, (int*)s->tCur);
-        hipLaunchKernelGGL(k_tri_cell_order_big, dim3(1024), dim3(64), 0, st, (const unsigned*)s->tStart,
-                           (const int*)bigCount, (const int*)s->tCur, s->tTriA, s->tTriB);
+                           (const unsigned*)s->tUse.start, (long long)cells, s->tUse.triA, bigCount, (int*)s->tUse.cur);
+        hipLaunchKernelGGL(k_tri_cell_order_big, dim3(1024), dim3(64), 0, st, (const unsigned*)s->tUse.start,
+                           (const int*)bigCount, (const int*)s->tUse.cur, s->tUse.triA, s->tUse.triB);
         TRY(launch_check("fillTriangleCells"));
         s->tStartCap = tableN;
         return DEM_OK;
@@ -3400,10 +3512,10 @@ int build_triangle_grid(dem_solver* s, bool counted = false)
             "upload triangle offsets");
     T = triset(s);
     hipLaunchKernelGGL(k_tri_fill, dim3(blocks_for(s->nt)), dim3(BLOCK), 0, st, T, s->inv, s->maxRadius,
-                       s->tCounts, s->tOffsets, s->tKeyA, s->tTriA);
+                       s->tCounts, s->tOffsets, s->tUse.keyA, s->tUse.triA);
     TRY(launch_check("fillTriangleCells"));
     const int nblk = (int)((total + RS_TILE - 1) / RS_TILE);
-    TRY(radix_sort(st, s->tKeyA, s->tTriA, s->tKeyB, s->tTriB, (long)total, s->tsCounts, nblk, s->tBits,
+    TRY(radix_sort(st, s->tUse.keyA, s->tUse.triA, s->tUse.keyB, s->tUse.triB, (long)total, s->tUse.counts, nblk, s->tBits,
                    s->rsDigits, (bits + 7) / 8));
     return DEM_OK;
 }
@@ -3427,7 +3539,9 @@ static int await_mirror(dem_solver* s, int seq)
     return DEM_OK;
 }
 
-int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int collect)
+// dem_compute_forces; noalloc: the triangle grid's per-step buffers come from
+// the device's scratch array instead of the solver's own (grown on demand).
+static int compute_forces_impl(dem_solver* s, const double* pose13, double* loads6, int collect, bool noalloc)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
     TRY(flush_finish(s));
@@ -3531,7 +3645,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
                         "copy scalars");
                 HIP_TRY(hipStreamSynchronize(st), "sort key bits");
             }
-            TRY(build_triangle_grid(s, triCounted));
+            TRY(build_triangle_grid(s, triCounted, noalloc));
             triDone = true;
             passes = (s->hScalars->nbits + 7) / 8;
         }
@@ -3606,7 +3720,7 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
             s->vlistValid = true;
         }
     }
-    if (s->nt > 0 && !triDone) TRY(build_triangle_grid(s, triCounted));
+    if (s->nt > 0 && !triDone) TRY(build_triangle_grid(s, triCounted, noalloc));
     if (timed && !scanned) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
     const int step = s->stepIndex++;
     const int nblk = blocks_for(nr);
@@ -3718,10 +3832,10 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         stt.body_history_high_water = h.body_high;
         stt.triangle_grid_overflows = h.tri_overflow;
         if (nb > 0) {
-            std::vector<double> fz(nb);
-            HIP_TRY(hipMemcpy(fz.data(), s->bodyResult + 6 * nb, sizeof(double) * nb, hipMemcpyDeviceToHost), "download fz");
+            double* fz = s->hLoads + 6 * nb;  // pinned, sized at initialisation
+            HIP_TRY(hipMemcpy(fz, s->bodyResult + 6 * nb, sizeof(double) * nb, hipMemcpyDeviceToHost), "download fz");
             double sum = 0.0;
-            for (double v : fz) sum += v;
+            for (int b = 0; b < nb; ++b) sum += fz[b];
             stt.body_contact_fz = sum;
             HIP_TRY(hipMemcpy(s->cRegInst.data(), s->regimeInstant, sizeof(uint64_t) * 7 * nb, hipMemcpyDeviceToHost), "download regime");
             HIP_TRY(hipMemcpy(s->cRegInt.data(), s->regimeInterval, sizeof(uint64_t) * 7 * nb, hipMemcpyDeviceToHost), "download regime");
@@ -3756,6 +3870,19 @@ int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int 
         }
     }
     return DEM_OK;
+}
+
+int dem_compute_forces(dem_solver* s, const double* pose13, double* loads6, int collect)
+{
+    return compute_forces_impl(s, pose13, loads6, collect, false);
+}
+
+int dem_compute_forces_noalloc(dem_solver* s, const double* pose13, double* loads6, int collect)
+{
+    const int rc = compute_forces_impl(s, pose13, loads6, collect, true);
+    // the step's kernels are done with the scratch array when the call returns
+    if (rc == DEM_OK && s->tInScratch) HIP_TRY(hipStreamSynchronize(s->stream), "computeForces (no allocation)");
+    return rc;
 }
 
 namespace {
@@ -4043,12 +4170,12 @@ int dem_debug_triangle_grid(const dem_solver* s, uint64_t* keys, int32_t* tris)
         if (s->tStartCap > 0) {
             // counting-sort layout: cell c holds entries [start[c], start[c + 1])
             std::vector<unsigned> start((size_t)s->tStartCap);
-            HIP_TRY(hipMemcpy(start.data(), s->tStart, sizeof(unsigned) * start.size(), hipMemcpyDeviceToHost),
+            HIP_TRY(hipMemcpy(start.data(), s->tUse.start, sizeof(unsigned) * start.size(), hipMemcpyDeviceToHost),
                     "download");
             for (size_t c = 0; c + 1 < start.size(); ++c)
                 for (unsigned k = start[c]; k < start[c + 1]; ++k) keys[k] = c;
         } else {
-            HIP_TRY(hipMemcpy(keys, s->tKeyA, sizeof(uint64_t) * s->tEntries, hipMemcpyDeviceToHost), "download");
+            HIP_TRY(hipMemcpy(keys, s->tUse.keyA, sizeof(uint64_t) * s->tEntries, hipMemcpyDeviceToHost), "download");
         }
         const unsigned long long Y = (unsigned long long)s->tDims[1], Z = (unsigned long long)s->tDims[2];
         const unsigned long long mask = (1ull << 21) - 1ull;
@@ -4060,7 +4187,7 @@ int dem_debug_triangle_grid(const dem_solver* s, uint64_t* keys, int32_t* tris)
                       (((unsigned long long)(y + (1 << 20)) & mask) << 21) | ((unsigned long long)(z + (1 << 20)) & mask);
         }
     }
-    if (tris) HIP_TRY(hipMemcpy(tris, s->tTriA, sizeof(int) * s->tEntries, hipMemcpyDeviceToHost), "download");
+    if (tris) HIP_TRY(hipMemcpy(tris, s->tUse.triA, sizeof(int) * s->tEntries, hipMemcpyDeviceToHost), "download");
     return DEM_OK;
 }
 

@@ -176,6 +176,25 @@ int dem_initialize_batch(dem_solver* solver, const dem_particles_view* particles
 int dem_compute_forces(dem_solver* solver, const double* body_pose13,
                        double* body_loads6, int collect);
 
+/* No reference counterpart.  dem_compute_forces for callers that cannot
+ * allocate device memory (serving loops and harnesses that forbid hipMalloc,
+ * code that runs between graph captures).  Same arguments and results, bit
+ * for bit.  The only device memory dem_compute_forces allocates after
+ * initialisation is the triangle grid's per-step buffers (sorted entries,
+ * cell table, sort counts), grown with hipMalloc/hipFree when a step needs
+ * more; here they come from one static device array of
+ * DEM_NOALLOC_SCRATCH_BYTES that every device the library runs on holds for
+ * the life of the process, and the call makes no hipMalloc or hipFree.  A step
+ * whose triangle grid needs more than the array fails with DEM_E_CAPACITY
+ * before any kernel touches it.  Solvers without triangles use no scratch.
+ * The call returns after its kernels are done with the array.  All calls on
+ * one device share its array: calls that overlap in time (solvers driven from
+ * several host threads) race, and dem_debug_triangle_grid after such a call
+ * reads the array as the device's last no-allocation call left it. */
+#define DEM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int dem_compute_forces_noalloc(dem_solver* solver, const double* body_pose13,
+                               double* body_loads6, int collect);
+
 /* CudaDemSolver::initializeParticleHalfStep (cuda_backend.h:56). */
 int dem_init_half_step(dem_solver* solver, double half_dt);
 /* CudaDemSolver::advanceParticles (cuda_backend.h:57); *vmax_out gets 0.0
