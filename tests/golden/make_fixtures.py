@@ -10,9 +10,6 @@ Outputs (committed):
   data1_mbd.npz       DATA1 inputs (validation/ctest_data1_mbd), 1,704
                       spheres, 5 walls, 2 bodies x 1,416 triangles
   data1_pure.npz      the same without bodies (validation/ctest_data1_pure_dem)
-  data1_ref_cpu.npz   ph/F/At values written by the reference's legacy CPU
-                      binary (oracle/_ref/dem_mbd_cpu) after 5 steps on both
-                      DATA1 cases (particle state + force files, body state)
   data1_zero_vel_ref_cpu.npz  the initial (frame 0) outputs of the same binary
                       on DATA1 with all particle velocities set to zero
   ref_output_text.npz the text of the reference binary's frame files on DATA1
@@ -22,6 +19,9 @@ Outputs (committed):
                       cases (validation/cases), as the reference holds them
   ref_case_outputs.npz every OutputFile/ file the reference binary writes for
                       those cases (text), keyed "<case>/<path under OutputFile>"
+  law_pins_ref_cpu.npz the numbers in every frame file the reference binary
+                      writes for the contact-law and integrator cases of
+                      law_cases.py, keyed "<case>/<path under OutputFile>"
 """
 from __future__ import annotations
 
@@ -91,13 +91,6 @@ def main():
     for tag, d in (("mbd", "ctest_data1_mbd"), ("pure", "ctest_data1_pure_dem")):
         sc = S.read_case(os.path.join(REF, "validation", d))
         sc.save(os.path.join(HERE, f"data1_{tag}.npz"))
-    ref = {}
-    with tempfile.TemporaryDirectory() as work:
-        for tag, d in (("mbd", "ctest_data1_mbd"), ("pure", "ctest_data1_pure_dem")):
-            out = run_reference_cpu(os.path.join(REF, "validation", d), work)
-            for k, v in read_outputs(out).items():
-                ref[f"{tag}__{k}"] = v
-    np.savez_compressed(os.path.join(HERE, "data1_ref_cpu.npz"), **ref)
     zref = {}
     with tempfile.TemporaryDirectory() as work:
         for tag, d in (("mbd", "ctest_data1_mbd"), ("pure", "ctest_data1_pure_dem")):
@@ -135,8 +128,30 @@ def main():
                     rel = os.path.relpath(os.path.join(root, f), base)
                     outs[f"{name}/{rel}"] = np.array(open(os.path.join(root, f)).read())
     np.savez_compressed(os.path.join(HERE, "ref_case_outputs.npz"), **outs)
+    law_pin_fixtures()
     print("fixtures written to", HERE)
 
 
+def law_pin_fixtures():
+    import law_cases
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "dem_mbd_cpu")
+    outs = {}
+    with tempfile.TemporaryDirectory() as work:
+        for name in law_cases.CASES:
+            d = law_cases.write_case(name, os.path.join(work, name))
+            res = subprocess.run([exe, d], cwd=work, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                 text=True, timeout=600)
+            if res.returncode != 0:
+                raise RuntimeError(res.stdout[-4000:])
+            base = os.path.join(d, "OutputFile")
+            for f in sorted(glob.glob(os.path.join(base, "*", "*.bt"))):
+                outs[f"{name}/{os.path.relpath(f, base)}"] = np.loadtxt(f, ndmin=2)
+    np.savez_compressed(os.path.join(HERE, "law_pins_ref_cpu.npz"), **outs)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["law_pins"]:
+        law_pin_fixtures()
+    else:
+        main()
