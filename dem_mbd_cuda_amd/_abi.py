@@ -155,6 +155,10 @@ class Library:
             bind("diff_backward", [vp, _dp, _dp, _dp])
         if hasattr(L, p + "debug_last_timings"):
             bind("debug_last_timings", [vp, _dp, C.c_int, C.POINTER(C.c_int)])
+        if hasattr(L, p + "set_threads"):
+            bind("set_threads", [vp, C.c_int])
+        if hasattr(L, p + "debug_run_counters"):
+            bind("debug_run_counters", [vp, C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)])
         if hasattr(L, p + "debug_enable_timings"):
             bind("debug_enable_timings", [vp, C.c_int])
 
@@ -324,11 +328,32 @@ class Solver:
         self.lib.call("debug_grid", self.h, _ptr(keys, C.c_uint64), _ptr(ids, C.c_int32))
         return keys, ids
 
+    def set_threads(self, threads):
+        """Owner-loop threads of a library that offers them (the oracle)."""
+        if self.lib.has("set_threads"):
+            self.lib.call("set_threads", self.h, int(threads))
+
+    def run_counters(self):
+        """dem_debug_run_counters as a dict (None for a library without it)."""
+        if not self.lib.has("debug_run_counters"):
+            return None
+        buf = (C.c_longlong * 6)()
+        cnt = C.c_int(0)
+        self.lib.call("debug_run_counters", self.h, buf, 6, C.byref(cnt))
+        names = ("evaluations", "sorts", "rebuilds", "graph_steps", "eager_steps", "last_sorted")
+        return {names[i]: int(buf[i]) for i in range(cnt.value)}
+
     def debug_history(self):
         nc = self.nc
+        nt = 64 if self.nb else 0  # no body table without bodies (NULL: not read)
         out = dict(pp_keys=np.zeros((nc, 64), np.uint64), pp_values=np.zeros((nc, 64, 9)),
                    pw_keys=np.zeros((nc, 16), np.uint64), pw_values=np.zeros((nc, 16, 9)),
-                   pt_keys=np.zeros((nc, 64), np.uint64), pt_values=np.zeros((nc, 64, 9)))
+                   pt_keys=np.zeros((nc, nt), np.uint64), pt_values=np.zeros((nc, nt, 9)))
+        if not nt:
+            self.lib.call("debug_history", self.h, _ptr(out["pp_keys"], C.c_uint64),
+                          _ptr(out["pp_values"], C.c_double), _ptr(out["pw_keys"], C.c_uint64),
+                          _ptr(out["pw_values"], C.c_double), None, None)
+            return out
         self.lib.call("debug_history", self.h, _ptr(out["pp_keys"], C.c_uint64),
                       _ptr(out["pp_values"], C.c_double), _ptr(out["pw_keys"], C.c_uint64),
                       _ptr(out["pw_values"], C.c_double), _ptr(out["pt_keys"], C.c_uint64),

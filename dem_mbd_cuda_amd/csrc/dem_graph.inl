@@ -78,6 +78,7 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
     int k = 0;
     auto step1 = [&]() {
         TRY(eager_step(s, dt));
+        ++s->nEagerSteps;
         ++k;
         return DEM_OK;
     };
@@ -100,6 +101,10 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
                 HIP_TRY(hipGraphLaunch(hit->exec, s->stream), "launch step graph");
                 s->stepIndex += P;
                 s->scalarsReset = false;  // the replayed evaluations wrote the counters
+                // every captured evaluation sorts (graphs run the full scan only)
+                s->nEvaluations += P;
+                s->nSorts += P;
+                s->lastSorted = true;
             } else {
                 GraphCache::Entry e{phase, s->cur, P, s->pairBlocks, s->reorderInterval, nullptr};
                 TRY(capture_steps(s, dt, P, &e.exec));
@@ -108,6 +113,7 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
                 G.entries.push_back(e);
             }
             k += P;
+            s->nGraphSteps += P;
         }
     }
     while (k < count) TRY(step1());

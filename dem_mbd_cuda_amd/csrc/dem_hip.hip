@@ -2647,6 +2647,11 @@ struct dem_solver {
     int mirrorSeq = 0;
     History H2{};              // second history buffer (candidate-list rebuilds move rows)
     long rebuilds = 0;
+    // run counters (dem_debug_run_counters): force evaluations, evaluations
+    // that sorted the particle grid on the device, steps of dem_run_steps
+    // replayed from graphs and launched eagerly
+    long long nEvaluations = 0, nSorts = 0, nGraphSteps = 0, nEagerSteps = 0;
+    bool lastSorted = false;  // the last evaluation sorted the grid on the device
     bool verletBroken = false;  // a rebuild overflowed CAND_MAX: lists stay off
     bool capturing = false;   // inside a HIP graph capture: no event records
     bool countAlways = false; // per-step counters on every step (differentiable runs read them)
@@ -3690,6 +3695,9 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
         hipLaunchKernelGGL(k_cellstart, dim3(cellBlocks), dim3(BLOCK), 0, st, s->keyA, nr, s->sc, s->cellStart,
                            s->cellCap);
     }
+    ++s->nEvaluations;
+    if (rebuild) ++s->nSorts;
+    s->lastSorted = rebuild;
     // with candidate lists the packed arrays are indexed by row, else by
     // sorted position
     if (!scanned)
@@ -4062,6 +4070,17 @@ int dem_device_memory_bytes(const dem_solver* s, size_t* out)
     return DEM_OK;
 }
 
+int dem_debug_run_counters(const dem_solver* s, long long* out, int capacity, int* count)
+{
+    if (!s) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    const long long v[6] = {s->nEvaluations, s->nSorts, (long long)s->rebuilds, s->nGraphSteps,
+                            s->nEagerSteps, s->lastSorted ? 1 : 0};
+    const int k = std::max(0, std::min(capacity, 6));
+    for (int i = 0; i < k; ++i) out[i] = v[i];
+    if (count) *count = k;
+    return DEM_OK;
+}
+
 int dem_debug_grid(const dem_solver* s, uint64_t* keys, int32_t* ids)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
@@ -4078,8 +4097,16 @@ int dem_debug_grid(const dem_solver* s, uint64_t* keys, int32_t* ids)
     HIP_TRY(hipMemcpy(hi.data(), di, sizeof(int) * n, hipMemcpyDeviceToHost), "download");
     hipFree(dk);
     hipFree(di);
-    // (cell key, input row) order of the current cells: the device order is
-    // already that after a sort, and the last rebuild's with candidate lists
+    // After an evaluation that sorted the grid the device order is returned
+    // as it is (the radix sort and its tie fix-up are what is checked).  A
+    // candidate-list step between rebuilds keeps the last rebuild's order
+    // while the cells move, so its current cells are put in (cell key, input
+    // row) order here.
+    if (s->lastSorted) {
+        if (keys) std::memcpy(keys, hk.data(), sizeof(uint64_t) * n);
+        if (ids) std::memcpy(ids, hi.data(), sizeof(int32_t) * n);
+        return DEM_OK;
+    }
     std::vector<long> idx(n);
     std::iota(idx.begin(), idx.end(), 0L);
     std::sort(idx.begin(), idx.end(), [&](long a, long b) {

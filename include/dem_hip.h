@@ -248,7 +248,8 @@ const char* dem_last_error(void);
  * Reads the state left by the last dem_compute_forces, in the reference's
  * own index space (components sorted by (owner, component id)).
  * dem_debug_grid: the particle grid as the reference sorts it: 64-bit packed
- * cell keys and component indices in (key, index) order, [c] each.
+ * cell keys and component indices in (key, index) order, [c] each; after an
+ * evaluation that sorted the grid this is the device's sorted order itself.
  * dem_debug_history: history tables in the reference's [component][slot]
  * layout; keys are the reference's (other id + 1, wall + 1, body + 1), 0 for
  * an empty slot; values are [component][slot][9] (roll, tangent, twist).
@@ -258,6 +259,16 @@ int dem_debug_grid(const dem_solver* solver, uint64_t* sorted_keys,
 int dem_debug_history(const dem_solver* solver, uint64_t* pp_keys,
                       double* pp_values, uint64_t* pw_keys, double* pw_values,
                       uint64_t* pt_keys, double* pt_values);
+/* Counters of the solver's work since initialisation (no reference
+ * counterpart; for measurement and tests), out[0..5] as far as capacity
+ * allows: force evaluations; evaluations that sorted the particle grid on
+ * the device; candidate-list rebuilds; steps of dem_run_steps replayed from
+ * HIP graphs; steps of dem_run_steps launched eagerly; 1 if the last
+ * evaluation sorted the grid (dem_debug_grid then returns the device's own
+ * sorted order as it is; between candidate-list rebuilds it puts the current
+ * cells in (key, index) order on the host). */
+int dem_debug_run_counters(const dem_solver* solver, long long* out, int capacity,
+                           int* count);
 /* Triangle grid entries (key, triangle) in sorted order, [entries] each. */
 int dem_debug_triangle_grid(const dem_solver* solver, uint64_t* keys,
                             int32_t* triangles);

@@ -30,6 +30,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define PP_SLOTS DEM_PARTICLE_HISTORY_SLOTS
 #define PW_SLOTS DEM_WALL_HISTORY_SLOTS
@@ -178,6 +181,7 @@ struct ora_solver {
     dem_step_stats cached;
     uint64_t *cachedRegimeInstant, *cachedRegimeInterval;
     double *cachedMicroInstant, *cachedMicroInterval;
+    int threads;      /* owner-loop threads (ora_set_threads) */
 };
 
 #define ALLOC(p, count) do { const size_t n_ = (size_t)(count); (p) = calloc(n_ > 0 ? n_ : 1, sizeof(*(p))); } while (0)
@@ -186,7 +190,24 @@ int ora_create(ora_solver** out)
 {
     if (!out) return fail(DEM_E_INVALID, "null output pointer");
     *out = calloc(1, sizeof(ora_solver));
-    return *out ? DEM_OK : fail(DEM_E_INVALID, "out of memory");
+    if (!*out) return fail(DEM_E_INVALID, "out of memory");
+#ifdef _OPENMP
+    (*out)->threads = omp_get_max_threads();
+#else
+    (*out)->threads = 1;
+#endif
+    return DEM_OK;
+}
+
+int ora_set_threads(ora_solver* s, int threads)
+{
+    if (!s || threads < 1) return fail(DEM_E_INVALID, "thread count must be positive");
+#ifdef _OPENMP
+    s->threads = threads;
+#else
+    s->threads = 1;
+#endif
+    return DEM_OK;
 }
 
 void ora_destroy(ora_solver* s)
@@ -708,8 +729,15 @@ static V3 body_omega_world(const ora_solver* s, int b)
     return rotate_iv(p[3], p[4], p[5], p[6], v3(p[10], p[11], p[12]));
 }
 
+/* Per-thread contact counters of one evaluation (merged after the owner
+   loop; counts add, high-water marks and depths take the maximum). */
+typedef struct {
+    int pcount, pover, phigh, bcount, bover, bhigh;
+    double bmaxdepth;
+} Tally;
+
 /* particleContact (contact:579-706) */
-static void particle_contact(ora_solver* s, const Mech* m, int ci, int cj, int step,
+static void particle_contact(ora_solver* s, Tally* t, const Mech* m, int ci, int cj, int step,
                              V3* forceSum, V3* torqueSum)
 {
     const int i = s->cowner[ci], j = s->cowner[cj];
@@ -740,7 +768,7 @@ static void particle_contact(ora_solver* s, const Mech* m, int ci, int cj, int s
     V3 fn = add(mul(dir, -m->kN * depth), mul(dir, cN * dot(erv, dir)));
     fn = add(fn, mul(dir, m->cohesion * 4.0 * brs));
     const uint64_t key = (uint64_t)(uint32_t)s->cid[cj] + 1ull;
-    const int slot = acquire_slot(s->hpk, s->hpl, s->hpv, ci, key, step, PP_SLOTS, &s->pover, &s->phigh);
+    const int slot = acquire_slot(s->hpk, s->hpl, s->hpv, ci, key, step, PP_SLOTS, &t->pover, &t->phigh);
     if (slot < 0) return;
     const double li = (ri * ri - rj * rj + d2) / (2.0 * dist);
     const double lj = dist - li;
@@ -765,11 +793,11 @@ static void particle_contact(ora_solver* s, const Mech* m, int ci, int cj, int s
     const V3 couple = add(add(twist, roll), mul(cross(dir, friction), li));
     *forceSum = add(*forceSum, cf);
     *torqueSum = add(*torqueSum, add(couple, cross(comp_off(s, ci), cf)));
-    ++s->pcount;
+    ++t->pcount;
 }
 
 /* wallContact (contact:975-1063) */
-static void wall_contact(ora_solver* s, const Mech* m, int comp, int w, int step, V3* forceSum,
+static void wall_contact(ora_solver* s, Tally* t, const Mech* m, int comp, int w, int step, V3* forceSum,
                          V3* torqueSum)
 {
     const int particle = s->cowner[comp];
@@ -781,7 +809,7 @@ static void wall_contact(ora_solver* s, const Mech* m, int comp, int w, int step
     const double depth = s->cr[comp] - distance;
     if (depth <= 0.0) return;
     const uint64_t key = (uint64_t)w + 1ull;
-    const int slot = acquire_slot(s->hwk, s->hwl, s->hwv, comp, key, step, PW_SLOTS, &s->bover, &s->bhigh);
+    const int slot = acquire_slot(s->hwk, s->hwl, s->hwv, comp, key, step, PW_SLOTS, &t->bover, &t->bhigh);
     if (slot < 0) return;
     const double radius = s->cr[comp];
     const double lever = radius - depth;
@@ -954,7 +982,7 @@ static double tri_area_scale(const ora_solver* s, int comp, int t)
 }
 
 /* bodyContact (contact:708-853) with the manifold history read-only */
-static void body_contact(ora_solver* s, const Mech* m, int comp, int t, int slot, V3* forceSum,
+static void body_contact(ora_solver* s, Tally* ty, const Mech* m, int comp, int t, int slot, V3* forceSum,
                          V3* torqueSum)
 {
     const int particle = s->cowner[comp];
@@ -1033,18 +1061,18 @@ static void body_contact(ora_solver* s, const Mech* m, int comp, int t, int slot
     r[0] += bnf.x + bfr.x; r[N] += bnf.y + bfr.y; r[2 * N] += bnf.z + bfr.z;
     r[3 * N] += btq.x; r[4 * N] += btq.y; r[5 * N] += btq.z;
     s->fzRow[(size_t)body * N + comp] += bnf.z + bfr.z;
-    ++s->bcount;
-    if (s->bmaxdepth < depth) s->bmaxdepth = depth;
+    ++ty->bcount;
+    if (ty->bmaxdepth < depth) ty->bmaxdepth = depth;
 }
 
 /* advanceBodyManifoldHistory (contact:855-949) */
-static int advance_manifold(ora_solver* s, const Mech* m, int comp, int body, int step, V3 normal,
+static int advance_manifold(ora_solver* s, Tally* ty, const Mech* m, int comp, int body, int step, V3 normal,
                             V3 cc, double distance, int* slide, int* twisty)
 {
     *slide = 0; *twisty = 0;
     const int particle = s->cowner[comp];
     const uint64_t key = (uint64_t)(body + 1);
-    const int slot = acquire_slot(s->hbk, s->hbl, s->hbv, comp, key, step, PT_SLOTS, &s->bover, &s->bhigh);
+    const int slot = acquire_slot(s->hbk, s->hbl, s->hbv, comp, key, step, PT_SLOTS, &ty->bover, &ty->bhigh);
     if (slot < 0) return -1;
     const double* bp = s->bpose + 13 * body;
     const V3 center = v3(bp[0], bp[1], bp[2]);
@@ -1083,8 +1111,26 @@ static int advance_manifold(ora_solver* s, const Mech* m, int comp, int body, in
 }
 
 /* computeContactsKernel (contact:1065-1221), owners in row order */
+static void merge_tally(ora_solver* s, const Tally* t)
+{
+    s->pcount += t->pcount; s->pover += t->pover; s->bcount += t->bcount; s->bover += t->bover;
+    if (t->phigh > s->phigh) s->phigh = t->phigh;
+    if (t->bhigh > s->bhigh) s->bhigh = t->bhigh;
+    if (t->bmaxdepth > s->bmaxdepth) s->bmaxdepth = t->bmaxdepth;
+}
+
+/* The owner loop runs on ora_set_threads() threads (OpenMP).  Every write of
+   an owner's iteration goes to that owner's rows (its forces, its
+   components' history slots, body rows and regime flags); only the counters
+   are shared, and they are merged after the loop, so results do not depend
+   on the thread count. */
 static void compute_contacts(ora_solver* s, const Mech* pm, const Mech* bm, const Mech* wm, int step)
 {
+#pragma omp parallel num_threads(s->threads)
+    {
+    Tally ty;
+    memset(&ty, 0, sizeof ty);
+#pragma omp for schedule(dynamic, 512)
     for (int i = 0; i < s->n; ++i) {
         V3 cF = v3(0, 0, 0), cT = v3(0, 0, 0), fF = v3(0, 0, 0), wF = v3(0, 0, 0), wT = v3(0, 0, 0);
         V3 bF = v3(0, 0, 0), bT = v3(0, 0, 0);
@@ -1103,7 +1149,7 @@ static void compute_contacts(ora_solver* s, const Mech* pm, const Mech* bm, cons
             expire_history(s->hpk, s->hpl, s->hpv, comp, step, PP_SLOTS);
             expire_history(s->hwk, s->hwl, s->hwv, comp, step, PW_SLOTS);
             expire_history(s->hbk, s->hbl, s->hbv, comp, step, PT_SLOTS);
-            for (int w = 0; w < s->nw; ++w) wall_contact(s, wm, comp, w, step, &wF, &wT);
+            for (int w = 0; w < s->nw; ++w) wall_contact(s, &ty, wm, comp, w, step, &wF, &wT);
             const V3 position = comp_pos(s, comp);
             const int ix = (int)floor(position.x * s->invMesh);
             const int iy = (int)floor(position.y * s->invMesh);
@@ -1116,7 +1162,7 @@ static void compute_contacts(ora_solver* s, const Mech* pm, const Mech* bm, cons
                         const int e = upper_bound_u64(s->gkey, s->nc, key);
                         for (int k = b; k < e; ++k) {
                             const int other = s->gid[k];
-                            if (s->cowner[other] != i) particle_contact(s, pm, comp, other, step, &cF, &cT);
+                            if (s->cowner[other] != i) particle_contact(s, &ty, pm, comp, other, step, &cF, &cT);
                         }
                     }
             if (s->nt <= 0 || s->tentries <= 0) continue;
@@ -1144,7 +1190,7 @@ static void compute_contacts(ora_solver* s, const Mech* pm, const Mech* bm, cons
                 const V3 mC = mul(wC, 1.0 / total);
                 const double mD = wD / total;
                 int slide = 0, twisty = 0;
-                const int slot = advance_manifold(s, bm, comp, body, step, mN, mC, mD, &slide, &twisty);
+                const int slot = advance_manifold(s, &ty, bm, comp, body, step, mN, mC, mD, &slide, &twisty);
                 if (slot < 0) continue;
                 s->regimeState[(size_t)body * s->nc + comp] = 1 | (slide ? 2 : 0) | (twisty ? 4 : 0);
                 Mech ro = *bm;
@@ -1152,7 +1198,7 @@ static void compute_contacts(ora_solver* s, const Mech* pm, const Mech* bm, cons
                 for (int k = b; k < e; ++k) {
                     const int t = s->ttri[k];
                     if (s->tbody[t] != body) continue;
-                    body_contact(s, &ro, comp, t, slot, &bF, &bT);
+                    body_contact(s, &ty, &ro, comp, t, slot, &bF, &bT);
                 }
             }
         }
@@ -1165,6 +1211,9 @@ static void compute_contacts(ora_solver* s, const Mech* pm, const Mech* bm, cons
         d = s->ftotal + 6 * i; d[0] += tF.x; d[1] += tF.y; d[2] += tF.z; d[3] += tT.x; d[4] += tT.y; d[5] += tT.z;
         s->fx[i] = tF.x; s->fy[i] = tF.y; s->fz[i] = tF.z;
         s->tx[i] = tT.x; s->ty[i] = tT.y; s->tz[i] = tT.z;
+    }
+#pragma omp critical
+    merge_tally(s, &ty);
     }
 }
 

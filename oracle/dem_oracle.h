@@ -29,6 +29,9 @@ typedef struct ora_solver ora_solver;
 
 int ora_create(ora_solver** out);
 void ora_destroy(ora_solver* s);
+/* Threads of the per-owner contact loop (OpenMP; default: all the runtime
+ * offers).  Results do not depend on it.  No reference counterpart. */
+int ora_set_threads(ora_solver* s, int threads);
 int ora_initialize(ora_solver* s, const dem_particles_view* particles,
                    const dem_components_view* components,
                    const dem_walls_view* walls, const dem_bodies_view* bodies,

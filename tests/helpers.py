@@ -14,7 +14,8 @@ def run_frames(lib, scene, steps=None, interval=None, debug=False):
     frames = {}
 
     def cb(step, t, extra):
-        fr = dict(state=s.state(), forces=s.forces(), stats=s.stats(), extra=extra)
+        fr = dict(state=s.state(), forces=s.forces(), stats=s.stats(), extra=extra,
+                  counters=s.run_counters())
         if debug:
             fr["grid"] = s.debug_grid()
             fr["history"] = s.debug_history()

@@ -267,3 +267,30 @@ def test_fine_mesh_rows_beyond_the_stored_hits(hip, oracle):
     o = run_frames(oracle, sc, steps=30, interval=10, debug=True)
     assert o[0]["stats"]["body_contacts"] > 16 * 20
     compare_frames(g, o)
+
+
+@pytest.mark.parametrize("skin", ["0", "0.01"])
+def test_device_sort_order(hip, oracle, skin, monkeypatch):
+    """The grid's sorted order as the device sorts it: 3,000 spheres in a
+    box with several spheres per cell (ties ordered by input row), agitated,
+    a frame every step.  With the full scan (DEM_SKIN=0) every evaluation
+    sorts on the device; with a small skin the candidate lists are rebuilt
+    every few steps.  On every frame whose evaluation sorted, dem_debug_grid
+    returns the device's keys and order untouched, and they equal the
+    oracle's (key, index) order bit for bit."""
+    from dem_mbd_cuda_amd import scene
+
+    monkeypatch.setenv("DEM_SKIN", skin)
+    sc = scene.box_scene(n=3000, seed=29, box=(0.05, 0.05, 0.06), steps=30, dt=1e-6, rmin=0.4e-3, rmax=1.5e-3)
+    rng = np.random.RandomState(30)
+    sc.vel = rng.uniform(-1.0, 1.0, (sc.n, 3))
+    g = run_frames(hip, sc, interval=1, debug=True)
+    o = run_frames(oracle, sc, interval=1, debug=True)
+    sorted_frames = [i for i in g if g[i]["counters"]["last_sorted"]]
+    if skin == "0":
+        assert len(sorted_frames) == len(g)
+    else:
+        assert 3 <= len(sorted_frames) < len(g), sorted_frames
+    keys = o[0]["grid"][0]
+    assert len(np.unique(keys)) < 0.9 * len(keys)  # ties exist
+    compare_frames(g, o)

@@ -349,3 +349,31 @@ def test_arc_tangent_accuracy():
     for y, x in [(0.0, 1.0), (-0.0, 1.0), (0.0, -1.0), (-0.0, -1.0), (1.0, 0.0), (-1.0, 0.0),
                  (0.0, 0.0), (0.0, -0.0)]:
         assert f(y, x) == math.atan2(y, x) and math.copysign(1, f(y, x)) == math.copysign(1, math.atan2(y, x))
+
+
+def test_oracle_threads_do_not_change_results(oracle):
+    """The oracle's owner loop runs on several threads; one thread and
+    eight give the same bits (states, forces, history tables, statistics)
+    on a box of 3,000 agitated spheres over 20 steps."""
+    from dem_mbd_cuda_amd import host
+
+    sc = scene.box_scene(n=3000, seed=23, box=(0.05, 0.05, 0.06), steps=20, dt=1e-6)
+    rng = np.random.RandomState(24)
+    sc.vel = rng.uniform(-0.5, 0.5, (sc.n, 3))
+    sc.angvel = rng.uniform(-40, 40, (sc.n, 3))
+    out = []
+    for threads in (1, 8):
+        s = oracle.solver()
+        s.set_threads(threads)
+        s.initialize(sc)
+        host.run_pure(s, sc)
+        out.append((s.state(), s.forces(), s.debug_history(), s.stats()))
+        s.close()
+    (a, fa, ha, sa), (b, fb, hb, sb) = out
+    assert sa["particle_contacts"] > 0 and sa["particle_contacts"] == sb["particle_contacts"]
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("contact", "wall", "total"):
+        assert np.array_equal(fa[k], fb[k]), k
+    for k in ha:
+        assert np.array_equal(ha[k], hb[k]), k

@@ -90,7 +90,7 @@ def _build(tmp_path, idx, old, new, occurrence):
     src = tmp_path / f"mutant{idx}.c"
     src.write_text(mutated)
     so = tmp_path / f"libmutant{idx}.so"
-    subprocess.run(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-std=c11", "-shared", "-I", os.path.join(ROOT, "oracle"),
+    subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c11", "-shared", "-I", os.path.join(ROOT, "oracle"),
                     "-o", str(so), str(src), "-lm"], check=True)
     return Library(str(so), "ora_")
 
