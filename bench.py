@@ -9,12 +9,16 @@ near-touching bed, about 3.7 stored contacts per sphere; --packing picks
 it), averaged over 1,000 steps; the loose seeded-insertion state (no
 contacts yet) is measured beside it.  A step is the reference's pure-DEM
 loop body: advanceParticles, computeForces, finishParticleStep, all state
-resident in HBM, replayed from captured HIP graphs.
+resident in HBM, run through dem_run_steps in the mode its autotuner picks
+(candidate lists with eager launches, or the full scan replayed from
+captured HIP graphs; the JSON line reports which ran and how many list
+rebuilds the timed window held).
 
 Prints one JSON line with the throughput, a roofline entry for the dominant
 kernel (the per-contact law k_pairs, timed with HIP events on the solver's
-stream), the per-step algorithmic-byte roofline, and a CPU baseline (the CPU
-oracle, single thread, on a bounded sample of the same workload).
+stream), the per-step algorithmic-byte roofline, and a CPU baseline: the
+reference's own CPU binary (built from its sources) on a bounded sample of
+the same workload, with the oracle on 1 and on all host threads beside it.
 """
 from __future__ import annotations
 
@@ -49,58 +53,84 @@ def step_bytes_per_particle(cbar: float, passes: int = 3) -> float:
     return 232.0 + 232.0 + 48.0 + 12.0 + 24.0 * passes + 160.0 * cbar
 
 
-def measure_sweep(dm, scene, steps, warmup, seq_steps, count, seed):
+def _counter_delta(before, after):
+    if before is None or after is None:
+        return None
+    return {k: after[k] - before[k] for k in ("evaluations", "sorts", "rebuilds", "graph_steps", "eager_steps")}
+
+
+def measure_sweep(dm, scene, steps, warmup, seq_steps, count, seed, agitate=False):
     """Parameter sweep of `count` independent box scenes: one batched solver
     against the same scenes run one after another in single-scene solvers
-    on the same GPU.  Both sides run their steps through dem_run_steps
-    (captured HIP graphs); the sequential side is also timed with eager
-    launches.  Returns ms per step of each and the ratios."""
+    on the same GPU.  Every solver runs `warmup` untimed steps through
+    dem_run_steps with the autotuner (candidate lists or the full scan with
+    captured HIP graphs, reorder interval, pair grid), then the timed steps.
+    agitate: seeded velocities U[-3, 3] m/s and spins U[-100, 100] rad/s, so
+    that contacts form during the run (the scenes are loose insertions).
+    Returns ms per step of each, the ratio and how the timed steps ran."""
     import time as _t
 
+    import numpy as np
+
     scenes = scene.sweep_scenes(count=count, seed=seed)
+    if agitate:
+        rng = np.random.RandomState(seed + 100)
+        for sc in scenes:
+            sc.vel = rng.uniform(-3.0, 3.0, (sc.n, 3))
+            sc.angvel = rng.uniform(-100.0, 100.0, (sc.n, 3))
     batch = scene.SceneBatch(scenes)
     dt = batch.combined.dt
     s = dm.CudaDemSolver()
     s.initialize_batch(batch)
     s.compute_forces(collect=True)
     s.init_half_step(0.5 * dt)
-    s.run_steps(warmup, dt)
+    s.run_steps(warmup, dt, autotune=True)
     s.synchronize()
+    c0 = s.run_counters()
     t0 = _t.perf_counter()
     s.run_steps(steps, dt)
     s.synchronize()
     batched = (_t.perf_counter() - t0) / steps
+    counters = _counter_delta(c0, s.run_counters())
+    s.advance(dt)
+    s.compute_forces(collect=True)
     contacts = s.stats()["particle_contacts"]
     s.close()
-    seq = {True: 0.0, False: 0.0}
+    seq = 0.0
+    seq_graph_steps = 0
     for sc in scenes:
-        for graph in (True, False):
-            q = dm.CudaDemSolver()
-            q.initialize(sc)
-            q.compute_forces(collect=True)
-            q.init_half_step(0.5 * dt)
-            q.run_steps(4, dt, graph=graph)
-            q.synchronize()
-            t0 = _t.perf_counter()
-            q.run_steps(seq_steps, dt, graph=graph)
-            q.synchronize()
-            seq[graph] += (_t.perf_counter() - t0) / seq_steps
-            q.close()
+        q = dm.CudaDemSolver()
+        q.initialize(sc)
+        q.compute_forces(collect=True)
+        q.init_half_step(0.5 * dt)
+        q.run_steps(warmup, dt, autotune=True)
+        q.synchronize()
+        c0 = q.run_counters()
+        t0 = _t.perf_counter()
+        q.run_steps(seq_steps, dt)
+        q.synchronize()
+        seq += (_t.perf_counter() - t0) / seq_steps
+        seq_graph_steps += _counter_delta(c0, q.run_counters())["graph_steps"]
+        q.close()
     return {"scenes": count, "particles": batch.n, "batched_ms_per_step": 1e3 * batched,
-            "sequential_ms_per_step": 1e3 * seq[True], "speedup": seq[True] / batched,
-            "sequential_eager_ms_per_step": 1e3 * seq[False], "speedup_vs_eager": seq[False] / batched,
-            "batched_particle_steps_per_s": batch.n / batched, "contacts": contacts,
-            "steps": steps, "sequential_steps_per_scene": seq_steps,
+            "sequential_ms_per_step": 1e3 * seq, "speedup": seq / batched,
+            "batched_particle_steps_per_s": batch.n / batched, "contacts_after": contacts,
+            "steps": steps, "warmup_autotuned": warmup, "sequential_steps_per_scene": seq_steps,
+            "batched_timed_steps": counters,
+            "sequential_graph_step_share": seq_graph_steps / (count * seq_steps),
             "workload": f"{count} boxes, N ~ U[500, 20000] spheres r ~ U[1, 1.5] mm, seeded RSA, "
-                        "per-scene mu/restitution/E, dt 1e-6, gravity"}
+                        "per-scene mu/restitution/E, dt 1e-6, gravity"
+                        + (", agitated (v ~ U[-3, 3] m/s, w ~ U[-100, 100] rad/s)" if agitate else ", at rest")}
 
 
 def measure_adjoint(dm, scene, count, n, steps, K, seed):
     """Differentiable batch: `count` settling beds of n spheres; the
     gradient of bed height + plate load w.r.t. every scene's (mu,
     restitution, E) through `steps` steps with checkpoints every K steps.
-    Cost ratio = (reverse sweep incl. recomputation) / (plain forward run of
-    the same steps)."""
+    adjoint_over_forward = (reverse sweep incl. recomputation) / (plain
+    forward run of the same steps); gradient_over_forward adds the
+    checkpointed forward run to the numerator (the whole cost of one
+    gradient against one plain forward run)."""
     import time as _t
 
     import numpy as np
@@ -130,7 +160,8 @@ def measure_adjoint(dm, scene, count, n, steps, K, seed):
     q.close()
     return {"scenes": count, "particles": batch.n, "steps": steps, "checkpoint_interval": K,
             "forward_s": t_fwd, "forward_with_checkpoints_s": t_dfwd, "backward_s": t_bwd,
-            "adjoint_over_forward": t_bwd / t_fwd, "device_bytes": mem,
+            "adjoint_over_forward": t_bwd / t_fwd,
+            "gradient_over_forward": (t_dfwd + t_bwd) / t_fwd, "device_bytes": mem,
             "finite_gradients": bool(np.all(np.isfinite(g))),
             "workload": f"{count} settling beds x {n} spheres r ~ U[1, 1.5] mm, 30 mm box, dt 1e-6, "
                         f"d(bed height + plate load)/d(mu, restitution, E) per scene"}
@@ -139,8 +170,11 @@ def measure_adjoint(dm, scene, count, n, steps, K, seed):
 def measure_hopper(dm, scene, args, packing, steps, warmup, timing_steps):
     """Builds the 1M hopper in the given packing, runs `warmup` steps (the
     autotuner picks candidate lists / reorder interval / pair grid), times
-    `steps` graph-replayed steps, then `timing_steps` eager steps with HIP
-    events around each phase (grid, scan, k_pairs, k_law, reductions)."""
+    `steps` steps through dem_run_steps (replayed from captured HIP graphs
+    in the full-scan modes, launched eagerly with candidate lists; the
+    counters of the timed window say which, and how many list rebuilds it
+    held), then `timing_steps` eager steps with HIP events around each phase
+    (grid, scan, k_pairs, k_law, reductions)."""
     import numpy as np
 
     t0 = time.time()
@@ -153,10 +187,12 @@ def measure_hopper(dm, scene, args, packing, steps, warmup, timing_steps):
     s.init_half_step(0.5 * dt)
     s.run_steps(warmup, dt, graph=True, autotune=True)
     s.synchronize()
+    c0 = s.run_counters()
     t1 = time.perf_counter()
     s.run_steps(steps, dt, graph=True)
     s.synchronize()
     elapsed = time.perf_counter() - t1
+    window = _counter_delta(c0, s.run_counters())
     tms = []
     s.enable_timings(True)
     for _ in range(timing_steps):
@@ -172,7 +208,7 @@ def measure_hopper(dm, scene, args, packing, steps, warmup, timing_steps):
     s.close()
     out = {"sc": sc, "value": sc.n * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
            "phase_ms": {"grid": tm[0], "scan": tm[1], "pairs": tm[2], "law": tm[3]},
-           "contacts": st["particle_contacts"], "cbar": st["particle_contacts"] / sc.n}
+           "contacts": st["particle_contacts"], "cbar": st["particle_contacts"] / sc.n, "timed_steps": window}
     print(f"# {packing}: ms/step {out['ms_per_step']:.3f}; grid {tm[0]:.3f} ms, scan {tm[1]:.3f} ms, "
           f"k_pairs {tm[2]:.3f} ms, k_law {tm[3]:.3f} ms; contacts/particle {out['cbar']:.3f}",
           file=sys.stderr, flush=True)
@@ -219,6 +255,85 @@ def measure_coupled(dm, scene, n, steps, warmup):
                         "solve every step"}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def measure_cpu(dm, scene, n, steps, packing, mesh_scale):
+    """CPU baseline on a bounded sample of the headline workload: a hopper of
+    n spheres with the same radii, walls, packing and bed depth (the width
+    scaled by sqrt(n / 1M)).
+    * The reference's own CPU path: oracle/_ref/dem_mbd_cpu, the reference's
+      host and legacy contact law built from its sources (ForceCaculate,
+      CreatTree and the leapfrog of simulation.cpp IntegrateDem, single
+      threaded by design), run on the sample written as a case directory
+      with outputs off, once for 1 step and once for 1 + `steps`; the
+      difference over `steps` is its time per step (reading the input, the
+      initial grid and evaluation cancel out).
+    * The oracle (the C restatement of the CUDA law) on the same sample: one
+      step after its initial evaluation, on 1 thread and on all threads."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    from dem_mbd_cuda_amd import _abi
+
+    sc = scene.hopper_scene(n=n, width=0.2 * (n / 1e6) ** 0.5, seed=2, mesh_scale=mesh_scale, packing=packing)
+    out = {"unit": "particle-steps/s", "cores": 1, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+           "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    exe = os.path.join(ROOT, "oracle", "_ref", "dem_mbd_cpu")
+    if os.path.exists(exe):
+        work = tempfile.mkdtemp(prefix="dem_refcpu_")
+        try:
+            t = {}
+            for k in (1, 1 + steps):
+                d = scene.write_case(sc, os.path.join(work, f"case{k}"), k, outputs=False)
+                t0 = time.perf_counter()
+                r = subprocess.run([exe, d], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                                   timeout=600)
+                t[k] = time.perf_counter() - t0
+                if r.returncode != 0:
+                    raise RuntimeError("reference CPU binary failed: " + r.stdout[-2000:])
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        per = (t[1 + steps] - t[1]) / steps
+        out.update(value=n / per, kind="reference",
+                   sample=f"the reference's CPU binary (its host and legacy law built from source, single-threaded "
+                          f"by design) on a {n}-sphere hopper of the headline workload ({packing} bed, width "
+                          f"scaled to keep the bed depth): {steps} steps timed as the difference of a "
+                          f"{1 + steps}-step and a 1-step run ({t[1 + steps]:.1f} s and {t[1]:.1f} s), outputs off",
+                   reference_s_per_step=per)
+    lib = _abi.Library(os.path.join(ROOT, "oracle", "liboracle.so"), "ora_")
+    port = {}
+    threads_all = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    for threads in sorted({1, threads_all}):
+        o = lib.solver()
+        o.set_threads(threads)
+        o.initialize(sc)
+        o.compute_forces(collect=True)
+        o.init_half_step(0.5 * sc.dt)
+        c1 = time.perf_counter()
+        o.advance(sc.dt)
+        o.compute_forces(collect=False)
+        o.finish(sc.dt)
+        o.state()
+        port[threads] = n / (time.perf_counter() - c1)
+        o.close()
+    out["port"] = {"kind": "port", "what": "the oracle (C restatement of the CUDA law) on the same sample, one step",
+                   "threads_1": port[1], "threads_all": port[threads_all], "threads": threads_all}
+    if "value" not in out:  # no reference build: the port on one thread is the baseline
+        out.update(value=port[1], kind="port", sample=out["port"]["what"] + ", 1 thread")
+    else:
+        out["port"]["ratio_to_reference_1_thread"] = port[1] / out["value"]
+    return out
+
+
 def copy_calibration(gib=1):
     """A plain device copy of `gib` GiB (read + write), the reference kernel
     for scaling FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -254,7 +369,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=120,
                     help="untimed steps; from 104 on they include the autotuner's trial spans")
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="reference-binary steps timed for the CPU baseline")
+    ap.add_argument("--cpu-n", type=int, default=100_000, help="spheres of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mesh-scale", type=float, default=2.0,
                     help="grid cell in max radii (the 1M configuration states 2)")
@@ -330,10 +446,15 @@ def main():
     sweep = None
     if not args.no_sweep:
         t0 = time.time()
-        sweep = measure_sweep(dm, scene, args.sweep_steps, 10, args.sweep_seq_steps, args.sweep_scenes, 4)
-        print(f"# sweep: {sweep['scenes']} scenes, {sweep['particles']} spheres: batched "
-              f"{sweep['batched_ms_per_step']:.3f} ms/step, sequential {sweep['sequential_ms_per_step']:.3f} "
-              f"ms/step, speedup {sweep['speedup']:.1f}x ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
+        sweep = measure_sweep(dm, scene, args.sweep_steps, 120, args.sweep_seq_steps, args.sweep_scenes, 4)
+        sweep["with_contacts"] = measure_sweep(dm, scene, args.sweep_steps, 120, args.sweep_seq_steps,
+                                               args.sweep_scenes, 4, agitate=True)
+        for tag, sw in (("at rest", sweep), ("agitated", sweep["with_contacts"])):
+            print(f"# sweep ({tag}): {sw['scenes']} scenes, {sw['particles']} spheres, {sw['contacts_after']} "
+                  f"contacts: batched {sw['batched_ms_per_step']:.3f} ms/step, sequential "
+                  f"{sw['sequential_ms_per_step']:.3f} ms/step, speedup {sw['speedup']:.1f}x; timed batch steps "
+                  f"{sw['batched_timed_steps']}", file=sys.stderr, flush=True)
+        print(f"# sweeps took {time.time() - t0:.0f}s", file=sys.stderr, flush=True)
 
     adjoint = None
     if not args.no_adjoint:
@@ -342,33 +463,16 @@ def main():
                                   args.adjoint_K, 5)
         print(f"# adjoint: {adjoint['scenes']} x {args.adjoint_n} spheres, {adjoint['steps']} steps: forward "
               f"{adjoint['forward_s']:.2f}s, forward+checkpoints {adjoint['forward_with_checkpoints_s']:.2f}s, "
-              f"reverse {adjoint['backward_s']:.2f}s, ratio {adjoint['adjoint_over_forward']:.2f} "
+              f"reverse {adjoint['backward_s']:.2f}s, ratio {adjoint['adjoint_over_forward']:.2f}, whole gradient {adjoint['gradient_over_forward']:.2f}x forward "
               f"({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
-        from dem_mbd_cuda_amd import _abi, host
-
-        lib = _abi.Library(os.path.join(ROOT, "oracle", "liboracle.so"), "ora_")
-        o = lib.solver()
-        o.initialize(sc)
-        c0 = time.perf_counter()
-        o.compute_forces(collect=True)  # the initial evaluation of the reference loop (untimed)
-        o.init_half_step(0.5 * sc.dt)
-        c1 = time.perf_counter()
-        for _ in range(args.cpu_steps):
-            o.advance(sc.dt)
-            o.compute_forces(collect=False)
-            o.finish(sc.dt)
-        c2 = time.perf_counter()
-        o.close()
-        cpu = {"value": sc.n * args.cpu_steps / (c2 - c1), "unit": "particle-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"CPU oracle (C restatement of the reference law, 1 thread) on the same "
-                         f"{sc.n}-sphere {args.packing} scene: {args.cpu_steps} full step(s) timed "
-                         f"({c2 - c1:.1f} s) after the untimed initial evaluation ({c1 - c0:.1f} s)"}
-        print(f"# cpu oracle: {cpu['value']:.3g} particle-steps/s ({c2 - c1:.1f} s timed, {c1 - c0:.1f} s set-up)",
-              file=sys.stderr, flush=True)
+        t0 = time.time()
+        cpu = measure_cpu(dm, scene, args.cpu_n, args.cpu_steps, args.packing, args.mesh_scale)
+        print(f"# cpu: reference {cpu.get('value', 0):.3g} particle-steps/s ({cpu.get('kind')}), port "
+              f"{cpu['port']['threads_1']:.3g} (1 thread) / {cpu['port']['threads_all']:.3g} "
+              f"({cpu['port']['threads']} threads) ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
 
     out = {
         "metric": "FP64 particle-steps/s @1M",
@@ -385,6 +489,7 @@ def main():
         "data": "synthetic (seeded lognormal radii and packing)",
         "config": {"workload": workload, "particles": sc.n, "dt": sc.dt, "packing": args.packing,
                    "contacts_per_particle": head["cbar"]},
+        "timed_window": head["timed_steps"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_pairs", "kernel_ms": pairs_ms, "bytes_per_launch": bytes_pairs,

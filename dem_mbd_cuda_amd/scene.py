@@ -751,6 +751,48 @@ def read_bodies_csv(path, resolve):
                 body_tri_count=np.array(counts, np.int32))
 
 
+def write_case(sc, case_dir, steps, output_interval=None, outputs=True):
+    """Writes a pure-DEM Scene (single-sphere owners, walls, no bodies) as a
+    reference case directory: InputFile/Particles/Particles.bt (the 17 fields
+    of particle.cpp:66-73, written with the ph-file writer, whose columns are
+    those fields), InputFile/Environment/Wall.bt and
+    InputFile/SettingData/world.par (simulation.cpp:158-307) with the
+    scene's MECH block, gravity, step size and `steps` steps; frame files
+    every `output_interval` steps when `outputs` is set, none otherwise."""
+    from . import output
+
+    if len(sc.body_mass) or len(sc.comp_id) != sc.n or np.any(sc.comp_pos_body != 0.0):
+        raise ValueError("write_case takes pure-DEM scenes of single spheres")
+    inp = os.path.join(case_dir, "InputFile")
+    output.write_particle_state(os.path.join(inp, "Particles", "Particles.bt"), sc.pid, sc.status, sc.mass,
+                                sc.radius, sc.pos, sc.vel, sc.quat, sc.angvel)
+    os.makedirs(os.path.join(inp, "Environment"), exist_ok=True)
+    with open(os.path.join(inp, "Environment", "Wall.bt"), "w") as f:
+        f.write(f"{len(sc.wall_origin)}\n")
+        for o, nrm in zip(np.reshape(sc.wall_origin, (-1, 3)), np.reshape(sc.wall_normal, (-1, 3))):
+            f.write(" ".join("%.17g" % v for v in (*o, *nrm)) + "\n")
+    m = sc.pp
+    g = "%.17g"
+    par = {"ODE_Option": 1, "ODE_StepSize": g % sc.dt, "ODE_EndStep": int(steps), "ODE_StartStep": 0,
+           "ODE_OutputInterval": int(output_interval or steps or 1),
+           "MECH_mu": g % m["mu"], "MECH_mu_T": g % m["muT"], "MECH_mu_R": g % m["muR"],
+           "MECH_epsN": g % m["epsN"], "MECH_epsS": g % m["epsS"], "MECH_cohesive": g % m["cohesion"],
+           "MECH_beta": g % m["beta"], "MECH_kN": g % m["kN"],
+           "CONTROL_GRAVITY_FLAG": int(sc.gravity_enabled), "CONTROL_GRAVITY_X": g % sc.gravity[0],
+           "CONTROL_GRAVITY_Y": g % sc.gravity[1], "CONTROL_GRAVITY_Z": g % sc.gravity[2],
+           "CONTROL_WALL_FLAG": 1 if len(sc.wall_origin) else 0,
+           "CONTROL_WALL_FILE": "Data\\DATA\\InputFile\\Environment\\Wall.bt", "CONTROL_MULTIBODY_FLAG": 0,
+           "CONTROL_UNIVERSAL_GRAVITATION_FLAG": int(sc.universal_gravity),
+           "CONTROL_PARTICLES_FILE": "Data\\DATA\\InputFile\\Particles\\Particles.bt"}
+    for key in ("PARTICLE_STATE", "PARTICLE_FORCE", "BODY_STATE", "BODY_FORCE"):
+        par[f"CONTROL_OUTPUT_{key}"] = 1 if outputs else 0
+    par["CONTROL_OUTPUT_SUMMARY"] = 0
+    os.makedirs(os.path.join(inp, "SettingData"), exist_ok=True)
+    with open(os.path.join(inp, "SettingData", "world.par"), "w") as f:
+        f.write("".join(f"{k} = {v}\n" for k, v in par.items()))
+    return case_dir
+
+
 def read_case(case_dir):
     """Loads a reference case directory (InputFile/...) into a Scene."""
     par = read_world_par(os.path.join(case_dir, "InputFile", "SettingData", "world.par"))
