@@ -21,6 +21,7 @@
 struct GraphCache {
     struct Entry {
         int phase, cur, period, pairBlocks, reorder;
+        const double* hist;  // the history buffer in use when captured
         hipGraphExec_t exec;
     };
     std::vector<Entry> entries;
@@ -94,7 +95,7 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
             GraphCache::Entry* hit = nullptr;
             for (auto& e : G.entries)
                 if (e.phase == phase && e.cur == s->cur && e.period == P && e.pairBlocks == s->pairBlocks &&
-                    e.reorder == s->reorderInterval)
+                    e.reorder == s->reorderInterval && e.hist == s->H.ppVal)
                     hit = &e;
             if (hit) {
                 // the captured steps reorder an even number of times: the buffer is unchanged
@@ -106,7 +107,7 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
                 s->nSorts += P;
                 s->lastSorted = true;
             } else {
-                GraphCache::Entry e{phase, s->cur, P, s->pairBlocks, s->reorderInterval, nullptr};
+                GraphCache::Entry e{phase, s->cur, P, s->pairBlocks, s->reorderInterval, s->H.ppVal, nullptr};
                 TRY(capture_steps(s, dt, P, &e.exec));
                 HIP_TRY(hipGraphLaunch(e.exec, s->stream), "launch step graph");
                 s->scalarsReset = false;

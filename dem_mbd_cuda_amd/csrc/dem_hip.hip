@@ -267,6 +267,14 @@ __device__ __forceinline__ T wave_min(T v)
     return v;
 }
 
+// Orders a wave's LDS accesses across its lanes (no block barrier).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Block-wide int reductions finished with one atomic per wave.
 __device__ __forceinline__ void atomic_add_wave(int* dst, int v)
 {
@@ -1290,14 +1298,6 @@ __global__ void __launch_bounds__(BLOCK) k_off_apply(const int* cnt, long n, con
     }
 }
 
-// Orders a wave's LDS accesses across its lanes (no block barrier).
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // A row's contacts are summed inside k_pairs when its list [off, off + cnt)
 // lies in one aligned 64-entry chunk, i.e. in one wave's iteration.
 __device__ __forceinline__ bool row_in_chunk(long base, int cnt)
@@ -1311,6 +1311,11 @@ __device__ __forceinline__ bool row_in_chunk(long base, int cnt)
 // summed here by its first contact's lane, from +0.0 in list order (the order
 // and start of the owner's own sum), into rowF[row]; the other rows' contacts
 // write their force and torque term to pairF for the owner to add.
+// ROWS: candidate-list steps, where the list's owner index is the row and
+// every row's history sits in its own row (the identity order and I_HIST of
+// the rebuilds), so no index is looked up.  ONE: a single scene, whose law
+// is uniform across the grid (scalar loads).
+template <bool ROWS, bool ONE>
 __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs rec,
                                                  const int2* pairs, const unsigned* off, const int* ncount,
                                                  History H, const Material* mats,
@@ -1328,7 +1333,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs 
         int len = 0;
         if (e < total) {
         const int2 pr = pairs[e];
-        const long p = sorted[pr.x];
+        const long p = ROWS ? (long)pr.x : (long)sorted[pr.x];
         const long q = pr.y >> SLOT_BITS;
         const int slot = pr.y & (PP_SLOTS - 1);
         Body I = rec_body(rec, p);
@@ -1336,8 +1341,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs 
         const Body J = rec_body(rec, q);
         const V3 delta = sub(J.pos, I.pos);
         const double d2 = dot(delta, delta);
-        const long o = S.k(I_HIST, p);
-        Material m = mats[rec.aux[p].scene];
+        const long o = ROWS ? p : (long)S.k(I_HIST, p);
+        Material m = ONE ? mats[0] : mats[rec.aux[p].scene];
         if (first) m.dt = 0.0;  // the first evaluation advances no history
         double* hv = pp_slot(H, o, slot);
         if (tapePair) {
@@ -3188,7 +3193,9 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     HIP_TRY(hipMemset(s->H.pwSeen, 0, sizeof(unsigned) * 2 * nr), "clear history");
     HIP_TRY(hipMemset(s->H.ppKey, 0, sizeof(uint32_t) * PP_SLOTS * nr), "clear history");
     HIP_TRY(hipMemset(s->H.pwKey, 0, sizeof(uint32_t) * PW_SLOTS * nr), "clear history");
-    if (s->verlet) {
+    // a second set of tables: reorders (candidate-list rebuilds and the
+    // full scan's periodic cell ordering) move each row's history with it
+    if (!clump) {
         s->H2.rows = nr;
         TRY(dalloc(s->H2.ppOcc, nr, "hipMalloc(particle history)"));
         TRY(dalloc(s->H2.ppSeen, 2 * nr, "hipMalloc(particle history)"));
@@ -3233,7 +3240,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         HIP_TRY(hipMemset(s->H.ptOcc, 0, sizeof(unsigned long long) * nr), "clear history");
         HIP_TRY(hipMemset(s->H.ptSeen, 0, sizeof(unsigned long long) * 2 * nr), "clear history");
         HIP_TRY(hipMemset(s->H.ptKey, 0, sizeof(uint32_t) * PT_SLOTS * nr), "clear history");
-        if (s->verlet) {
+        if (s->H2.ppOcc) {
             TRY(dalloc(s->H2.ptOcc, nr, "hipMalloc(body history)"));
             TRY(dalloc(s->H2.ptSeen, 2 * nr, "hipMalloc(body history)"));
             TRY(dalloc(s->H2.ptKey, (size_t)PT_SLOTS * nr, "hipMalloc(body history keys)"));
@@ -3686,7 +3693,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             S = s->soa();
             R = S;
             s->recFresh = false;  // the records are in the old row order
-            if (verlet && s->H2.ppOcc) {
+            if (s->H2.ppOcc) {
                 hipLaunchKernelGGL(k_hist_move, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->H, s->H2);
                 std::swap(s->H, s->H2);
             }
@@ -3755,9 +3762,13 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
                        s->listOff, s->pairs);
     TRY(launch_check("contact scan"));
     if (timed) HIP_TRY(hipEventRecord(s->ev[2], st), "event");
-    hipLaunchKernelGGL(k_pairs, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, ord, s->recs(), s->pairs,
-                       (const unsigned*)s->listOff, (const int*)s->ncount, s->H, s->dMatPP, step == 0, s->offs,
-                       s->sc, s->pairF, s->rowF, s->tape.pair, s->tape.pairPrev);
+    {
+        auto* kp = useLists ? (s->nscenes == 1 ? k_pairs<true, true> : k_pairs<true, false>)
+                            : (s->nscenes == 1 ? k_pairs<false, true> : k_pairs<false, false>);
+        hipLaunchKernelGGL(kp, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, ord, s->recs(), s->pairs,
+                           (const unsigned*)s->listOff, (const int*)s->ncount, s->H, s->dMatPP, step == 0,
+                           s->offs, s->sc, s->pairF, s->rowF, s->tape.pair, s->tape.pairPrev);
+    }
     TRY(launch_check("particle contact law"));
     if (timed) HIP_TRY(hipEventRecord(s->ev[3], st), "event");
     LawArgs A{};
