@@ -18,8 +18,12 @@
 // particle-particle and particle-wall laws (dem_adjoint.hpp), component
 // velocities and advance.  The history adjoint follows the slots: a slot
 // that a contact acquired afresh takes no derivative from the older values.
-// Particles never move rows during a differentiable run (no reordering), so
-// row = input index throughout.
+// Particles never move rows during a differentiable run: the rows are put in
+// cell order once before it starts, and neither the full scan nor the
+// candidate-list rebuilds reorder them afterwards, so a row on the tape names
+// the same particle throughout.  Candidate lists (when the solver has them)
+// replace the full scan on the forward run and on every recomputed segment;
+// they yield the full scan's contacts in its order, so the tape is the same.
 
 namespace {
 
@@ -502,6 +506,7 @@ int restore_checkpoint(dem_solver* s, const Checkpoint& c)
     const long n = s->n;
     hipStream_t st = s->stream;
     s->recFresh = false;
+    s->vlistValid = false;  // the candidate lists are rebuilt from the restored centres
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
     };
@@ -618,7 +623,8 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
     R.K = checkpoint_interval;
     R.plate = plate_wall;
     s->reorderInterval = 0;  // rows stay put (after diff_reorder): every tape entry is a row
-    s->verlet = false;       // (candidate-list rebuilds reorder rows)
+    s->keepRows = true;      // (and candidate-list rebuilds leave them in place)
+    s->verlet = s->refC != nullptr && !s->verletBroken;
     s->countAlways = true;   // checkpoints size history copies by the high-water marks
     s->vlistValid = false;
     s->recFresh = false;

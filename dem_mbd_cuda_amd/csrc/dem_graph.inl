@@ -119,6 +119,13 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
     }
     while (k < count) TRY(step1());
     *done += k;
+    // replayed steps read nothing back: the span ends with the range count of
+    // all its evaluations
+    HIP_TRY(hipMemcpyAsync(&s->hScalars->grid_overflow, &s->sc->grid_overflow, sizeof(int), hipMemcpyDeviceToHost,
+                           s->stream),
+            "read range count");
+    HIP_TRY(hipStreamSynchronize(s->stream), "read range count");
+    if (range_seen(s)) return range_error(s);
     return DEM_OK;
 }
 

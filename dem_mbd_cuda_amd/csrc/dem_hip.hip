@@ -163,7 +163,7 @@ struct Scalars {
     int nbits;
     int mbits;          // key bits of the Morton row order (candidate-list rebuilds)
     int dense;
-    int grid_overflow;
+    int grid_overflow;  // components out of the 21-bit cell range; kept across steps until the host clears it
     int list_overflow;
     int pp_overflow, pw_overflow, pt_overflow;  // pw+pt feed the "body" counter
     int pp_high, body_high;
@@ -365,7 +365,7 @@ __device__ __forceinline__ void reset_step_scalars(Scalars* sc, int* sceneAcc, i
 {
     for (int i = threadIdx.x; i < 6 * nscenes; i += BLOCK) sceneAcc[i] = (i % 6) < 3 ? INT_MAX : INT_MIN;
     if (threadIdx.x == 0) {
-        sc->grid_overflow = 0;
+        // grid_overflow is not reset here: it stays set until the host reads it
         sc->list_overflow = 0;
         sc->need_rebuild = 0;
         sc->cand_overflow = 0;
@@ -2658,6 +2658,7 @@ struct dem_solver {
     long long nEvaluations = 0, nSorts = 0, nGraphSteps = 0, nEagerSteps = 0;
     bool lastSorted = false;  // the last evaluation sorted the grid on the device
     bool verletBroken = false;  // a rebuild overflowed CAND_MAX: lists stay off
+    bool keepRows = false;      // list rebuilds leave the rows in place (differentiable runs)
     bool capturing = false;   // inside a HIP graph capture: no event records
     bool countAlways = false; // per-step counters on every step (differentiable runs read them)
     GraphCache* graphs = nullptr;
@@ -3551,6 +3552,25 @@ static int await_mirror(dem_solver* s, int seq)
     return DEM_OK;
 }
 
+// The range count of k_cells stays set on the device until the host reads it
+// here: every copy of the counters still in flight lands first (so none can
+// bring the old count back), then the count is cleared on both sides.
+static int range_error(dem_solver* s)
+{
+    HIP_TRY(hipStreamSynchronize(s->stream), "range check");
+    HIP_TRY(hipMemsetAsync(&s->sc->grid_overflow, 0, sizeof(int), s->stream), "clear range count");
+    HIP_TRY(hipStreamSynchronize(s->stream), "clear range count");
+    __atomic_store_n(&s->hScalars->grid_overflow, 0, __ATOMIC_RELAXED);
+    return set_error(DEM_E_CAPACITY, "component grid coordinate exceeds signed 21-bit hash range");
+}
+
+// Whether the last copy of the counters that reached the host carries a range
+// count (a copy still in flight shows an older, equally valid count).
+static bool range_seen(const dem_solver* s)
+{
+    return !s->capturing && __atomic_load_n(&s->hScalars->grid_overflow, __ATOMIC_RELAXED) != 0;
+}
+
 // dem_compute_forces; noalloc: the triangle grid's per-step buffers come from
 // the device's scratch array instead of the solver's own (grown on demand).
 static int compute_forces_impl(dem_solver* s, const double* pose13, double* loads6, int collect, bool noalloc)
@@ -3560,11 +3580,9 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     hipStream_t st = s->stream;
     const long n = s->n;
     const int nb = s->nb;
-    // a range error seen on an earlier step aborts here (reference throws at the step)
-    if (s->hScalars->grid_overflow != 0) {
-        s->hScalars->grid_overflow = 0;
-        return set_error(DEM_E_CAPACITY, "component grid coordinate exceeds signed 21-bit hash range");
-    }
+    // a range error of an earlier step whose counters reached the host after
+    // that step returned aborts here (the reference throws at the step)
+    if (range_seen(s)) return range_error(s);
     if (pose13 && nb > 0) {
         std::memcpy(s->hPose, pose13, sizeof(double) * 13 * nb);
         HIP_TRY(hipMemcpyAsync(s->pose, s->hPose, sizeof(double) * 13 * nb, hipMemcpyHostToDevice, st),
@@ -3638,10 +3656,11 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             s->hScalars->grid_overflow = s->hMirror[2];
             scanned = !rebuild;
         } else {
-            HIP_TRY(hipMemcpyAsync(s->hFlag, &s->sc->need_rebuild, sizeof(int), hipMemcpyDeviceToHost, st),
+            // the flag with the range count of k_cells
+            HIP_TRY(hipMemcpyAsync(s->hScalars, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st),
                     "read displacement flag");
             HIP_TRY(hipStreamSynchronize(st), "read displacement flag");
-            rebuild = s->hFlag[0] != 0;
+            rebuild = s->hScalars->need_rebuild != 0;
         }
     }
     bool triDone = false;
@@ -3668,8 +3687,9 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
         hipLaunchKernelGGL(k_fixup, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, s->keyA, s->valA, nr,
                            R.i + (long)I_ORIG * nr);
         // with candidate lists the rows are put in cell order at every rebuild
-        const bool reorder = !s->clump && (verlet || (s->reorderInterval > 0 &&
-                                                      (s->stepIndex % s->reorderInterval) == 0));
+        // (unless they must stay in place)
+        const bool reorder = !s->clump && ((verlet && !s->keepRows) ||
+                                           (s->reorderInterval > 0 && (s->stepIndex % s->reorderInterval) == 0));
         if (reorder) {
             Soa D{s->dState[1 - s->cur], s->iState[1 - s->cur], n};
             if (verlet) {
@@ -3835,10 +3855,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     if (collect) {
         HIP_TRY(hipStreamSynchronize(st), "collect diagnostics");
         const Scalars& h = *s->hScalars;
-        if (h.grid_overflow != 0) {
-            s->hScalars->grid_overflow = 0;
-            return set_error(DEM_E_CAPACITY, "component grid coordinate exceeds signed 21-bit hash range");
-        }
+        if (h.grid_overflow != 0) return range_error(s);
         dem_step_stats stt{};
         stt.particle_contacts = h.particle_contacts;
         stt.body_contacts = h.body_contacts;
@@ -3878,6 +3895,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     }
     if (needLoads) {
         HIP_TRY(hipStreamSynchronize(st), "download body resultant forces");
+        if (range_seen(s)) return range_error(s);
         const std::vector<double>& mass = s->hBodyMass;
         for (int b = 0; b < nb; ++b) {
             for (int k = 0; k < 6; ++k) loads6[6 * b + k] = s->hLoads[6 * b + k];
@@ -3888,6 +3906,9 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             }
         }
     }
+    // list steps have the count of this step already (it came with the
+    // displacement flag); other steps report it once their copy has landed
+    if (range_seen(s)) return range_error(s);
     return DEM_OK;
 }
 

@@ -119,6 +119,75 @@ def test_cell_range_error(hip, oracle):
     assert msgs[0] == msgs[1] and "21-bit" in msgs[0]
 
 
+def _runaway():
+    """A sphere 2,000 m out moving at 1e8 m/s (100 m a step): with 2.5 mm
+    cells it leaves the signed 21-bit range (2,621 m) on the seventh step."""
+    P = scene.sphere_particles([[0.0, 0.0, 0.0], [2000.0, 0.0, 0.0]], [1e-3, 1e-3], density=2500.0)
+    P["vel"] = np.array([[0.0, 0.0, 0.0], [1.0e8, 0.0, 0.0]])
+    return scene.Scene(**P, pp=MECH, pt=dict(MECH), pw=dict(MECH), dt=1e-6, steps=0)
+
+
+def _step_until_error(s, sc, steps, collect_last):
+    """Initial evaluation and `steps` steps without diagnostics (the last one
+    with diagnostics if asked); returns (step index of the error, text)."""
+    s.initialize(sc)
+    s.compute_forces(collect=False)
+    s.init_half_step(0.5 * sc.dt)
+    for k in range(1, steps + 1):
+        s.advance(sc.dt)
+        try:
+            s.compute_forces(collect=collect_last and k == steps)
+        except DemError as e:
+            return k, str(e)
+        s.finish(sc.dt)
+    return None, None
+
+
+@pytest.mark.parametrize("skin", ["0.2", "0"])
+def test_cell_range_error_mid_run(hip, oracle, monkeypatch, skin):
+    """A sphere leaves the 21-bit range on a step without diagnostics.  The
+    oracle (like the reference) fails at that step.  With candidate lists the
+    count comes back with the displacement flag, so the HIP path fails at the
+    same step; on full-scan steps it fails once the step's counters reach the
+    host, at the latest on the next diagnostics step."""
+    monkeypatch.setenv("DEM_SKIN", skin)
+    sc = _runaway()
+    so = oracle.solver()
+    ko, eo = _step_until_error(so, sc, 12, collect_last=True)
+    so.close()
+    assert ko == 7 and "21-bit" in eo
+    s = hip.solver()
+    kg, eg = _step_until_error(s, sc, 12, collect_last=True)
+    assert eg == eo
+    if skin != "0":
+        assert kg == ko
+    else:
+        assert ko <= kg <= 12
+    s.close()
+
+
+def test_cell_range_error_in_graph_run(hip, oracle, monkeypatch):
+    """The same runaway sphere in a multi-step run of graph replays (full
+    scan, no bodies): the run fails at its end with the reference's message,
+    having run its steps; the oracle fails at the seventh step."""
+    monkeypatch.setenv("DEM_SKIN", "0")
+    sc = _runaway()
+    msgs = []
+    for lib in (hip, oracle):
+        s = lib.solver()
+        s.initialize(sc)
+        s.compute_forces(collect=False)
+        s.init_half_step(0.5 * sc.dt)
+        with pytest.raises(DemError) as e:
+            s.run_steps(40, sc.dt, graph=True)
+        msgs.append(str(e.value))
+        if lib is hip:
+            c = s.run_counters()
+            assert c["graph_steps"] > 0, c
+        s.close()
+    assert msgs[0] == msgs[1] and "21-bit" in msgs[0]
+
+
 def test_stage_timings_only_when_enabled(hip):
     """Stage events are recorded only while enabled: the last evaluation made
     with timings on reports five non-negative stage times, and nothing is
