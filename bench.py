@@ -4,10 +4,13 @@ Workload: pure DEM, 1,000,000 spheres, radius lognormal (median 1 mm,
 sigma 0.2, clipped to [0.5, 2] mm), density 2500 kg/m^3, kN = E * median
 radius with E = 1e7 Pa, mu 0.5, restitution 0.5, in a 200 x 200 mm hopper
 (floor + 4 walls inclined 10 degrees), gravity -9.81 z, dt = 5e-7 s, grid
-cell = 2 x max radius.  The headline value is the settled state (a
-near-touching bed, about 3.7 stored contacts per sphere; --packing picks
-it), averaged over 1,000 steps; the loose seeded-insertion state (no
-contacts yet) is measured beside it.  A step is the reference's pure-DEM
+cell = 2 x max radius.  The headline value is the settled state (a bed
+built by sequential deposition under gravity, each sphere rolled into a
+hollow of the bed, resting contacts preloaded by 0.2 um; about 5.6 stored
+contacts per sphere at the start, 4.5-5 while the bed sags under its weight
+over the first thousand steps; --packing picks it), averaged over 1,000
+steps; the loose seeded-insertion state (no contacts yet) is measured beside
+it.  A step is the reference's pure-DEM
 loop body: advanceParticles, computeForces, finishParticleStep, all state
 resident in HBM, run through dem_run_steps in the mode its autotuner picks
 (candidate lists with eager launches, or the full scan replayed from
@@ -374,8 +377,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mesh-scale", type=float, default=2.0,
                     help="grid cell in max radii (the 1M configuration states 2)")
-    ap.add_argument("--packing", choices=("dense", "rsa"), default="dense",
-                    help="headline state: settled near-touching bed (dense) or loose seeded insertion (rsa)")
+    ap.add_argument("--packing", choices=("settled", "dense", "rsa"), default="settled",
+                    help="headline state: deposited bed at rest (settled), compressed relaxed lattice "
+                         "(dense, not at rest) or loose seeded insertion (rsa)")
     ap.add_argument("--no-insertion", action="store_true", help="skip the loose-insertion figure")
     ap.add_argument("--timing-steps", type=int, default=20, help="eager steps timed per phase with HIP events")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -417,14 +421,15 @@ def main():
     sc = head["sc"]
     other = None
     if not args.no_insertion:
-        packing = "rsa" if args.packing == "dense" else "dense"
+        packing = "rsa" if args.packing != "rsa" else "settled"
         o = measure_hopper(dm, scene, args, packing, min(args.steps, 500), args.warmup, 5)
         other = {"packing": packing, "value": o["value"], "ms_per_step": o["ms_per_step"],
                  "contacts_per_particle": o["cbar"], "phase_ms": o["phase_ms"], "steps": min(args.steps, 500)}
         del o
 
-    state = ("settled near-touching bed (lattice layers relaxed to contact)" if args.packing == "dense"
-             else "loose seeded random sequential insertion")
+    state = {"settled": "settled bed (sequential deposition under gravity, contacts preloaded 0.2 um)",
+             "dense": "compressed lattice layers partly relaxed (not at rest)",
+             "rsa": "loose seeded random sequential insertion"}[args.packing]
     workload = (f"pure DEM hopper, {sc.n} spheres lognormal r (median 1 mm, sigma 0.2, clipped 0.5-2 mm), "
                 f"floor + 4 walls inclined 10 deg, gravity, dt 5e-7, grid cell {args.mesh_scale} x max radius, "
                 + state)

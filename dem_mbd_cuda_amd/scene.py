@@ -237,6 +237,33 @@ def relax_packing(radius, pos, lo, hi_xy, max_iter=200, tol=1e-3):
     return it
 
 
+def deposit_packing(radius, lo, hi_xy, wall_origin, wall_normal, seed, preload=1e-4, batch=32):
+    """Sequential deposition under gravity (see dem_deposit_packing): the
+    spheres, in the given order, dropped at seeded (x, y) over the floor
+    footprint [lo, hi_xy] and rolled down into a hollow of their supports,
+    with the planes (wall_origin, wall_normal) as obstacles.  Resting
+    contacts overlap by `preload` of the radius sum.  `batch` spheres are
+    computed at a time (in parallel); the result is the same for any batch.
+    Returns positions."""
+    lib = _rsa_lib()
+    fn = lib.dem_deposit_packing
+    dp = C.POINTER(C.c_double)
+    fn.argtypes = [C.c_int, dp, dp, dp, C.c_int, dp, dp, C.c_uint64, C.c_double, C.c_int, dp]
+    fn.restype = C.c_int
+    radius = np.ascontiguousarray(radius, np.float64)
+    lo2 = np.ascontiguousarray(np.asarray(lo, float)[:2])
+    hi2 = np.ascontiguousarray(np.asarray(hi_xy, float)[:2])
+    wo = np.ascontiguousarray(wall_origin, np.float64)
+    wn = np.ascontiguousarray(wall_normal, np.float64)
+    pos = np.zeros((len(radius), 3))
+    rc = fn(len(radius), radius.ctypes.data_as(dp), lo2.ctypes.data_as(dp), hi2.ctypes.data_as(dp), len(wo),
+            wo.ctypes.data_as(dp), wn.ctypes.data_as(dp), int(seed) & ((1 << 64) - 1), float(preload),
+            int(batch), pos.ctypes.data_as(dp))
+    if rc != 0:
+        raise RuntimeError("sequential deposition: out of memory")
+    return pos
+
+
 def rsa_positions(radius, lo, hi, seed, max_attempts=2000, gap=0.0):
     radius = np.ascontiguousarray(radius, np.float64)
     lo = np.ascontiguousarray(lo, np.float64)
@@ -298,13 +325,24 @@ def hopper_scene(n=1_000_000, seed=2, width=0.2, fill=0.36, median=1.0e-3, sigma
 
     packing="rsa": seeded random sequential insertion into the hopper's
     footprint column at volume fraction `fill` (largest radii first; loose,
-    no initial contacts).  packing="dense": jittered lattice at `fill`
-    relaxed to a near-touching packing (a settled-bed stand-in with contacts).
+    no initial contacts).  packing="settled": the spheres deposited one by
+    one under gravity, each rolled into a hollow of the bed (sequential
+    deposition, see deposit_packing) with resting contacts preloaded by 1e-4
+    of the radius sum (0.2 um, the order of the weight-induced overlap of
+    the stiffness kN = E * median at mid-depth): a bed at rest with about
+    5.5 contacts per sphere.  packing="dense": size-sorted lattice layers
+    compressed by 1.5 % and partly relaxed; its remaining overlaps (up to
+    tens of um) push the bed apart within a few thousand steps, so it is
+    not at rest.
     """
     rng = np.random.RandomState(seed)
     radius = np.clip(median * np.exp(sigma * rng.standard_normal(n)), rclip[0], rclip[1])
     lo = np.array([-0.5 * width, -0.5 * width, 0.0])
-    if packing == "dense":
+    wo, wn = hopper_walls(0.5 * width, angle_deg)
+    if packing == "settled":
+        pos = deposit_packing(radius, lo, (0.5 * width, 0.5 * width), wo, wn, seed=seed * 7919 + 11,
+                              preload=1e-4)
+    elif packing == "dense":
         radius, pos = layered_lattice(radius, lo, (0.5 * width, 0.5 * width), factor=0.985)
         relax_packing(radius, pos, lo, (0.5 * width, 0.5 * width), max_iter=60, tol=2e-3)
     else:
@@ -319,7 +357,6 @@ def hopper_scene(n=1_000_000, seed=2, width=0.2, fill=0.36, median=1.0e-3, sigma
     kN = E * median
     m = mech(epsS=restitution, mu=mu, epsN=restitution, cohesion=0.0, beta=beta, muT=muT, muR=muR,
              kN=kN)
-    wo, wn = hopper_walls(0.5 * width, angle_deg)
     return Scene(**sphere_particles(pos, radius, density), wall_origin=wo, wall_normal=wn,
                  pp=m, pt=dict(m), pw=dict(m), dt=dt, gravity=tuple(gravity), gravity_enabled=1,
                  mesh_scale=mesh_scale, steps=steps)

@@ -2,10 +2,11 @@
 oracle and (for the batch) against single-scene runs:
 
 * the 1M-sphere hopper (lognormal radii clipped to [0.5, 2] mm, floor and
-  four inclined walls, cell 2 x max radius): the initial evaluation and two
-  steps bit for bit -- cell keys and the device's own sorted order of the
-  initial evaluation, contact-pair sets, history tables, states, forces;
-* a 30,000-sphere hopper with the same radii, walls and cell, agitated, over
+  four inclined walls, cell 2 x max radius) in its settled bed (sequential
+  deposition): the initial evaluation and two steps bit for bit -- cell keys
+  and the device's own sorted order of the initial evaluation, contact-pair
+  sets, history tables, states, forces;
+* a 30,000-sphere settled hopper bed with the same radii, walls and cell, agitated, over
   100 steps with a small candidate-list skin, so that lists are rebuilt
   several times (each rebuild sorts on the device), bit for bit;
 * all 256 scenes of the batched parameter sweep (N ~ U[500, 20000], per-scene
@@ -25,7 +26,7 @@ pytestmark = pytest.mark.gpu
 def test_hopper_1m_initial_evaluation_and_two_steps(hip, oracle):
     from dem_mbd_cuda_amd import scene
 
-    sc = scene.hopper_scene(n=1_000_000, packing="dense", mesh_scale=2.0, steps=2)
+    sc = scene.hopper_scene(n=1_000_000, packing="settled", mesh_scale=2.0, steps=2)
     print("1M hopper built", flush=True)
     g = run_frames(hip, sc, steps=2, interval=2, debug=True)
     print("HIP run done", flush=True)
@@ -40,7 +41,7 @@ def test_hopper_lists_rebuilt_on_the_headline_cell(hip, oracle, monkeypatch):
     from dem_mbd_cuda_amd import scene
 
     monkeypatch.setenv("DEM_SKIN", "0.0005")  # rebuild threshold 0.5 um
-    sc = scene.hopper_scene(n=30_000, width=0.2 * 0.03 ** 0.5, packing="dense", mesh_scale=2.0, steps=100)
+    sc = scene.hopper_scene(n=30_000, width=0.2 * 0.03 ** 0.5, packing="settled", mesh_scale=2.0, steps=100)
     rng = np.random.RandomState(41)
     sc.vel = rng.uniform(-0.05, 0.05, (sc.n, 3))
     sc.angvel = rng.uniform(-50.0, 50.0, (sc.n, 3))
