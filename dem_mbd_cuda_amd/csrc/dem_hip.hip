@@ -2644,7 +2644,6 @@ struct dem_solver {
     double* refC = nullptr;    // [3][n] centres at the last rebuild
     int* cand = nullptr;       // [CAND_MAX][n]
     int* ncand = nullptr;      // [n]
-    uint32_t* nkey = nullptr;  // [MAX_LIST][n] order keys of the current contacts
     int4* meta = nullptr;      // [n] cell and input row in cell order
     int* hFlag = nullptr;      // pinned: displacement / overflow flags
     int* hMirror = nullptr;    // pinned, coherent: [seq, need_rebuild, grid_overflow] written by k_vscan
@@ -2792,7 +2791,7 @@ struct dem_solver {
         diff_release(diff);
         graph_release(graphs);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB, ident, mKey, mVal,
-                        rec, recAux, refC, cand, ncand, nkey, meta, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
+                        rec, recAux, refC, cand, ncand, meta, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, rowF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
@@ -3162,7 +3161,6 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
             TRY(dalloc(s->refC, (size_t)3 * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->cand, (size_t)CAND_MAX * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->ncand, n, "hipMalloc(candidate lists)"));
-            TRY(dalloc(s->nkey, (size_t)MAX_LIST * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->meta, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->ident, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->mKey, n, "hipMalloc(candidate lists)"));
@@ -3629,7 +3627,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
                 if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
                 hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ident, s->posr,
                                    s->meta, s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr,
-                                   s->nkey, s->ncount, s->tape.ppOcc, collect || s->countAlways, s->listBlk,
+                                   s->ncount, s->tape.ppOcc, collect || s->countAlways, s->listBlk,
                                    (const int*)&s->sc->need_rebuild, (int*)nullptr, 0);
                 TRY(launch_check("contact scan"));
                 HIP_TRY(hipEventSynchronize(s->flagEv), "read displacement flag");
@@ -3648,7 +3646,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             const int seq = ++s->mirrorSeq;
             if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
             hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ident, s->posr, s->meta,
-                               s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr, s->nkey, s->ncount,
+                               s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr, s->ncount,
                                s->tape.ppOcc, collect || s->countAlways, s->listBlk, skip, s->dMirror, seq);
             TRY(launch_check("contact scan"));
             TRY(await_mirror(s, seq));
@@ -3767,7 +3765,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
         ;
     else if (useLists)
         hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->ident, s->posr, s->meta, s->cidS,
-                           s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->nkey, s->ncount, s->tape.ppOcc,
+                           s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
                            counters, s->listBlk, (const int*)nullptr, (int*)nullptr, 0);
     else if (s->clump)
         hipLaunchKernelGGL(k_scan<true>, dim3(nblk), dim3(BLOCK), 0, st, R, s->valA, s->posr, s->cidS, s->keyA,

@@ -69,11 +69,20 @@ __global__ void __launch_bounds__(BLOCK) k_vbuild(Soa S, const double4* posr, co
     ncand[t] = min(count, CAND_MAX);
 }
 
+// The reference's candidate order of a contact of a component in cell
+// `mine` with one in cell `other` (at most one cell apart on each axis):
+// cell offset (dx, dy, dz) lexicographic, then input row.
+__device__ __forceinline__ uint32_t order_key(int4 mine, int4 other)
+{
+    const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
+    return ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) | (uint32_t)other.w;
+}
+
 // Step between rebuilds: exact contact test over the listed partners, the
 // reference's candidate order, then history slots (assign_slots).
 __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const double4* posr, const int4* meta,
                                                  const uint32_t* cidS, const int* cand, const int* ncand,
-                                                 Scalars* sc, History H, int step, int* nbr, uint32_t* nkey,
+                                                 Scalars* sc, History H, int step, int* nbr,
                                                  int* ncount, unsigned long long* occTape, bool counters,
                                                  unsigned* bsum, const int* skip, int* mirror, int seq)
 {
@@ -89,6 +98,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
     const long t = b * BLOCK + threadIdx.x;
     const long n = S.n;
     int found = 0, stored = 0, high = 0, over = 0;
+    uint32_t last = 0;  // order key of the last contact in the list
     if (t < n) {
         if (S.k(I_VALID, t)) {
             const double4 me = posr[t];
@@ -116,22 +126,29 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
                     const int4 other = meta[k];
                     const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
                     if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
-                    const uint32_t kk = ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) |
-                                        (uint32_t)other.w;
+                    const uint32_t kk = order_key(mine, other);
                     if (found >= MAX_LIST) {
                         ++found;
                         continue;
                     }
-                    // insertion into the ordered list (nearly sorted: lists keep
-                    // their order from step to step)
-                    int pos = found;
-                    while (pos > 0 && nkey[(long)(pos - 1) * n + t] > kk) {
-                        nkey[(long)pos * n + t] = nkey[(long)(pos - 1) * n + t];
-                        nbr[(long)pos * n + t] = nbr[(long)(pos - 1) * n + t];
-                        --pos;
+                    // the contacts in order: candidate lists keep the order of
+                    // the rebuild, so a contact almost always comes after the
+                    // last one (kept in a register); otherwise it is inserted,
+                    // the keys of the entries it passes recomputed from their
+                    // cell records
+                    if (found == 0 || kk > last) {
+                        nbr[(long)found * n + t] = (int)k;
+                        last = kk;
+                    } else {
+                        int pos = found;
+                        while (pos > 0) {
+                            const int kp = nbr[(long)(pos - 1) * n + t];
+                            if (order_key(mine, meta[kp]) <= kk) break;
+                            nbr[(long)pos * n + t] = kp;
+                            --pos;
+                        }
+                        nbr[(long)pos * n + t] = (int)k;
                     }
-                    nkey[(long)pos * n + t] = kk;
-                    nbr[(long)pos * n + t] = (int)k;
                     ++found;
                 }
             }
@@ -160,8 +177,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
                         const int4 other = meta[k];
                         const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
                         if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
-                        const uint32_t kk = ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) |
-                                            (uint32_t)other.w;
+                        const uint32_t kk = order_key(mine, other);
                         if ((r == 0 || kk > last) && kk < best) {
                             best = kk;
                             bk = k;
