@@ -1314,11 +1314,11 @@ __device__ __forceinline__ bool row_in_chunk(long base, int cnt)
 // ROWS: candidate-list steps, where the list's owner index is the row and
 // every row's history sits in its own row (the identity order and I_HIST of
 // the rebuilds), so no index is looked up.  ONE: a single scene, whose law
-// is uniform across the grid (scalar loads).
+// is uniform across the grid, passed by value (scalar registers).
 template <bool ROWS, bool ONE>
 __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs rec,
                                                  const int2* pairs, const unsigned* off, const int* ncount,
-                                                 History H, const Material* mats,
+                                                 History H, const Material* mats, Material mat1,
                                                  bool first, bool offs, const Scalars* sc, double* pairF,
                                                  double* rowF, int2* tapePair, double* tapePrev)
 {
@@ -1326,34 +1326,50 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs 
     const long total = sc->total_pairs;
     const int lane = threadIdx.x & 63;
     double* ws = stage + (threadIdx.x - lane) * 6;  // this wave's 64 results
-    for (long e0 = (long)blockIdx.x * BLOCK; e0 < total; e0 += (long)gridDim.x * BLOCK) {
-        const long e = e0 + threadIdx.x;
+    const long stride = (long)gridDim.x * BLOCK;
+    long e = (long)blockIdx.x * BLOCK + threadIdx.x;
+    // the list entry of the lane's next contact is fetched one iteration
+    // ahead, so each iteration starts with its gathers
+    int2 next = e < total ? pairs[e] : make_int2(0, 0);
+    for (long e0 = (long)blockIdx.x * BLOCK; e0 < total; e0 += stride, e += stride) {
         V3 cf = zero3(), tq = zero3();
         long row = -1;
         int len = 0;
+        const int2 pr = next;
         if (e < total) {
-        const int2 pr = pairs[e];
         const long p = ROWS ? (long)pr.x : (long)sorted[pr.x];
         const long q = pr.y >> SLOT_BITS;
         const int slot = pr.y & (PP_SLOTS - 1);
         Body I = rec_body(rec, p);
         if (offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
         const Body J = rec_body(rec, q);
+        const long o = ROWS ? p : (long)S.k(I_HIST, p);
+        Material m = ONE ? mat1 : mats[rec.aux[p].scene];
+        if (first) m.dt = 0.0;  // the first evaluation advances no history
+        // the slot's nine values are read once, kept in registers through
+        // the law and written back once
+        double* hv = pp_slot(H, o, slot);
+        double h[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) h[k] = hv[k];
+        long base = 0;
+        int cnt = 0;
+        if (rowF) {
+            base = off[pr.x];
+            cnt = ncount[pr.x];
+        }
+        if (e + stride < total) next = pairs[e + stride];
         const V3 delta = sub(J.pos, I.pos);
         const double d2 = dot(delta, delta);
-        const long o = ROWS ? p : (long)S.k(I_HIST, p);
-        Material m = ONE ? mats[0] : mats[rec.aux[p].scene];
-        if (first) m.dt = 0.0;  // the first evaluation advances no history
-        double* hv = pp_slot(H, o, slot);
         if (tapePair) {
             tapePair[e] = make_int2((int)p, pr.y);
-            for (int k = 0; k < 9; ++k) tapePrev[e * 9 + k] = hv[k];
+            for (int k = 0; k < 9; ++k) tapePrev[e * 9 + k] = h[k];
         }
-        pp_law(I, J, delta, d2, m, hv, cf, tq);
+        pp_law(I, J, delta, d2, m, h, cf, tq);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) hv[k] = h[k];
         bool whole = false;
         if (rowF) {
-            const long base = off[pr.x];
-            const int cnt = ncount[pr.x];
             whole = row_in_chunk(base, cnt);
             if (whole && e == base) {
                 row = pr.x;
@@ -2701,6 +2717,7 @@ struct dem_solver {
     SceneParams* dSp = nullptr;
     SceneGrid* dGrid = nullptr;
     Material* dMatPP = nullptr;
+    Material hMatPP0{};  // the first scene's particle-particle law (k_pairs of one scene)
     Material* dMatPW = nullptr;
     std::vector<SceneParams> hsp;
     unsigned long long cellCap = 0;
@@ -3139,6 +3156,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     TRY(dalloc(s->dMatPW, nsc, "hipMalloc(scene materials)"));
     HIP_TRY(hipMemcpy(s->dSp, s->hsp.data(), sizeof(SceneParams) * nsc, hipMemcpyHostToDevice), "upload scenes");
     HIP_TRY(hipMemcpy(s->dMatPP, hpp.data(), sizeof(Material) * nsc, hipMemcpyHostToDevice), "upload scenes");
+    s->hMatPP0 = hpp[0];
     HIP_TRY(hipMemcpy(s->dMatPW, hpw.data(), sizeof(Material) * nsc, hipMemcpyHostToDevice), "upload scenes");
     TRY(dalloc(s->nbr, (size_t)MAX_LIST * nr, "hipMalloc(contact lists)"));
     TRY(dalloc(s->ncount, nr, "hipMalloc(contact counts)"));
@@ -3784,7 +3802,8 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
         auto* kp = useLists ? (s->nscenes == 1 ? k_pairs<true, true> : k_pairs<true, false>)
                             : (s->nscenes == 1 ? k_pairs<false, true> : k_pairs<false, false>);
         hipLaunchKernelGGL(kp, dim3(s->pairBlocks), dim3(BLOCK), 0, st, R, ord, s->recs(), s->pairs,
-                           (const unsigned*)s->listOff, (const int*)s->ncount, s->H, s->dMatPP, step == 0,
+                           (const unsigned*)s->listOff, (const int*)s->ncount, s->H, s->dMatPP, s->hMatPP0,
+                           step == 0,
                            s->offs, s->sc, s->pairF, s->rowF, s->tape.pair, s->tape.pairPrev);
     }
     TRY(launch_check("particle contact law"));
