@@ -38,10 +38,15 @@ void graph_release(GraphCache* g)
 
 namespace {
 
-int eager_step(dem_solver* s, double dt)
+// One step of a multi-step run; `more`: another step of the run follows, so
+// the law kernel may finish this step and advance the next one itself.
+int eager_step(dem_solver* s, double dt, bool more)
 {
     TRY(dem_advance_particles(s, dt, nullptr));
-    TRY(dem_compute_forces(s, nullptr, nullptr, 0));
+    s->nextDt = more ? dt : 0.0;
+    const int rc = dem_compute_forces(s, nullptr, nullptr, 0);
+    s->nextDt = 0.0;
+    TRY(rc);
     return dem_finish_particle_step(s, dt, 0.5 * dt);
 }
 
@@ -56,7 +61,7 @@ int capture_steps(dem_solver* s, double dt, int period, hipGraphExec_t* out)
     HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
     s->capturing = true;
     int rc = DEM_OK;
-    for (int k = 0; k < period && rc == DEM_OK; ++k) rc = eager_step(s, dt);
+    for (int k = 0; k < period && rc == DEM_OK; ++k) rc = eager_step(s, dt, k + 1 < period);
     s->capturing = false;
     const hipError_t e = hipStreamEndCapture(s->stream, &graph);
     TRY(rc);
@@ -77,8 +82,8 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
 {
     GraphCache& G = *s->graphs;
     int k = 0;
-    auto step1 = [&]() {
-        TRY(eager_step(s, dt));
+    auto step1 = [&](bool more) {
+        TRY(eager_step(s, dt, more));
         ++s->nEagerSteps;
         ++k;
         return DEM_OK;
@@ -86,7 +91,7 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
     // the first evaluation after initialisation or a load runs eagerly; a
     // captured sequence starts with a finish pending (its first kernel is the
     // fused finish + advance), as every replay does
-    while (k < count && (s->stepIndex == 0 || !s->pendingFinish)) TRY(step1());
+    while (k < count && (s->stepIndex == 0 || !s->pendingFinish)) TRY(step1(false));
     const bool graphs = allowGraphs && s->nb == 0 && !s->verlet;
     if (graphs) {
         const int P = graph_period(s);
@@ -117,7 +122,7 @@ int run_span(dem_solver* s, double dt, int count, bool allowGraphs, int* done)
             s->nGraphSteps += P;
         }
     }
-    while (k < count) TRY(step1());
+    while (k < count) TRY(step1(k + 1 < count));
     *done += k;
     // replayed steps read nothing back: the span ends with the range count of
     // all its evaluations

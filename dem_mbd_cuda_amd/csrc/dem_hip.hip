@@ -16,7 +16,8 @@
 //   k_posr       packed (x, y, z, r) and component id in cell order
 //   k_pack       per-particle contact record (one 160-byte line per row)
 //   k_scan       27-cell candidate scan, nine contiguous column ranges per
-//                particle, then history expiry and one slot per contact
+//                particle, the block's neighbourhood staged in LDS column by
+//                column, then history expiry and one slot per contact
 //   k_off_*      prefix sum of list lengths, compacted (owner, entry) list
 //   k_pairs      particle-particle law, one thread per contact
 //   k_tri_sift   (coupled runs) rows whose cell holds no triangle entry take
@@ -1051,6 +1052,9 @@ struct LawArgs {
     int nb;
     Tape tape;
     bool counters;  // per-step statistics (contacts, high-water marks) wanted
+    double finishDt, nextDt;  // k_law<.., true>: this step's finish and the next step's advance
+    int* sceneAcc;            // k_law<.., true>: per-scene grid boxes reset for the next evaluation
+    int nscenes;
 };
 
 // History lookup with the first 16 keys of the owner held in registers (kept
@@ -1099,41 +1103,56 @@ __device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long
 }
 
 // History expiry at the start of the owner's step (expireHistory,
-// contact:217-229) and one slot per contact in list order
-// (acquireHistorySlot, contact:190-215).  nbr[c][t] holds the sorted index of
-// the c-th contact on entry and (partner row << SLOT_BITS | slot) on return,
-// contacts without a slot (overflow) dropped; returns the entries kept.
-__device__ __forceinline__ int assign_slots(const Soa& S, long t, long p, int found, const int* sorted,
-                                            const uint32_t* cidS, const History& H, int step, int* nbr,
-                                            unsigned long long* occTape, int& high, int& over)
+// contact:217-229; the caller reads the row's expired occupancy and first 16
+// keys ahead of its scan, row_history) and one slot per contact in list order
+// (acquireHistorySlot, contact:190-215).  nbr[c][t] holds the sorted position
+// of the c-th contact on entry and (partner row << SLOT_BITS | slot) on
+// return, contacts without a slot (overflow) dropped; returns the entries
+// kept.  The contacts' ids and rows are gathered four at a time.
+template <typename Order>
+__device__ __forceinline__ int assign_batched(long t, long n, long o, int found, unsigned long long occ,
+                                              uint32_t (&k16)[16], const Order& sorted, const uint32_t* cidS,
+                                              const History& H, int step, int* nbr, int& high, int& over)
 {
-    const long n = S.n;
-    const long o = S.k(I_HIST, p);
-    unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
-    if (occTape) occTape[o] = occ;
     unsigned long long cur = 0;
     uint32_t* keys = H.ppKey + o * PP_SLOTS;
     double* vals = pp_slot(H, o, 0);
-    uint32_t k16[16];
-    {
-        const uint4* kv = reinterpret_cast<const uint4*>(keys);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint4 v = kv[i];
-            k16[4 * i] = v.x; k16[4 * i + 1] = v.y; k16[4 * i + 2] = v.z; k16[4 * i + 3] = v.w;
-        }
-    }
     int stored = 0;
-    for (int c = 0; c < found; ++c) {
-        const long k = nbr[(long)c * n + t];
-        const int slot = acquire_pp(occ, cur, k16, keys, vals, H.rows * 9, cidS[k] + 1u, high, over);
-        if (slot < 0) continue;  // slot overflow: counted, no force
-        nbr[(long)stored * n + t] = (sorted[k] << SLOT_BITS) | slot;
-        ++stored;
+    for (int c0 = 0; c0 < found; c0 += 4) {
+        int kq[4], rq[4];
+        uint32_t cq[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) kq[j] = nbr[(long)min(c0 + j, found - 1) * n + t];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cq[j] = cidS[kq[j]];
+            rq[j] = sorted[kq[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (c0 + j >= found) break;
+            const int slot = acquire_pp(occ, cur, k16, keys, vals, H.rows * 9, cq[j] + 1u, high, over);
+            if (slot < 0) continue;  // slot overflow: counted, no force
+            nbr[(long)stored * n + t] = (rq[j] << SLOT_BITS) | slot;
+            ++stored;
+        }
     }
     H.ppOcc[o] = occ;
     H.ppSeen[(long)(step & 1) * n + o] = cur;
     return stored;
+}
+
+// The row's expired occupancy and first 16 history keys.
+__device__ __forceinline__ unsigned long long row_history(const History& H, long n, long o, int step,
+                                                          uint32_t (&k16)[16])
+{
+    const uint4* kv = reinterpret_cast<const uint4*>(H.ppKey + o * PP_SLOTS);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint4 v = kv[i];
+        k16[4 * i] = v.x; k16[4 * i + 1] = v.y; k16[4 * i + 2] = v.z; k16[4 * i + 3] = v.w;
+    }
+    return H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
 }
 
 // More overlapping partners than a contact list holds (more than there are
@@ -1142,8 +1161,15 @@ __device__ __forceinline__ int assign_slots(const Soa& S, long t, long p, int fo
 // candidate order, as the reference's loop does; a contact left without a
 // slot is counted (pp_overflow) and dropped.  At most PP_SLOTS contacts can
 // hold a slot, so the list never overflows.
-template <typename Visit>
-__device__ __forceinline__ int assign_inline(const Soa& S, long t, long p, const int* sorted, const uint32_t* cidS,
+// The identity as a sorted order (candidate-list steps: rows are their own
+// sorted positions), for the helpers that map a sorted position to a row.
+struct IdentOrder {
+    __device__ int operator[](long k) const { return (int)k; }
+};
+__device__ __forceinline__ IdentOrder ident_order() { return IdentOrder{}; }
+
+template <typename Order, typename Visit>
+__device__ __forceinline__ int assign_inline(const Soa& S, long t, long p, const Order& sorted, const uint32_t* cidS,
                              const History& H, int step, int* nbr, unsigned long long* occTape, int& high,
                              int& over, Visit&& visit)
 {
@@ -1213,43 +1239,169 @@ __device__ __forceinline__ void scan_overlaps(const Soa& S, long t, long p, int 
 // particle-particle history: expiry at the start of the owner's step
 // (expireHistory, contact:217-229) and one slot per contact in candidate
 // order (acquireHistorySlot, contact:190-215).  The scan first records the
-// overlapping candidates, so the slot lookups (dependent loads) run once per
-// contact rather than once per candidate iteration of the wave.  Leaves the
-// owner's contact list as entries (partner row << SLOT_BITS | slot), k-major.
-// CLUMP: rows are clump components; partners of the row's own owner are
-// skipped (computeContactsKernel compares owners, contact:1138).
+// overlapping candidates, so the slot lookups run once per contact rather
+// than once per candidate iteration of the wave.  Leaves the owner's contact
+// list as entries (partner row << SLOT_BITS | slot), k-major.  CLUMP: rows
+// are clump components; partners of the row's own owner are skipped
+// (computeContactsKernel compares owners, contact:1138).
+//
+// The block's cell neighbourhood is staged in LDS.  A block's 256 sorted
+// rows lie in a few (x, y) columns of the grid (segments, each a run of z
+// cells).  The scan runs in three passes, dx = -1, 0, 1; each
+// pass first copies, for every segment and dy = -1..1, the sorted range of
+// column (x + dx, y + dy) over the segment's z cells +-1 into LDS (centre and
+// radius; with clumps also the owner), then every row tests its three
+// columns of that pass from LDS.  The passes and columns come in the
+// reference's cell order, so the contacts are found in the reference's
+// order.  A block with more segments than STAGE_SEGS, a pass with more
+// entries than STAGE_CAP, or a grid without the dense cell table reads its
+// candidates from HBM instead.
+constexpr int STAGE_SEGS = 4;
+constexpr int STAGE_CAP = 1024;
 template <bool CLUMP>
 __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const double4* posr,
-                                                const uint32_t* cidS, const unsigned long long* key,
-                                                const unsigned* start, const SceneGrid* grid, Scalars* sc,
-                                                History H, int step, int* nbr, int* ncount,
-                                                unsigned long long* occTape, bool counters, const int* ownS,
-                                                unsigned* bsum)
+                                                    const uint32_t* cidS, const unsigned long long* key,
+                                                    const unsigned* start, const SceneGrid* grid, Scalars* sc,
+                                                    History H, int step, int* nbr, int* ncount,
+                                                    unsigned long long* occTape, bool counters, const int* ownS,
+                                                    unsigned* bsum)
 {
+    __shared__ double4 stage[STAGE_CAP];
+    __shared__ int stageOwn[CLUMP ? STAGE_CAP : 1];
+    __shared__ unsigned long long colOf[BLOCK];
+    __shared__ int segX[STAGE_SEGS], segY[STAGE_SEGS], segZa[STAGE_SEGS], segZb[STAGE_SEGS], segScene[STAGE_SEGS];
+    __shared__ long rngB[3 * STAGE_SEGS];
+    __shared__ int rngN[3 * STAGE_SEGS], rngOff[3 * STAGE_SEGS + 1];
+    __shared__ int nseg, staged;
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
-    int found = 0, stored = 0, high = 0, over = 0;
+    const bool dense = sc->dense;
+    long p = -1, hrow = 0;
+    int mine = 0;
+    bool valid = false;
+    long cx = 0, cy = 0, cz = 0, X = 0, Y = 0, Z = 0;
+    unsigned long long base = 0;
+    unsigned long long occ = 0;
+    uint32_t k16[16];
     if (t < n) {
-        const long p = sorted[t];
-        const int mine = CLUMP ? ownS[t] : 0;
-        const bool valid = S.k(I_VALID, p) != 0;
-        const bool dense = sc->dense;
-        if (valid)
-            scan_overlaps<CLUMP>(S, t, p, mine, posr, ownS, key, start, grid, dense, [&](long k) {
-                if (found < MAX_LIST) nbr[(long)found * n + t] = (int)k;
-                ++found;
-            });
-        if (found > MAX_LIST)
+        p = sorted[t];
+        mine = CLUMP ? ownS[t] : 0;
+        valid = S.k(I_VALID, p) != 0;
+        // the row's history state, read ahead of the scan
+        hrow = S.k(I_HIST, p);
+        occ = row_history(H, n, hrow, step, k16);
+        if (valid) {
+            const SceneGrid& g = grid[S.k(I_SCENE, p)];
+            X = (long)g.dims[0]; Y = (long)g.dims[1]; Z = (long)g.dims[2];
+            cx = S.k(I_CX, p) - g.lo[0]; cy = S.k(I_CY, p) - g.lo[1]; cz = S.k(I_CZ, p) - g.lo[2];
+            base = g.base;
+        }
+    }
+    // segments: runs of rows in one (scene, x, y) column
+    const unsigned long long col = valid ? base + ((unsigned long long)cx * Y + cy) * Z : ~0ull;
+    colOf[threadIdx.x] = col;
+    if (threadIdx.x == 0) nseg = 0;
+    __syncthreads();
+    const bool head = valid && (threadIdx.x == 0 || colOf[threadIdx.x - 1] != col);
+    const bool tail = valid && (threadIdx.x == BLOCK - 1 || colOf[threadIdx.x + 1] != col);
+    unsigned nsegs;
+    const unsigned si = block_exclusive_scan(head ? 1u : 0u, &nsegs);
+    const int seg = head ? (int)si : (int)si - 1;  // the row's segment (valid rows)
+    if (head && si < STAGE_SEGS) {
+        segX[si] = (int)cx; segY[si] = (int)cy; segZa[si] = (int)cz;
+        segScene[si] = S.k(I_SCENE, p);
+    }
+    if (tail && seg < STAGE_SEGS) segZb[seg] = (int)cz;
+    const bool useLds = dense && nsegs <= (unsigned)STAGE_SEGS;
+    __syncthreads();
+    int found = 0, stored = 0, high = 0, over = 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        if (useLds) {
+            // the pass's ranges: segment-major, dy = -1, 0, 1
+            if (threadIdx.x < 3 * (int)nsegs) {
+                const int sg = threadIdx.x / 3, dy = threadIdx.x % 3 - 1;
+                const SceneGrid& g = grid[segScene[sg]];
+                const long gX = (long)g.dims[0], gY = (long)g.dims[1], gZ = (long)g.dims[2];
+                const long xx = segX[sg] + dx, yy = segY[sg] + dy;
+                long b = 0, cnt = 0;
+                if (xx >= 0 && xx < gX && yy >= 0 && yy < gY) {
+                    const unsigned long long c0 = g.base + ((unsigned long long)xx * gY + yy) * gZ;
+                    const long zlo = segZa[sg] > 0 ? segZa[sg] - 1 : 0;
+                    const long zhi = segZb[sg] + 1 < gZ ? segZb[sg] + 1 : gZ - 1;
+                    b = start[c0 + zlo];
+                    cnt = (long)start[c0 + zhi + 1] - b;
+                }
+                rngB[threadIdx.x] = b;
+                rngN[threadIdx.x] = (int)cnt;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int o = 0;
+                for (int r = 0; r < 3 * (int)nsegs; ++r) {
+                    rngOff[r] = o;
+                    o += rngN[r];
+                }
+                rngOff[3 * nsegs] = o;
+                staged = o <= STAGE_CAP ? o : -1;
+            }
+            __syncthreads();
+            const int total = staged;
+            for (int i = threadIdx.x; i < total; i += BLOCK) {
+                int r = 0;
+                while (rngOff[r + 1] <= i) ++r;
+                const long k = rngB[r] + (i - rngOff[r]);
+                stage[i] = posr[k];
+                if (CLUMP) stageOwn[i] = ownS[k];
+            }
+            __syncthreads();
+        }
+        if (valid) {
+            const double4 me = posr[t];
+            const long xx = cx + dx;
+            if (xx >= 0 && xx < X) {
+                const long zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < Z ? cz + 1 : Z - 1;
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const long yy = cy + dy;
+                    if (yy < 0 || yy >= Y) continue;
+                    const unsigned long long c0 = base + ((unsigned long long)xx * Y + yy) * Z;
+                    long b, e;
+                    if (dense) {
+                        b = start[c0 + zlo];
+                        e = start[c0 + zhi + 1];
+                    } else {
+                        b = lower_bound_u64(key, n, c0 + zlo);
+                        e = lower_bound_u64(key, n, c0 + zhi + 1);
+                    }
+                    const int r = seg * 3 + dy + 1;
+                    const bool fromLds = useLds && staged >= 0;
+                    const long shift = fromLds ? (long)rngOff[r] - rngB[r] : 0;
+                    for (long k = b; k < e; ++k) {
+                        if (CLUMP ? (fromLds ? stageOwn[k + shift] : ownS[k]) == mine : k == t) continue;
+                        const double4 c = fromLds ? stage[k + shift] : posr[k];
+                        const double dx_ = c.x - me.x, dy_ = c.y - me.y, dz_ = c.z - me.z;
+                        const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
+                        const double rs = me.w + c.w;
+                        if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
+                        if (found < MAX_LIST) nbr[(long)found * n + t] = (int)k;
+                        ++found;
+                    }
+                }
+            }
+        }
+        if (useLds) __syncthreads();  // the next pass overwrites the stage
+    }
+    if (t < n) {
+        if (found > MAX_LIST) {
             stored = assign_inline(S, t, p, sorted, cidS, H, step, nbr, occTape, high, over, [&](auto&& acq) {
                 scan_overlaps<CLUMP>(S, t, p, mine, posr, ownS, key, start, grid, dense, acq);
             });
-        else
-            stored = assign_slots(S, t, p, found, sorted, cidS, H, step, nbr, occTape, high, over);
+        } else {
+            if (occTape) occTape[hrow] = occ;
+            stored = assign_batched(t, n, hrow, found, occ, k16, sorted, cidS, H, step, nbr, high, over);
+        }
         ncount[t] = stored;
     }
     block_count(stored, bsum, blockIdx.x);
-    // per-step statistics only when they are read; overflow counts are
-    // cumulative and (almost always) zero
     if (counters) {
         atomic_add_wave(&sc->particle_contacts, stored);
         atomic_max_wave(&sc->pp_high, high);
@@ -1779,10 +1931,51 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
     tri_pass_geo(A, I, r, u, hr, o, listed, geo, bF, bT, bodyHigh, ptOver, bcount, depthBits);
 }
 
+// advanceHalfVelocityKernel of the previous step followed by
+// advancePositionKernel of the next one (particle:213-265) for row p, in one
+// pass over its state, given the row's accelerations (a, aw) of the force
+// evaluation in between: the reference's loop calls them back to back with
+// only host work (the body solve) in between.
+__device__ __forceinline__ void finish_advance_row(const Soa& S, long p, double dtf, double dt, const Recs& R,
+                                                   double ax, double ay, double az, double awx, double awy,
+                                                   double awz)
+{
+    double vhx = S.f(F_VHX, p) + ax * dtf, vhy = S.f(F_VHY, p) + ay * dtf, vhz = S.f(F_VHZ, p) + az * dtf;
+    double whx = S.f(F_WHX, p) + awx * dtf, why = S.f(F_WHY, p) + awy * dtf, whz = S.f(F_WHZ, p) + awz * dtf;
+    double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+    const double h = 0.5 * dtf;
+    double dw, dx, dy, dz;
+    quat_dot(qw, qx, qy, qz, S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p), dw, dx, dy, dz);
+    double hw = qw + dw * h, hx = qx + dx * h, hy = qy + dy * h, hz = qz + dz * h;
+    quat_unit(hw, hx, hy, hz);
+    double gw, gx, gy, gz;
+    quat_dot(hw, hx, hy, hz, whx, why, whz, gw, gx, gy, gz);
+    S.f(F_VHX, p) = vhx; S.f(F_VHY, p) = vhy; S.f(F_VHZ, p) = vhz;
+    S.f(F_WHX, p) = whx; S.f(F_WHY, p) = why; S.f(F_WHZ, p) = whz;
+    S.f(F_HQW, p) = gw; S.f(F_HQX, p) = gx; S.f(F_HQY, p) = gy; S.f(F_HQZ, p) = gz;
+    V3 pos = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p)), vel, w;
+    if (S.k(I_STATUS, p) != -1) {
+        pos = mk(pos.x + vhx * dt, pos.y + vhy * dt, pos.z + vhz * dt);
+        S.f(F_X, p) = pos.x; S.f(F_Y, p) = pos.y; S.f(F_Z, p) = pos.z;
+        qw = qw + gw * dt; qx = qx + gx * dt; qy = qy + gy * dt; qz = qz + gz * dt;
+        quat_unit(qw, qx, qy, qz);
+        S.f(F_QW, p) = qw; S.f(F_QX, p) = qx; S.f(F_QY, p) = qy; S.f(F_QZ, p) = qz;
+        vel = mk(vhx + ax * (0.5 * dt), vhy + ay * (0.5 * dt), vhz + az * (0.5 * dt));
+        w = mk(whx + awx * (0.5 * dt), why + awy * (0.5 * dt), whz + awz * (0.5 * dt));
+        S.f(F_VX, p) = vel.x; S.f(F_VY, p) = vel.y; S.f(F_VZ, p) = vel.z;
+        S.f(F_WX, p) = w.x; S.f(F_WY, p) = w.y; S.f(F_WZ, p) = w.z;
+    } else {
+        if (!R.main) return;
+        vel = mk(S.f(F_VX, p), S.f(F_VY, p), S.f(F_VZ, p));
+        w = mk(S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p));
+    }
+    if (R.main) put_rec(R, S, p, pos, vel, mk(vhx, vhy, vhz), qw, qx, qy, qz, w, mk(whx, why, whz));
+}
+
 // Force outputs of owner row p (contact:1211-1220) and its acceleration
-// (accelerationKernel, particle:71-101).
+// (accelerationKernel, particle:71-101), also returned in acc.
 __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3 cF, V3 cT, V3 fF, V3 wF, V3 wT,
-                                          V3 bF, V3 bT)
+                                          V3 bF, V3 bT, double (&acc)[6])
 {
     const Soa& S = A.S;
     const long n = S.n;
@@ -1796,9 +1989,12 @@ __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3
     of[0 * n + p] = fF.x; of[1 * n + p] = fF.y; of[2 * n + p] = fF.z;
     const V3 tF = add(add(cF, fF), add(wF, bF));
     const V3 tT = add(cT, add(wT, bT));
-    S.f(F_AX, p) = tF.x / m;
-    S.f(F_AY, p) = tF.y / m;
-    S.f(F_AZ, p) = tF.z / m;
+    acc[0] = tF.x / m;
+    acc[1] = tF.y / m;
+    acc[2] = tF.z / m;
+    S.f(F_AX, p) = acc[0];
+    S.f(F_AY, p) = acc[1];
+    S.f(F_AZ, p) = acc[2];
     const double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
     const double tx = (1.0 - 2.0 * (qy * qy + qz * qz)) * tT.x + 2.0 * (qx * qy + qw * qz) * tT.y +
                       2.0 * (qx * qz - qw * qy) * tT.z;
@@ -1808,17 +2004,27 @@ __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3
                       (1.0 - 2.0 * (qx * qx + qy * qy)) * tT.z;
     const double wx = S.f(F_WX, p), wy = S.f(F_WY, p), wz = S.f(F_WZ, p);
     const double ix = S.f(F_IX, p), iy = S.f(F_IY, p), iz = S.f(F_IZ, p);
-    S.f(F_AWX, p) = (tx - (iz - iy) * wz * wy) / ix;
-    S.f(F_AWY, p) = (ty - (ix - iz) * wx * wz) / iy;
-    S.f(F_AWZ, p) = (tz - (iy - ix) * wy * wx) / iz;
+    acc[3] = (tx - (iz - iy) * wz * wy) / ix;
+    acc[4] = (ty - (ix - iz) * wx * wz) / iy;
+    acc[5] = (tz - (iy - ix) * wy * wx) / iz;
+    S.f(F_AWX, p) = acc[3];
+    S.f(F_AWY, p) = acc[4];
+    S.f(F_AWZ, p) = acc[5];
 }
 
 // computeContactsKernel (contact:1065-1221) for one single-sphere owner,
 // plus the acceleration (particle:71-101).  UNIVERSAL: runs with universal
 // gravitation (without it the kernel fits 3 waves per SIMD instead of 2).
-template <bool UNIVERSAL>
+// FINISH: a step of a multi-step run that another step follows: the row's
+// finish of this step and advance of the next (finish_advance_row) follow its
+// acceleration here, from registers, instead of in k_finish_advance (no row
+// reads another row's state or record in this kernel), and the first block
+// resets the next evaluation's step counters (this evaluation's are not read:
+// it collects no diagnostics).
+template <bool UNIVERSAL, bool FINISH>
 __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
 {
+    if (FINISH && blockIdx.x == 0) reset_step_scalars(A.sc, A.sceneAcc, A.nscenes);
     const long t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     const Soa& S = A.S;
     const long n = S.n;
@@ -1873,7 +2079,10 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
             bF = mk(ob_[0 * n + p], ob_[1 * n + p], ob_[2 * n + p]);
             bT = mk(ob_[3 * n + p], ob_[4 * n + p], ob_[5 * n + p]);
         }
-        owner_out(A, p, mass, cF, cT, fF, wF, wT, bF, bT);
+        double acc[6];
+        owner_out(A, p, mass, cF, cT, fF, wF, wT, bF, bT, acc);
+        if (FINISH)
+            finish_advance_row(S, p, A.finishDt, A.nextDt, A.rec, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5]);
     }
     if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
@@ -2047,7 +2256,8 @@ __global__ void __launch_bounds__(BLOCK) k_law_clump(LawArgs A)
             ob[0 * n + i] = bF.x; ob[1 * n + i] = bF.y; ob[2 * n + i] = bF.z;
             ob[3 * n + i] = bT.x; ob[4 * n + i] = bT.y; ob[5 * n + i] = bT.z;
         }
-        owner_out(A, i, mass, cF, cT, fF, wF, wT, bF, bT);
+        double acc[6];
+        owner_out(A, i, mass, cF, cT, fF, wF, wT, bF, bT, acc);
     }
     atomic_add_wave(&A.sc->body_contacts, bcount);
     if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
@@ -2239,50 +2449,14 @@ __global__ void __launch_bounds__(BLOCK) k_finish(Soa S, double dt)
     half_quat(S, p, 0.5 * dt);
 }
 
-// advanceHalfVelocityKernel of the previous step followed by
-// advancePositionKernel of the next one (particle:213-265), in one pass over
-// the state: the reference's loop calls them back to back with only host work
-// (the body solve) in between.
 __global__ void __launch_bounds__(BLOCK) k_finish_advance(Soa S, double dtf, double dt, Recs R, Scalars* sc,
                                                           int* sceneAcc, int nscenes)
 {
     if (sc && blockIdx.x == 0) reset_step_scalars(sc, sceneAcc, nscenes);
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     if (p >= S.n) return;
-    double vhx = S.f(F_VHX, p) + S.f(F_AX, p) * dtf, vhy = S.f(F_VHY, p) + S.f(F_AY, p) * dtf,
-           vhz = S.f(F_VHZ, p) + S.f(F_AZ, p) * dtf;
-    double whx = S.f(F_WHX, p) + S.f(F_AWX, p) * dtf, why = S.f(F_WHY, p) + S.f(F_AWY, p) * dtf,
-           whz = S.f(F_WHZ, p) + S.f(F_AWZ, p) * dtf;
-    double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
-    const double h = 0.5 * dtf;
-    double dw, dx, dy, dz;
-    quat_dot(qw, qx, qy, qz, S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p), dw, dx, dy, dz);
-    double hw = qw + dw * h, hx = qx + dx * h, hy = qy + dy * h, hz = qz + dz * h;
-    quat_unit(hw, hx, hy, hz);
-    double gw, gx, gy, gz;
-    quat_dot(hw, hx, hy, hz, whx, why, whz, gw, gx, gy, gz);
-    S.f(F_VHX, p) = vhx; S.f(F_VHY, p) = vhy; S.f(F_VHZ, p) = vhz;
-    S.f(F_WHX, p) = whx; S.f(F_WHY, p) = why; S.f(F_WHZ, p) = whz;
-    S.f(F_HQW, p) = gw; S.f(F_HQX, p) = gx; S.f(F_HQY, p) = gy; S.f(F_HQZ, p) = gz;
-    V3 pos = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p)), vel, w;
-    if (S.k(I_STATUS, p) != -1) {
-        pos = mk(pos.x + vhx * dt, pos.y + vhy * dt, pos.z + vhz * dt);
-        S.f(F_X, p) = pos.x; S.f(F_Y, p) = pos.y; S.f(F_Z, p) = pos.z;
-        qw = qw + gw * dt; qx = qx + gx * dt; qy = qy + gy * dt; qz = qz + gz * dt;
-        quat_unit(qw, qx, qy, qz);
-        S.f(F_QW, p) = qw; S.f(F_QX, p) = qx; S.f(F_QY, p) = qy; S.f(F_QZ, p) = qz;
-        vel = mk(vhx + S.f(F_AX, p) * (0.5 * dt), vhy + S.f(F_AY, p) * (0.5 * dt),
-                 vhz + S.f(F_AZ, p) * (0.5 * dt));
-        w = mk(whx + S.f(F_AWX, p) * (0.5 * dt), why + S.f(F_AWY, p) * (0.5 * dt),
-               whz + S.f(F_AWZ, p) * (0.5 * dt));
-        S.f(F_VX, p) = vel.x; S.f(F_VY, p) = vel.y; S.f(F_VZ, p) = vel.z;
-        S.f(F_WX, p) = w.x; S.f(F_WY, p) = w.y; S.f(F_WZ, p) = w.z;
-    } else {
-        if (!R.main) return;
-        vel = mk(S.f(F_VX, p), S.f(F_VY, p), S.f(F_VZ, p));
-        w = mk(S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p));
-    }
-    if (R.main) put_rec(R, S, p, pos, vel, mk(vhx, vhy, vhz), qw, qx, qy, qz, w, mk(whx, why, whz));
+    finish_advance_row(S, p, dtf, dt, R, S.f(F_AX, p), S.f(F_AY, p), S.f(F_AZ, p), S.f(F_AWX, p), S.f(F_AWY, p),
+                       S.f(F_AWZ, p));
 }
 
 // ------------------------------------------------------------ triangles
@@ -2661,6 +2835,7 @@ struct dem_solver {
     int* cand = nullptr;       // [CAND_MAX][n]
     int* ncand = nullptr;      // [n]
     int4* meta = nullptr;      // [n] cell and input row in cell order
+    uint32_t* ncid = nullptr;  // [MAX_LIST][n] component id of each contact the list scan found
     int* hFlag = nullptr;      // pinned: displacement / overflow flags
     int* hMirror = nullptr;    // pinned, coherent: [seq, need_rebuild, grid_overflow] written by k_vscan
     int* dMirror = nullptr;    // device address of hMirror
@@ -2793,6 +2968,8 @@ struct dem_solver {
     bool evReady = false;
     bool timing = false;  // record the stage events of dem_debug_last_timings
     bool scalarsReset = false;  // the last advance reset the step counters (no k_step_init needed)
+    double nextDt = 0.0;        // set by a multi-step run while another step follows this evaluation
+    bool advancedAhead = false; // the last evaluation also finished its step and advanced the next one
     // metadata for checkpoints
     std::vector<int> compIds, ownerIds;  // per component: id, owner id
     std::vector<int> pids;               // per owner: id
@@ -2808,7 +2985,7 @@ struct dem_solver {
         diff_release(diff);
         graph_release(graphs);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB, ident, mKey, mVal,
-                        rec, recAux, refC, cand, ncand, meta, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
+                        rec, recAux, refC, cand, ncand, meta, ncid, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, rowF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
@@ -3180,6 +3357,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
             TRY(dalloc(s->cand, (size_t)CAND_MAX * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->ncand, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->meta, n, "hipMalloc(candidate lists)"));
+            TRY(dalloc(s->ncid, (size_t)MAX_LIST * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->ident, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->mKey, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->mVal, n, "hipMalloc(candidate lists)"));
@@ -3643,7 +3821,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             if (fusedPosr) {
                 HIP_TRY(hipEventRecord(s->flagEv, st), "event");
                 if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
-                hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ident, s->posr,
+                hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ncid, s->posr,
                                    s->meta, s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr,
                                    s->ncount, s->tape.ppOcc, collect || s->countAlways, s->listBlk,
                                    (const int*)&s->sc->need_rebuild, (int*)nullptr, 0);
@@ -3663,7 +3841,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             const int* skip = &s->sc->need_rebuild;
             const int seq = ++s->mirrorSeq;
             if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
-            hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ident, s->posr, s->meta,
+            hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ncid, s->posr, s->meta,
                                s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr, s->ncount,
                                s->tape.ppOcc, collect || s->countAlways, s->listBlk, skip, s->dMirror, seq);
             TRY(launch_check("contact scan"));
@@ -3782,7 +3960,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     if (scanned)
         ;
     else if (useLists)
-        hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->ident, s->posr, s->meta, s->cidS,
+        hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->ncid, s->posr, s->meta, s->cidS,
                            s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
                            counters, s->listBlk, (const int*)nullptr, (int*)nullptr, 0);
     else if (s->clump)
@@ -3838,8 +4016,26 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
                                (const int*)s->triHitN, (const double2*)s->triHitVal);
             TRY(launch_check("particle-triangle contact law"));
         }
-        if (s->universal) hipLaunchKernelGGL(k_law<true>, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
-        else hipLaunchKernelGGL(k_law<false>, dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+        // a step of a multi-step run that another step follows finishes and
+        // advances its rows in the law kernel (zero-offset single spheres,
+        // no bodies, no diagnostics or tape, no universal gravitation)
+        const bool finishHere = s->nextDt > 0.0 && !collect && !s->countAlways && nb == 0 && !s->offs &&
+                                !s->universal && !s->tape.pair && !s->tape.pwOcc && !s->tape.wallCount &&
+                                !s->tape.plateAcc;
+        if (s->universal) {
+            hipLaunchKernelGGL((k_law<true, false>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+        } else if (finishHere) {
+            A.finishDt = s->nextDt;
+            A.nextDt = s->nextDt;
+            A.sceneAcc = s->sceneAcc;
+            A.nscenes = s->nscenes;
+            hipLaunchKernelGGL((k_law<false, true>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+            s->advancedAhead = true;
+            s->scalarsReset = true;
+            s->recFresh = true;
+        } else {
+            hipLaunchKernelGGL((k_law<false, false>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+        }
     }
     TRY(launch_check("contact law"));
     if (timed) HIP_TRY(hipEventRecord(s->ev[4], st), "event");
@@ -3990,6 +4186,12 @@ int dem_init_half_step(dem_solver* s, double h)
 int dem_advance_particles(dem_solver* s, double dt, double* vmax)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    if (s->advancedAhead) {
+        // the law kernel of the last evaluation advanced the rows already
+        s->advancedAhead = false;
+        if (vmax) *vmax = 0.0;
+        return DEM_OK;
+    }
     // zero-offset single-sphere rows: the advance also writes the contact records
     const Recs R = (!s->offs && !s->clump) ? s->recs() : Recs{nullptr, nullptr};
     if (s->pendingFinish) {
@@ -4010,6 +4212,7 @@ int dem_advance_particles(dem_solver* s, double dt, double* vmax)
 int dem_finish_particle_step(dem_solver* s, double dt, double)
 {
     if (!s || !s->initialized) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
+    if (s->advancedAhead) return DEM_OK;  // finished by the law kernel
     TRY(flush_finish(s));
     s->pendingFinish = true;
     s->pendingDt = dt;

@@ -78,9 +78,51 @@ __device__ __forceinline__ uint32_t order_key(int4 mine, int4 other)
     return ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) | (uint32_t)other.w;
 }
 
+// History slots of a row's listed contacts (assign_batched for candidate-list
+// steps, where rows are their own sorted positions): nbr[c][t] and
+// cid[c][t] hold the c-th contact's row and component id in the reference's
+// order; on return nbr[c][t] = (row << SLOT_BITS | slot) for the contacts
+// that got a slot.  The row's occupancy, seen mask and first 16 keys are
+// read by the caller before its scan (occ, seen, k16), and the entries are
+// read four at a time, so no lookup waits on another.
+__device__ __forceinline__ int assign_listed(long t, long n, long o, int found, unsigned long long occ,
+                                             uint32_t (&k16)[16], const History& H, int step, int* nbr,
+                                             const uint32_t* cid, int& high, int& over)
+{
+    unsigned long long cur = 0;
+    uint32_t* keys = H.ppKey + o * PP_SLOTS;
+    double* vals = pp_slot(H, o, 0);
+    int stored = 0;
+    for (int c0 = 0; c0 < found; c0 += 4) {
+        int kq[4];
+        uint32_t cq[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long cc = min(c0 + j, found - 1);
+            kq[j] = nbr[cc * n + t];
+            cq[j] = cid[cc * n + t];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (c0 + j >= found) break;
+            const int slot = acquire_pp(occ, cur, k16, keys, vals, H.rows * 9, cq[j] + 1u, high, over);
+            if (slot < 0) continue;  // slot overflow: counted, no force
+            nbr[(long)stored * n + t] = (kq[j] << SLOT_BITS) | slot;
+            ++stored;
+        }
+    }
+    H.ppOcc[o] = occ;
+    H.ppSeen[(long)(step & 1) * n + o] = cur;
+    return stored;
+}
+
 // Step between rebuilds: exact contact test over the listed partners, the
-// reference's candidate order, then history slots (assign_slots).
-__global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const double4* posr, const int4* meta,
+// reference's candidate order, then history slots.  Rows are their own
+// sorted positions on these steps.  Per batch of VSCAN_BATCH candidates the
+// indices, then the centres, then (for the overlapping ones) the cell
+// records and component ids are fetched together; the row's history state
+// is fetched before the scan.
+__global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const double4* posr, const int4* meta,
                                                  const uint32_t* cidS, const int* cand, const int* ncand,
                                                  Scalars* sc, History H, int step, int* nbr,
                                                  int* ncount, unsigned long long* occTape, bool counters,
@@ -100,30 +142,54 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
     int found = 0, stored = 0, high = 0, over = 0;
     uint32_t last = 0;  // order key of the last contact in the list
     if (t < n) {
+        // history state of the row (expiry: slots not seen last step drop)
+        const long o = S.k(I_HIST, t);
+        unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
+        uint32_t k16[16];
+        {
+            const uint4* kv = reinterpret_cast<const uint4*>(H.ppKey + o * PP_SLOTS);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint4 v = kv[i];
+                k16[4 * i] = v.x; k16[4 * i + 1] = v.y; k16[4 * i + 2] = v.z; k16[4 * i + 3] = v.w;
+            }
+        }
         if (S.k(I_VALID, t)) {
             const double4 me = posr[t];
             const int4 mine = meta[t];
             const int m = ncand[t];
-            // candidates in list order, their indices and centres fetched
-            // VSCAN_BATCH at a time so that the gathers overlap
             constexpr int VSCAN_BATCH = 4;
             for (int c0 = 0; c0 < m; c0 += VSCAN_BATCH) {
                 long kq[VSCAN_BATCH];
                 double4 oq[VSCAN_BATCH];
+                bool hit[VSCAN_BATCH];
 #pragma unroll
                 for (int j = 0; j < VSCAN_BATCH; ++j) kq[j] = c0 + j < m ? cand[(long)(c0 + j) * n + t] : t;
 #pragma unroll
                 for (int j = 0; j < VSCAN_BATCH; ++j) oq[j] = posr[kq[j]];
 #pragma unroll
                 for (int j = 0; j < VSCAN_BATCH; ++j) {
-                    if (c0 + j >= m) break;
-                    const long k = kq[j];
-                    const double4 o = oq[j];
-                    const double dx_ = o.x - me.x, dy_ = o.y - me.y, dz_ = o.z - me.z;
+                    const double4 c = oq[j];
+                    const double dx_ = c.x - me.x, dy_ = c.y - me.y, dz_ = c.z - me.z;
                     const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
-                    const double rs = me.w + o.w;
-                    if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
-                    const int4 other = meta[k];
+                    const double rs = me.w + c.w;
+                    hit[j] = c0 + j < m && !(d2 >= rs * rs || d2 <= 1.0e-28);
+                }
+                // cell records and ids of the overlapping candidates (the
+                // others read the row's own, which it holds already)
+                int4 mq[VSCAN_BATCH];
+                uint32_t cq[VSCAN_BATCH];
+#pragma unroll
+                for (int j = 0; j < VSCAN_BATCH; ++j) {
+                    const long k = hit[j] ? kq[j] : t;
+                    mq[j] = meta[k];
+                    cq[j] = cidS[k];
+                }
+#pragma unroll
+                for (int j = 0; j < VSCAN_BATCH; ++j) {
+                    if (!hit[j]) continue;
+                    const long k = kq[j];
+                    const int4 other = mq[j];
                     const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
                     if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
                     const uint32_t kk = order_key(mine, other);
@@ -138,6 +204,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
                     // cell records
                     if (found == 0 || kk > last) {
                         nbr[(long)found * n + t] = (int)k;
+                        ncid[(long)found * n + t] = cq[j];
                         last = kk;
                     } else {
                         int pos = found;
@@ -145,9 +212,11 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
                             const int kp = nbr[(long)(pos - 1) * n + t];
                             if (order_key(mine, meta[kp]) <= kk) break;
                             nbr[(long)pos * n + t] = kp;
+                            ncid[(long)pos * n + t] = ncid[(long)(pos - 1) * n + t];
                             --pos;
                         }
                         nbr[(long)pos * n + t] = (int)k;
+                        ncid[(long)pos * n + t] = cq[j];
                     }
                     ++found;
                 }
@@ -161,7 +230,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
             const int4 mine = meta[t];
             const int m = ncand[t];
             const int total = found;
-            stored = assign_inline(S, t, sorted[t], sorted, cidS, H, step, nbr, occTape, high, over,
+            stored = assign_inline(S, t, t, ident_order(), cidS, H, step, nbr, occTape, high, over,
                                    [&](auto&& acq) {
                 uint32_t last = 0;
                 for (int r = 0; r < total; ++r) {
@@ -189,7 +258,8 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, const int* sorted, const
                 }
             });
         } else {
-            stored = assign_slots(S, t, sorted[t], found, sorted, cidS, H, step, nbr, occTape, high, over);
+            if (occTape) occTape[o] = occ;
+            stored = assign_listed(t, n, o, found, occ, k16, H, step, nbr, ncid, high, over);
         }
         ncount[t] = stored;
     }

@@ -34,9 +34,14 @@ def start(lib, sc):
     return s
 
 
+@pytest.mark.parametrize("skin", ["0.2", "0"])
 @pytest.mark.parametrize("reorder", ["0", "1", "3"])
-def test_graph_replays_equal_eager_steps(hip, reorder, monkeypatch):
+def test_graph_replays_equal_eager_steps(hip, reorder, skin, monkeypatch):
+    """Multi-step runs (candidate lists with skin 0.2, or the full scan from
+    captured graphs with skin 0), whose steps finish and advance their rows in
+    the law kernel while another step follows, against step-by-step calls."""
     monkeypatch.setenv("DEM_REORDER_INTERVAL", reorder)
+    monkeypatch.setenv("DEM_SKIN", skin)
     sc = box()
     a, b = start(hip, sc), start(hip, sc)
     dt = sc.dt
