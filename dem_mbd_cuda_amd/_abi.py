@@ -337,10 +337,11 @@ class Solver:
         """dem_debug_run_counters as a dict (None for a library without it)."""
         if not self.lib.has("debug_run_counters"):
             return None
-        buf = (C.c_longlong * 6)()
+        buf = (C.c_longlong * 8)()
         cnt = C.c_int(0)
-        self.lib.call("debug_run_counters", self.h, buf, 6, C.byref(cnt))
-        names = ("evaluations", "sorts", "rebuilds", "graph_steps", "eager_steps", "last_sorted")
+        self.lib.call("debug_run_counters", self.h, buf, 8, C.byref(cnt))
+        names = ("evaluations", "sorts", "rebuilds", "graph_steps", "eager_steps", "last_sorted",
+                 "scan_lds_passes", "scan_hbm_passes")
         return {names[i]: int(buf[i]) for i in range(cnt.value)}
 
     def debug_history(self):

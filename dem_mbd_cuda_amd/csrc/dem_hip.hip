@@ -178,6 +178,9 @@ struct Scalars {
     int cand_overflow;  // a rebuild listed more partners than CAND_MAX
     int tri_work;       // rows listed by k_tri_sift (their cell holds triangle entries)
     unsigned long long tri_total;  // triangle grid entries
+    // full-scan block passes (k_scan, diagnostics evaluations only, since
+    // initialisation): candidates staged in LDS / read from HBM
+    unsigned long long scan_lds, scan_hbm;
 };
 
 struct History {
@@ -1388,6 +1391,8 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
                 }
             }
         }
+        if (counters && threadIdx.x == 0)
+            atomicAdd(useLds && staged >= 0 ? &sc->scan_lds : &sc->scan_hbm, 1ull);
         if (useLds) __syncthreads();  // the next pass overwrites the stage
     }
     if (t < n) {
@@ -4325,9 +4330,15 @@ int dem_device_memory_bytes(const dem_solver* s, size_t* out)
 int dem_debug_run_counters(const dem_solver* s, long long* out, int capacity, int* count)
 {
     if (!s) return set_error(DEM_E_INVALID, "CUDA DEM solver is not initialized");
-    const long long v[6] = {s->nEvaluations, s->nSorts, (long long)s->rebuilds, s->nGraphSteps,
-                            s->nEagerSteps, s->lastSorted ? 1 : 0};
-    const int k = std::max(0, std::min(capacity, 6));
+    unsigned long long passes[2] = {0ull, 0ull};
+    if (s->initialized && capacity > 6) {
+        HIP_TRY(hipMemcpyAsync(passes, &s->sc->scan_lds, sizeof passes, hipMemcpyDeviceToHost, s->stream),
+                "read scan counters");
+        HIP_TRY(hipStreamSynchronize(s->stream), "read scan counters");
+    }
+    const long long v[8] = {s->nEvaluations, s->nSorts, (long long)s->rebuilds, s->nGraphSteps,
+                            s->nEagerSteps, s->lastSorted ? 1 : 0, (long long)passes[0], (long long)passes[1]};
+    const int k = std::max(0, std::min(capacity, 8));
     for (int i = 0; i < k; ++i) out[i] = v[i];
     if (count) *count = k;
     return DEM_OK;

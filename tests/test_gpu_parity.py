@@ -294,3 +294,60 @@ def test_device_sort_order(hip, oracle, skin, monkeypatch):
     keys = o[0]["grid"][0]
     assert len(np.unique(keys)) < 0.9 * len(keys)  # ties exist
     compare_frames(g, o)
+
+
+def _bimodal_scene():
+    """Small spheres (r 0.5 mm) on a slightly overlapping cubic lattice under
+    two large ones (r 4 mm): cells of 2 x 4 mm hold about 500 small spheres,
+    so a block's staged neighbourhood exceeds the LDS stage."""
+    from dem_mbd_cuda_amd import scene
+
+    r, a = 0.5e-3, 0.999e-3
+    g = np.arange(20) * a + r
+    xs, ys, zs = np.meshgrid(g, g, g[:10], indexing="ij")
+    small = np.stack([xs.ravel(), ys.ravel(), zs.ravel()], axis=1)
+    big = np.array([[0.006, 0.006, 0.030], [0.015, 0.015, 0.030]])
+    pos = np.vstack([small, big])
+    radius = np.concatenate([np.full(len(small), r), np.full(2, 4e-3)])
+    perm = np.random.RandomState(51).permutation(len(pos))
+    pos, radius = pos[perm].copy(), radius[perm].copy()
+    m = scene.mech(epsS=0.5, mu=0.5, epsN=0.5, cohesion=0.0, beta=0.5, muT=0.1, muR=0.1, kN=1e4)
+    wo, wn = scene.box_walls(np.zeros(3), np.array([0.021, 0.021, 0.04]))
+    sc = scene.Scene(**scene.sphere_particles(pos, radius, 2500.0), wall_origin=wo, wall_normal=wn, pp=m,
+                     pt=dict(m), pw=dict(m), dt=1e-6, gravity=(0.0, 0.0, -9.81), gravity_enabled=1,
+                     mesh_scale=2.0, steps=10)
+    sc.vel = np.random.RandomState(52).uniform(-0.05, 0.05, (sc.n, 3))
+    return sc
+
+
+@pytest.mark.parametrize("case", ["dense_bed", "bimodal", "dilute_box"])
+def test_full_scan_staging_paths(hip, oracle, case, monkeypatch):
+    """The full 27-cell scan (DEM_SKIN=0) on its three paths, bit for bit
+    against the oracle: a settled hopper bed at a cell of 2 x max radius
+    (the blocks' neighbourhoods staged in LDS), a bimodal packing whose
+    staged neighbourhoods exceed the stage, and a dilute box whose blocks
+    span more columns than are staged (both read the candidates from device
+    memory).  The run counters say which path the diagnostics evaluations
+    took."""
+    from dem_mbd_cuda_amd import scene
+
+    monkeypatch.setenv("DEM_SKIN", "0")
+    if case == "dense_bed":
+        sc = scene.hopper_scene(n=20_000, width=0.2 * 0.02 ** 0.5, packing="settled", mesh_scale=2.0, steps=10)
+    elif case == "bimodal":
+        sc = _bimodal_scene()
+    else:
+        sc = scene.box_scene(n=2000, seed=53, steps=10)
+        sc.vel = np.random.RandomState(54).uniform(-1.0, 1.0, (sc.n, 3))
+    g = run_frames(hip, sc, steps=10, interval=5, debug=True)
+    o = run_frames(oracle, sc, steps=10, interval=5, debug=True)
+    c = g[10]["counters"]
+    print(case, "scan passes staged / from device memory:", c["scan_lds_passes"], c["scan_hbm_passes"], flush=True)
+    assert o[10]["stats"]["particle_contacts"] > 0
+    if case == "dense_bed":
+        # a bed this narrow has short columns, so some blocks span more
+        # columns than are staged (about a quarter; 3 % on the 1M bed)
+        assert c["scan_lds_passes"] > 2 * c["scan_hbm_passes"], c
+    else:
+        assert c["scan_hbm_passes"] > c["scan_lds_passes"], c
+    compare_frames(g, o)
