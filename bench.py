@@ -372,7 +372,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=120,
                     help="untimed steps; from 104 on they include the autotuner's trial spans")
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="reference-binary steps timed for the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="reference-binary steps timed for the CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=100_000, help="spheres of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mesh-scale", type=float, default=2.0,
