@@ -394,6 +394,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=100)
     ap.add_argument("--sweep-seq-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-batch figure")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="measure only the differentiable batch and print its own JSON line (for profiling it alone)")
     ap.add_argument("--adjoint-scenes", type=int, default=64)
     ap.add_argument("--adjoint-n", type=int, default=5000)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
@@ -414,6 +416,14 @@ def main():
                           "ms_per_step": c["ms_per_step"], "higher_is_better": True, "scaling": "weak",
                           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded radii and packing)",
                           "config": {k: v for k, v in c.items() if k not in ("ms_per_step", "particle_steps_per_s")}}))
+        return
+    if args.only_adjoint:
+        a = measure_adjoint(dm, scene, args.adjoint_scenes, args.adjoint_n, args.adjoint_steps, args.adjoint_K, 5)
+        print(json.dumps({"metric": "differentiable batch: whole gradient cost over one plain forward run",
+                          "value": a["gradient_over_forward"], "unit": "x forward", "n_gpus": 1,
+                          "steps": a["steps"], "warmup": 0, "ms_per_step": 1e3 * a["forward_s"] / a["steps"],
+                          "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                          "data": "synthetic (seeded radii and packing)", "config": a}))
         return
     if args.copy_calibration:
         copy_calibration()

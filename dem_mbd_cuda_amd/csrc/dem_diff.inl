@@ -574,8 +574,10 @@ int diff_reorder(dem_solver* s)
 int diff_step(dem_solver* s, double dt)
 {
     const long n = s->n;
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, Recs{nullptr, nullptr});
-    s->recFresh = false;
+    // the advance also writes the rows' contact records (runs take
+    // zero-offset single spheres), so the evaluation packs none
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, s->recs());
+    s->recFresh = true;
     TRY(dem_compute_forces(s, nullptr, nullptr, 0));
     hipLaunchKernelGGL(k_finish, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
     return launch_check("differentiable step");

@@ -482,11 +482,24 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
         }
         if (threadIdx.x == 0) partScene[blockIdx.x] = smin;
     } else {
-        if (scene >= 0 && lo[0] <= hi[0])
+        // a block over several scenes: each wave reduces every scene it
+        // holds and adds it with one set of atomics (lanes of the same scene
+        // are contiguous, so a wave holds few)
+        const int lane = threadIdx.x & 63;
+        const int wmin = wave_min(scene < 0 ? INT_MAX : scene), wmax = wave_max(scene);
+        for (int sc_ = wmin; sc_ <= wmax; ++sc_) {
+            const bool mine = scene == sc_;
+            int v[6];
             for (int a = 0; a < 3; ++a) {
-                atomicMin(&sceneAcc[scene * 6 + a], lo[a]);
-                atomicMax(&sceneAcc[scene * 6 + 3 + a], hi[a]);
+                v[a] = wave_min(mine ? lo[a] : INT_MAX);
+                v[3 + a] = wave_max(mine ? hi[a] : INT_MIN);
             }
+            if (lane == 0 && v[0] <= v[3])
+                for (int a = 0; a < 3; ++a) {
+                    atomicMin(&sceneAcc[sc_ * 6 + a], v[a]);
+                    atomicMax(&sceneAcc[sc_ * 6 + 3 + a], v[3 + a]);
+                }
+        }
         if (threadIdx.x == 0) partScene[blockIdx.x] = -1;
     }
 }
@@ -653,13 +666,40 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* t
     return before + x - v;
 }
 
+// Block-wide maximum (wave maxima through LDS); every thread of the block
+// calls it and receives the result.
+__device__ __forceinline__ int block_max(int v)
+{
+    __shared__ int wmax[BLOCK / 64];
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int m = wmax[0];
+    for (int w = 1; w < BLOCK / 64; ++w) m = max(m, wmax[w]);
+    __syncthreads();
+    return m;
+}
+
 // The block's sum of the rows' contact counts, for the list offsets
-// (bsum[b] covers rows b * BLOCK ...); every thread of the block calls it.
-__device__ __forceinline__ void block_count(int stored, unsigned* bsum, long b)
+// (bsum[b] covers rows b * BLOCK ...), and the block's history high-water
+// mark in bsum[gridDim.x + b] (k_off_scan folds both, so no contact counter
+// is an atomic); every thread of the block calls it.
+__device__ __forceinline__ void block_count(int stored, int high, unsigned* bsum, long b)
 {
     unsigned tot;
     block_exclusive_scan((unsigned)stored, &tot);
-    if (threadIdx.x == 0) bsum[b] = tot;
+    const int h = block_max(high);
+    if (threadIdx.x == 0) {
+        bsum[b] = tot;
+        bsum[gridDim.x + b] = (unsigned)h;
+    }
+}
+
+// atomicMax of the block's maximum of v (one atomic per block, none if 0).
+__device__ __forceinline__ void atomic_max_block(int* dst, int v)
+{
+    v = block_max(v);
+    if (threadIdx.x == 0 && v) atomicMax(dst, v);
 }
 
 // Digit-major counts [256][nblocks]: block d turns row d into exclusive
@@ -1406,11 +1446,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
         }
         ncount[t] = stored;
     }
-    block_count(stored, bsum, blockIdx.x);
-    if (counters) {
-        atomic_add_wave(&sc->particle_contacts, stored);
-        atomic_max_wave(&sc->pp_high, high);
-    }
+    block_count(stored, high, bsum, blockIdx.x);
     atomic_add_wave(&sc->pp_overflow, over);
 }
 
@@ -1419,9 +1455,13 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
 // block scanning the sums, per-block offsets), the last also writing each
 // contact's (owner, entry) at its position.
 
+// The scan's block sums (bsum[0, nblocks)) become block offsets; the total
+// is the step's stored contact count and the block high-water marks
+// (bsum[nblocks, 2 nblocks)) fold into the step's high-water mark.
 __global__ void __launch_bounds__(BLOCK) k_off_scan(unsigned* bsum, int nblocks, Scalars* sc)
 {
     unsigned carry = 0;
+    int high = 0;
     for (long base = 0; base < nblocks; base += SCAN_TILE) {
         const long b = base + (long)threadIdx.x * SCAN_ITEMS;
         unsigned v[SCAN_ITEMS];
@@ -1429,6 +1469,7 @@ __global__ void __launch_bounds__(BLOCK) k_off_scan(unsigned* bsum, int nblocks,
         for (int j = 0; j < SCAN_ITEMS; ++j) {
             v[j] = b + j < nblocks ? bsum[b + j] : 0u;
             sum += v[j];
+            if (b + j < nblocks) high = max(high, (int)bsum[nblocks + b + j]);
         }
         unsigned tot;
         unsigned run = carry + block_exclusive_scan(sum, &tot);
@@ -1438,7 +1479,12 @@ __global__ void __launch_bounds__(BLOCK) k_off_scan(unsigned* bsum, int nblocks,
         }
         carry += tot;
     }
-    if (threadIdx.x == 0) sc->total_pairs = carry;
+    high = block_max(high);
+    if (threadIdx.x == 0) {
+        sc->total_pairs = carry;
+        sc->particle_contacts = (int)carry;
+        sc->pp_high = high;
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_off_apply(const int* cnt, long n, const unsigned* bsum,
@@ -2089,7 +2135,7 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
         if (FINISH)
             finish_advance_row(S, p, A.finishDt, A.nextDt, A.rec, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5]);
     }
-    if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
+    if (A.counters) atomic_max_block(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
 }
 
@@ -2265,7 +2311,7 @@ __global__ void __launch_bounds__(BLOCK) k_law_clump(LawArgs A)
         owner_out(A, i, mass, cF, cT, fF, wF, wT, bF, bT, acc);
     }
     atomic_add_wave(&A.sc->body_contacts, bcount);
-    if (A.counters) atomic_max_wave(&A.sc->body_high, bodyHigh);
+    if (A.counters) atomic_max_block(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
     atomic_add_wave(&A.sc->pt_overflow, ptOver);
     atomic_max_wave_u64(&A.sc->body_max_depth_bits, depthBits);
@@ -2912,7 +2958,7 @@ struct dem_solver {
     int* nbr = nullptr;      // k-major contact lists [MAX_LIST][n]
     int* ncount = nullptr;   // list length per owner (sorted index)
     unsigned* listOff = nullptr;  // [n] owner's first position in the compacted list
-    unsigned* listBlk = nullptr;  // [blocks] per-block offsets
+    unsigned* listBlk = nullptr;  // [2][blocks] per-block offsets, then per-block high-water marks
     int2* pairs = nullptr;   // compacted (owner, entry) [MAX_LIST * n]
     double* pairF = nullptr; // per-contact force and torque term [MAX_LIST * n][6]
     double* rowF = nullptr;  // [n][6] contact sums of rows summed in k_pairs (single-sphere runs)
@@ -3343,7 +3389,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     TRY(dalloc(s->nbr, (size_t)MAX_LIST * nr, "hipMalloc(contact lists)"));
     TRY(dalloc(s->ncount, nr, "hipMalloc(contact counts)"));
     TRY(dalloc(s->listOff, nr + 1, "hipMalloc(contact list offsets)"));
-    TRY(dalloc(s->listBlk, blocks_for(nr), "hipMalloc(contact list offsets)"));
+    TRY(dalloc(s->listBlk, (size_t)2 * blocks_for(nr), "hipMalloc(contact list offsets)"));
     TRY(dalloc(s->pairs, (size_t)MAX_LIST * nr, "hipMalloc(compacted contacts)"));
     TRY(dalloc(s->pairF, (size_t)6 * MAX_LIST * nr, "hipMalloc(contact forces)"));
     if (!clump) TRY(dalloc(s->rowF, (size_t)6 * nr, "hipMalloc(contact sums)"));

@@ -263,11 +263,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
         }
         ncount[t] = stored;
     }
-    block_count(stored, bsum, b);
-    if (counters) {
-        atomic_add_wave(&sc->particle_contacts, stored);
-        atomic_max_wave(&sc->pp_high, high);
-    }
+    block_count(stored, high, bsum, b);
     atomic_add_wave(&sc->pp_overflow, over);
 }
 
