@@ -131,9 +131,10 @@ def measure_adjoint(dm, scene, count, n, steps, K, seed):
     gradient of bed height + plate load w.r.t. every scene's (mu,
     restitution, E) through `steps` steps with checkpoints every K steps.
     adjoint_over_forward = (reverse sweep incl. recomputation) / (plain
-    forward run of the same steps); gradient_over_forward adds the
-    checkpointed forward run to the numerator (the whole cost of one
-    gradient against one plain forward run)."""
+    forward run of the same steps through dem_run_steps);
+    gradient_over_forward adds the checkpointed forward run to the
+    numerator (the whole cost of one gradient against one plain forward
+    run).  forward_python_loop_s: the plain run stepped from Python."""
     import time as _t
 
     import numpy as np
@@ -141,13 +142,27 @@ def measure_adjoint(dm, scene, count, n, steps, K, seed):
 
     scenes = scene.settle_scenes(count=count, n=n, seed=seed)
     batch = scene.SceneBatch(scenes)
+    dt = batch.combined.dt
+    # the plain forward run: the multi-step path (one call, no host work
+    # between steps), in the differentiable run's mode (candidate lists)
+    s = dm.CudaDemSolver()
+    s.initialize_batch(batch)
+    s.synchronize()
+    t0 = _t.perf_counter()
+    s.compute_forces(collect=True)
+    s.init_half_step(0.5 * dt)
+    s.run_steps(steps, dt)
+    s.synchronize()
+    t_fwd = _t.perf_counter() - t0
+    s.close()
+    # the same run driven step by step from Python (the reference's loop)
     s = dm.CudaDemSolver()
     s.initialize_batch(batch)
     s.synchronize()
     t0 = _t.perf_counter()
     host.run_pure(s, batch.combined, steps=steps)
     s.synchronize()
-    t_fwd = _t.perf_counter() - t0
+    t_loop = _t.perf_counter() - t0
     s.close()
     q = dm.CudaDemSolver()
     q.initialize_batch(batch)
@@ -162,7 +177,8 @@ def measure_adjoint(dm, scene, count, n, steps, K, seed):
     mem = q.device_memory_bytes()
     q.close()
     return {"scenes": count, "particles": batch.n, "steps": steps, "checkpoint_interval": K,
-            "forward_s": t_fwd, "forward_with_checkpoints_s": t_dfwd, "backward_s": t_bwd,
+            "forward_s": t_fwd, "forward_python_loop_s": t_loop,
+            "forward_with_checkpoints_s": t_dfwd, "backward_s": t_bwd,
             "adjoint_over_forward": t_bwd / t_fwd,
             "gradient_over_forward": (t_dfwd + t_bwd) / t_fwd, "device_bytes": mem,
             "finite_gradients": bool(np.all(np.isfinite(g))),

@@ -1711,7 +1711,14 @@ __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long 
         if (slot < 0) continue;
         double* hv = H.pwVal + (hr * PW_SLOTS + slot) * 9;
         if (A.tape.wallCount) {
-            const unsigned e = atomicAdd(A.tape.wallCount, 1u);
+            // one tape entry per wall contact, reserved for the wave's
+            // contacting lanes with one atomic (the lowest lane's)
+            const unsigned long long act = __ballot(1);
+            const int lane = threadIdx.x & 63, lead = __ffsll((long long)act) - 1;
+            unsigned first = 0u;
+            if (lane == lead) first = atomicAdd(A.tape.wallCount, (unsigned)__popcll(act));
+            first = __builtin_amdgcn_readlane(first, lead);
+            const unsigned e = first + (unsigned)__popcll(act & ((1ull << lane) - 1ull));
             if (A.tape.wall) {
                 A.tape.wall[e] = make_int4((int)r, w, slot, 0);
                 for (int k = 0; k < 9; ++k) A.tape.wallPrev[(long)e * 9 + k] = hv[k];
