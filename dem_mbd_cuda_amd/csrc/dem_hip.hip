@@ -1487,17 +1487,26 @@ __global__ void __launch_bounds__(BLOCK) k_off_scan(unsigned* bsum, int nblocks,
     }
 }
 
+// The block's entries are one contiguous run of the compacted list: they are
+// put in order in LDS and written out in full lines (a block with more
+// entries than the stage writes each row's run directly).
+constexpr int OFF_STAGE = 2048;
 __global__ void __launch_bounds__(BLOCK) k_off_apply(const int* cnt, long n, const unsigned* bsum,
                                                      const int* nbr, unsigned* off, int2* pairs)
 {
+    __shared__ int2 stage[OFF_STAGE];
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const int c = t < n ? cnt[t] : 0;
     unsigned tot;
     const unsigned ex = block_exclusive_scan((unsigned)c, &tot);
-    if (t < n) {
-        const unsigned base = bsum[blockIdx.x] + ex;
-        off[t] = base;
-        for (int k = 0; k < c; ++k) pairs[base + k] = make_int2((int)t, nbr[(long)k * n + t]);
+    const unsigned first = bsum[blockIdx.x];
+    if (t < n) off[t] = first + ex;
+    if (tot <= (unsigned)OFF_STAGE) {
+        for (int k = 0; k < c; ++k) stage[ex + k] = make_int2((int)t, nbr[(long)k * n + t]);
+        __syncthreads();
+        for (unsigned i = threadIdx.x; i < tot; i += BLOCK) pairs[first + i] = stage[i];
+    } else {
+        for (int k = 0; k < c; ++k) pairs[first + ex + k] = make_int2((int)t, nbr[(long)k * n + t]);
     }
 }
 
@@ -1518,8 +1527,9 @@ __device__ __forceinline__ bool row_in_chunk(long base, int cnt)
 // every row's history sits in its own row (the identity order and I_HIST of
 // the rebuilds), so no index is looked up.  ONE: a single scene, whose law
 // is uniform across the grid, passed by value (scalar registers).
+// (kept at 3 waves per SIMD: at most 168 VGPRs)
 template <bool ROWS, bool ONE>
-__global__ void __launch_bounds__(BLOCK) k_pairs(Soa S, const int* sorted, Recs rec,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) k_pairs(Soa S, const int* sorted, Recs rec,
                                                  const int2* pairs, const unsigned* off, const int* ncount,
                                                  History H, const Material* mats, Material mat1,
                                                  bool first, bool offs, const Scalars* sc, double* pairF,
@@ -1686,6 +1696,9 @@ __device__ __forceinline__ V3 field_force(const LawArgs& A, long p, double mass)
 // Walls of one contact row r (history row hr) in wall order (wallContact,
 // contact:975-1063), after the row's wall-history expiry (expireHistory,
 // contact:217-229: slots not seen last step are dropped); adds to wF / wT.
+// TAPE: the differentiable run's records are compiled in (k_law's variant
+// that finishes the step never runs with a tape).
+template <bool TAPE = true>
 __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long r, long hr, int scene, V3& wF,
                                           V3& wT, int& bodyHigh, int& pwOver)
 {
@@ -1694,7 +1707,7 @@ __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long 
     const int s = A.step;
     unsigned pwOcc = H.pwOcc[hr] & H.pwSeen[(long)((s + 1) & 1) * nr + hr];
     unsigned pwCur = 0;
-    if (A.tape.pwOcc) A.tape.pwOcc[hr] = pwOcc;
+    if (TAPE && A.tape.pwOcc) A.tape.pwOcc[hr] = pwOcc;
     // walls of the row's scene (history key: the scene's wall index + 1)
     const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
     Material mw = A.matPW[scene];
@@ -1710,7 +1723,7 @@ __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long 
                                  PW_SLOTS, (uint32_t)(w + 1), bodyHigh, pwOver);
         if (slot < 0) continue;
         double* hv = H.pwVal + (hr * PW_SLOTS + slot) * 9;
-        if (A.tape.wallCount) {
+        if (TAPE && A.tape.wallCount) {
             // one tape entry per wall contact, reserved for the wave's
             // contacting lanes with one atomic (the lowest lane's)
             const unsigned long long act = __ballot(1);
@@ -1725,7 +1738,7 @@ __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long 
             }
         }
         const V3 cf = pw_law(I, outward, depth, mw, hv, wF, wT);
-        if (A.tape.plateAcc && w == A.tape.plate && !A.first) A.tape.plateAcc[r] += cf.z;
+        if (TAPE && A.tape.plateAcc && w == A.tape.plate && !A.first) A.tape.plateAcc[r] += cf.z;
     }
     H.pwOcc[hr] = pwOcc;
     H.pwSeen[(long)(s & 1) * nr + hr] = pwCur;
@@ -2078,9 +2091,10 @@ __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3
 // acceleration here, from registers, instead of in k_finish_advance (no row
 // reads another row's state or record in this kernel), and the first block
 // resets the next evaluation's step counters (this evaluation's are not read:
-// it collects no diagnostics).
+// it collects no diagnostics).  Kept at 3 waves per SIMD (at most 168
+// VGPRs) without universal gravitation.
 template <bool UNIVERSAL, bool FINISH>
-__global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIVERSAL ? 2 : 3))) k_law(LawArgs A)
 {
     if (FINISH && blockIdx.x == 0) reset_step_scalars(A.sc, A.sceneAcc, A.nscenes);
     const long t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -2108,12 +2122,12 @@ __global__ void __launch_bounds__(BLOCK) k_law(LawArgs A)
         if (touch) {
             Body I = rec_body(A.rec, p);
             if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
-            wall_pass(A, I, p, hr, scene, wF, wT, bodyHigh, pwOver);
+            wall_pass<!FINISH>(A, I, p, hr, scene, wF, wT, bodyHigh, pwOver);
         } else {
             // expiry only: no wall slot is seen this step
             const History& H = A.H;
             const unsigned occ = H.pwOcc[hr] & H.pwSeen[(long)((A.step + 1) & 1) * n + hr];
-            if (A.tape.pwOcc) A.tape.pwOcc[hr] = occ;
+            if (!FINISH && A.tape.pwOcc) A.tape.pwOcc[hr] = occ;
             H.pwOcc[hr] = occ;
             H.pwSeen[(long)(A.step & 1) * n + hr] = 0u;
         }
