@@ -211,7 +211,9 @@ def measure_hopper(dm, scene, args, packing, steps, warmup, timing_steps):
     s.run_steps(steps, dt, graph=True)
     s.synchronize()
     elapsed = time.perf_counter() - t1
-    window = _counter_delta(c0, s.run_counters())
+    c1 = s.run_counters()
+    window = _counter_delta(c0, c1)
+    window["skin_m"] = c1.get("skin_pm", 0) * 1e-12  # candidate-list skin at the end of the window
     tms = []
     s.enable_timings(True)
     for _ in range(timing_steps):

@@ -337,11 +337,11 @@ class Solver:
         """dem_debug_run_counters as a dict (None for a library without it)."""
         if not self.lib.has("debug_run_counters"):
             return None
-        buf = (C.c_longlong * 8)()
+        buf = (C.c_longlong * 9)()
         cnt = C.c_int(0)
-        self.lib.call("debug_run_counters", self.h, buf, 8, C.byref(cnt))
+        self.lib.call("debug_run_counters", self.h, buf, 9, C.byref(cnt))
         names = ("evaluations", "sorts", "rebuilds", "graph_steps", "eager_steps", "last_sorted",
-                 "scan_lds_passes", "scan_hbm_passes")
+                 "scan_lds_passes", "scan_hbm_passes", "skin_pm")
         return {names[i]: int(buf[i]) for i in range(cnt.value)}
 
     def debug_history(self):

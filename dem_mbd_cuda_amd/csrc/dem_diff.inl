@@ -554,7 +554,7 @@ int diff_reorder(dem_solver* s)
     hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
     hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dSp, s->offs, s->sc, s->boxPart,
                        s->partScene, s->sceneAcc, (const double*)nullptr, 0.0, (double4*)nullptr,
-                       (uint32_t*)nullptr, (int4*)nullptr);
+                       (uint32_t*)nullptr, (int4*)nullptr, (double*)nullptr);
     hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,
                        s->partScene, s->sceneAcc, s->dGrid, s->nscenes, s->cellCap, s->gridDone);
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dGrid, s->keyA, s->valA);

@@ -110,6 +110,7 @@ constexpr int PW_SLOTS = DEM_WALL_HISTORY_SLOTS;
 constexpr int PT_SLOTS = DEM_BODY_HISTORY_SLOTS;
 constexpr int BODY_ROW = 29;  // per (body, row): 6 load components, Fz, 22 microstate terms
 constexpr int MAX_LIST = PP_SLOTS;  // contact-list capacity per component
+constexpr long kSkinSteps = 1024;   // steps a candidate list is sized to last (adaptive skin)
 constexpr int SLOT_BITS = 6;        // list entry = partner row << SLOT_BITS | slot
 static_assert((1 << SLOT_BITS) == PP_SLOTS, "slot field width");
 constexpr long MAX_PARTICLES = (1L << (31 - SLOT_BITS)) - 1;
@@ -418,12 +419,14 @@ __global__ void __launch_bounds__(BLOCK) k_expand(Soa S, Soa R, const int* owner
 // a step without rebuild then uses as they are.
 __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, bool offs, Scalars* sc,
                                                  int* part, int* partScene, int* sceneAcc, const double* refC,
-                                                 double halfSkin2, double4* posr, uint32_t* cidS, int4* meta)
+                                                 double halfSkin2, double4* posr, uint32_t* cidS, int4* meta,
+                                                 double* dispMax)
 {
     const long p = (long)blockIdx.x * BLOCK + threadIdx.x;
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
     int scene = -1;
     int moved = 0;
+    double disp2 = 0.0;
     if (p < S.n) {
         scene = S.k(I_SCENE, p);
         const double inv = sp[scene].inv;
@@ -437,7 +440,8 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
         }
         if (refC) {
             const double ex = cx - refC[p], ey = cy - refC[S.n + p], ez = cz - refC[2 * S.n + p];
-            moved = ex * ex + ey * ey + ez * ez > halfSkin2 ? 1 : 0;
+            disp2 = ex * ex + ey * ey + ez * ez;
+            moved = disp2 > halfSkin2 ? 1 : 0;
         }
         const double fx = floor(cx * inv), fy = floor(cy * inv), fz = floor(cz * inv);
         const double minCell = -(double)(1 << 20), maxCell = (double)((1 << 20) - 1);
@@ -460,6 +464,19 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
         }
     }
     if (refC) atomic_max_wave(&sc->need_rebuild, moved);
+    if (dispMax) {
+        // the block's largest displacement since the rebuild (read by the
+        // host now and then to size the skin)
+        __shared__ double dmax[BLOCK / 64];
+        disp2 = wave_max(disp2);
+        if ((threadIdx.x & 63) == 0) dmax[threadIdx.x >> 6] = disp2;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = dmax[0];
+            for (int w = 1; w < BLOCK / 64; ++w) m = fmax(m, dmax[w]);
+            dispMax[blockIdx.x] = m;
+        }
+    }
     __shared__ int red[BLOCK / 64][8];
     const int wave = threadIdx.x >> 6;
     {
@@ -501,6 +518,21 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
                 }
         }
         if (threadIdx.x == 0) partScene[blockIdx.x] = -1;
+    }
+}
+
+// The largest of the blocks' displacements (k_cells' dispMax) into *out.
+__global__ void __launch_bounds__(BLOCK) k_disp_max(const double* dispMax, int nblocks, double* out)
+{
+    __shared__ double w[BLOCK / 64];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += BLOCK) m = fmax(m, dispMax[b]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < BLOCK / 64; ++k) m = fmax(m, w[k]);
+        *out = m;
     }
 }
 
@@ -2903,6 +2935,16 @@ struct dem_solver {
     bool verlet = false;       // active
     bool vlistValid = false;   // lists built and not yet invalidated
     double skin = 0.0;
+    // skin sized to the motion (no DEM_SKIN given): every so often the host
+    // reads the largest displacement since the rebuild and, if a much
+    // thinner skin would still last kSkinSteps steps, rebuilds with it; a
+    // rebuild that comes sooner than a quarter of that widens the skin
+    bool skinAdapt = false;
+    double skinMin = 0.0, skinMax = 0.0;
+    long sinceRebuild = 0, nextSkinCheck = 0;
+    double* dispMax = nullptr;  // [blocks] k_cells' largest displacement since the rebuild
+    double* dDisp = nullptr;    // the reduced maximum
+    double* hDisp = nullptr;    // pinned copy
     double* refC = nullptr;    // [3][n] centres at the last rebuild
     int* cand = nullptr;       // [CAND_MAX][n]
     int* ncand = nullptr;      // [n]
@@ -3057,7 +3099,7 @@ struct dem_solver {
         diff_release(diff);
         graph_release(graphs);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB, ident, mKey, mVal,
-                        rec, recAux, refC, cand, ncand, meta, ncid, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
+                        rec, recAux, refC, cand, ncand, meta, ncid, dispMax, dDisp, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, rowF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
@@ -3073,6 +3115,7 @@ struct dem_solver {
         if (hScalars) hipHostFree(hScalars);
         if (hFlag) hipHostFree(hFlag);
         if (hMirror) hipHostFree(hMirror);
+        if (hDisp) hipHostFree(hDisp);
         if (hTri) hipHostFree(hTri);
         if (hTriOff) hipHostFree(hTriOff);
         for (auto& e : ev)
@@ -3424,12 +3467,26 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
         s->skin = frac * rmin;
         bool ok = frac > 0.0 && !clump;
         for (int k = 0; k < nsc && ok; ++k) ok = 2.0 / s->hsp[k].inv >= 2.0 * maxrScene[k] + s->skin;
+        if (ok && !e) {
+            // the skin follows the motion unless DEM_SKIN fixes it (or DEM_SKIN_ADAPT=0)
+            const char* a = getenv("DEM_SKIN_ADAPT");
+            s->skinAdapt = !(a && atoi(a) == 0);
+            s->skinMin = 0.025 * rmin;
+            s->skinMax = 0.4 * rmin;
+            for (int k = 0; k < nsc; ++k) s->skinMax = std::min(s->skinMax, 2.0 / s->hsp[k].inv - 2.0 * maxrScene[k]);
+            s->skinMax = std::max(s->skinMax, s->skin);
+        }
         if (ok) {
             TRY(dalloc(s->refC, (size_t)3 * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->cand, (size_t)CAND_MAX * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->ncand, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->meta, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->ncid, (size_t)MAX_LIST * n, "hipMalloc(candidate lists)"));
+            if (s->skinAdapt) {
+                TRY(dalloc(s->dispMax, blocks_for(n), "hipMalloc(candidate lists)"));
+                TRY(dalloc(s->dDisp, 1, "hipMalloc(candidate lists)"));
+                HIP_TRY(hipHostMalloc((void**)&s->hDisp, sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+            }
             TRY(dalloc(s->ident, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->mKey, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->mVal, n, "hipMalloc(candidate lists)"));
@@ -3870,13 +3927,32 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     // candidate lists (single-sphere owners; body loads are reduced by input row,
     // so coupled steps need no per-step sort either)
     const bool verlet = s->verlet;
+    bool shrunk = false;  // this step rebuilds the lists with a thinner skin
+    if (verlet && s->vlistValid && s->skinAdapt && s->sinceRebuild >= s->nextSkinCheck && !s->capturing) {
+        // the largest displacement since the rebuild (k_cells of the last
+        // step): if the motion would let a much thinner skin last
+        // kSkinSteps steps, rebuild with it now
+        hipLaunchKernelGGL(k_disp_max, dim3(1), dim3(BLOCK), 0, st, (const double*)s->dispMax, blocks_for(nr),
+                           s->dDisp);
+        HIP_TRY(hipMemcpyAsync(s->hDisp, s->dDisp, sizeof(double), hipMemcpyDeviceToHost, st), "read displacement");
+        HIP_TRY(hipStreamSynchronize(st), "read displacement");
+        const double rate = std::sqrt(*s->hDisp) / (double)s->sinceRebuild;
+        const double want = std::min(s->skinMax, std::max(s->skinMin, 2.0 * rate * (double)kSkinSteps));
+        if (want <= 0.5 * s->skin) {
+            s->skin = want;
+            s->vlistValid = false;
+            shrunk = true;
+        }
+        s->nextSkinCheck *= 2;
+    }
     const bool lists = verlet && s->vlistValid;
     if (s->clump) hipLaunchKernelGGL(k_expand, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, R, s->cOwner);
     // a list step on current records gets k_posr's outputs from k_cells
     const bool fusedPosr = lists && s->recFresh;
     hipLaunchKernelGGL(k_cells, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dSp, s->offs, s->sc,
                        s->boxPart, s->partScene, s->sceneAcc, lists ? s->refC : nullptr,
-                       0.25 * s->skin * s->skin, fusedPosr ? s->posr : nullptr, s->cidS, s->meta);
+                       0.25 * s->skin * s->skin, fusedPosr ? s->posr : nullptr, s->cidS, s->meta,
+                       lists ? s->dispMax : nullptr);
     bool rebuild = true;
     bool triCounted = false;
     bool scanned = false;  // k_cells wrote k_posr's outputs and the list scan is queued
@@ -3929,6 +4005,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             rebuild = s->hScalars->need_rebuild != 0;
         }
     }
+    if (lists && !rebuild) ++s->sinceRebuild;
     bool triDone = false;
     if (rebuild) {
         hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,
@@ -4000,6 +4077,12 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     TRY(launch_check("grid build"));
     bool useLists = verlet;
     if (verlet && rebuild) {
+        if (s->skinAdapt) {
+            // lists the motion outran in under a quarter of kSkinSteps: widen
+            if (lists && !shrunk && s->sinceRebuild < kSkinSteps / 4) s->skin = std::min(s->skinMax, 2.0 * s->skin);
+            s->sinceRebuild = 0;
+            s->nextSkinCheck = 64;
+        }
         hipLaunchKernelGGL(k_vbuild, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->posr, s->keyA, s->valA,
                            s->cellStart, s->dGrid, s->sc, s->skin, s->cand, s->ncand, s->refC);
         TRY(launch_check("candidate lists"));
@@ -4403,9 +4486,10 @@ int dem_debug_run_counters(const dem_solver* s, long long* out, int capacity, in
                 "read scan counters");
         HIP_TRY(hipStreamSynchronize(s->stream), "read scan counters");
     }
-    const long long v[8] = {s->nEvaluations, s->nSorts, (long long)s->rebuilds, s->nGraphSteps,
-                            s->nEagerSteps, s->lastSorted ? 1 : 0, (long long)passes[0], (long long)passes[1]};
-    const int k = std::max(0, std::min(capacity, 8));
+    const long long v[9] = {s->nEvaluations, s->nSorts, (long long)s->rebuilds, s->nGraphSteps,
+                            s->nEagerSteps, s->lastSorted ? 1 : 0, (long long)passes[0], (long long)passes[1],
+                            s->verlet ? std::llround(s->skin * 1e12) : 0LL};
+    const int k = std::max(0, std::min(capacity, 9));
     for (int i = 0; i < k; ++i) out[i] = v[i];
     if (count) *count = k;
     return DEM_OK;

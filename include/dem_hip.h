@@ -260,7 +260,7 @@ int dem_debug_history(const dem_solver* solver, uint64_t* pp_keys,
                       double* pp_values, uint64_t* pw_keys, double* pw_values,
                       uint64_t* pt_keys, double* pt_values);
 /* Counters of the solver's work since initialisation (no reference
- * counterpart; for measurement and tests), out[0..7] as far as capacity
+ * counterpart; for measurement and tests), out[0..8] as far as capacity
  * allows: force evaluations; evaluations that sorted the particle grid on
  * the device; candidate-list rebuilds; steps of dem_run_steps replayed from
  * HIP graphs; steps of dem_run_steps launched eagerly; 1 if the last
@@ -268,7 +268,8 @@ int dem_debug_history(const dem_solver* solver, uint64_t* pp_keys,
  * sorted order as it is; between candidate-list rebuilds it puts the current
  * cells in (key, index) order on the host); block passes of the full 27-cell
  * scan on diagnostics evaluations whose candidates were staged in LDS, and
- * those that read them from device memory. */
+ * those that read them from device memory; the candidate-list skin in
+ * picometres (0 without candidate lists). */
 int dem_debug_run_counters(const dem_solver* solver, long long* out, int capacity,
                            int* count);
 /* Triangle grid entries (key, triangle) in sorted order, [entries] each. */

@@ -351,3 +351,29 @@ def test_full_scan_staging_paths(hip, oracle, case, monkeypatch):
     else:
         assert c["scan_hbm_passes"] > c["scan_lds_passes"], c
     compare_frames(g, o)
+
+
+def test_adaptive_skin(hip, oracle, monkeypatch):
+    """The candidate-list skin follows the motion (no DEM_SKIN given): spheres
+    falling from rest under a strong field move slowly at first, so the
+    first check shrinks the skin and rebuilds; as they speed up, rebuilds come
+    sooner and the skin widens again.  Every frame equals the oracle's bit
+    for bit, whatever the skin (any skin lists a superset of the contacts)."""
+    from dem_mbd_cuda_amd import scene
+
+    monkeypatch.delenv("DEM_SKIN", raising=False)
+    monkeypatch.delenv("DEM_SKIN_ADAPT", raising=False)
+    # g = 500 m/s^2: 1 um of fall in the first 64 steps (the first check),
+    # then rebuilds every ~200 steps as the spheres speed up
+    sc = scene.box_scene(n=2000, seed=61, steps=500, gravity=(0.0, 0.0, -500.0))
+    sc.vel = np.random.RandomState(62).uniform(-0.005, 0.005, (sc.n, 3))
+    g = run_frames(hip, sc, steps=500, interval=25, debug=False)
+    o = run_frames(oracle, sc, steps=500, interval=25, debug=False)
+    skins = [g[k]["counters"]["skin_pm"] for k in sorted(g)]
+    rebuilds = g[max(g)]["counters"]["rebuilds"]
+    print("skin (pm) per frame:", skins, "rebuilds", rebuilds, flush=True)
+    start = skins[0]
+    assert min(skins) <= start // 4, skins      # shrunk while the spheres barely moved
+    assert skins[-1] >= 2 * min(skins), skins   # widened as they sped up
+    assert rebuilds >= 3  # the first evaluation, the shrink, at least one widening
+    compare_frames(g, o)
