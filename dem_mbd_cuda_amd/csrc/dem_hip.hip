@@ -145,10 +145,26 @@ struct Soa {
 // reordering, since the cell index below is scene-major), its walls
 // [wall0, wall0 + nwall) and its own cell size.
 struct SceneParams {
-    double inv;  // 1 / cell size
+    double inv;   // 1 / cell size
+    double maxr;  // the scene's largest component radius
     long row0, row1;
     int wall0, nwall;
 };
+
+// Distance along one axis from coordinate x to the slab of cells with index
+// c (cell size h); 0 inside.  A candidate scan skips a cell whose box lies
+// farther from the centre than any partner's reach: with cells assigned as
+// floor(x / h) in floating point, a centre lies in its cell's box up to a
+// relative 1e-16, far inside the margin cell_margin() adds.
+__device__ __forceinline__ double slab_gap(double x, long c, double h)
+{
+    const double lo = (double)c * h, hi = (double)(c + 1) * h;
+    return fmax(0.0, fmax(lo - x, x - hi));
+}
+__device__ __forceinline__ double cell_margin(double4 me, double h)
+{
+    return 1.0e-9 * (fabs(me.x) + fabs(me.y) + fabs(me.z) + h);
+}
 
 // Per-step grid of one scene: cell bounding box and the scene's first index
 // in the global linear cell numbering.
@@ -3329,6 +3345,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
     for (int k = 0; k < nsc; ++k) {
         SceneParams& q = s->hsp[k];
         q.inv = 1.0 / (scale * maxrScene[k]);
+        q.maxr = maxrScene[k];
         q.row0 = spec.row[k];
         q.row1 = clump ? nc : spec.row[k + 1];  // grid rows of the scene
         q.wall0 = spec.wall[k];
@@ -4084,7 +4101,8 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             s->nextSkinCheck = 64;
         }
         hipLaunchKernelGGL(k_vbuild, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->posr, s->keyA, s->valA,
-                           s->cellStart, s->dGrid, s->sc, s->skin, s->cand, s->ncand, s->refC);
+                           s->cellStart, s->dGrid, (const SceneParams*)s->dSp, s->sc, s->skin, s->cand, s->ncand,
+                           s->refC);
         TRY(launch_check("candidate lists"));
         if (s->capturing) return set_error(DEM_E_INVALID, "candidate-list steps cannot be captured");
         HIP_TRY(hipMemcpyAsync(s->hFlag + 1, &s->sc->cand_overflow, sizeof(int), hipMemcpyDeviceToHost, st),

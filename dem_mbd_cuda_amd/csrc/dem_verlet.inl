@@ -18,10 +18,15 @@ constexpr int CAND_MAX = 96;  // listed partners per component
 
 // Rebuild: partners within r_i + r_j + skin (rows; the grid's cell ranges
 // are sorted positions, mapped to rows by `sorted`), and the centres the
-// displacement test compares against.
+// displacement test compares against.  Cells whose box lies farther from the
+// centre than r_i + the scene's largest radius + skin are skipped (whole
+// columns, and the ends of a column's z run, so each column stays one sorted
+// range in the same order): with cells of 2 r_max the +-2 ring is out of
+// reach unless the centre is within a skin of its own cell's face.
 __global__ void __launch_bounds__(BLOCK) k_vbuild(Soa S, const double4* posr, const unsigned long long* key,
                                                   const int* sorted, const unsigned* start, const SceneGrid* grid,
-                                                  Scalars* sc, double skin, int* cand, int* ncand, double* refC)
+                                                  const SceneParams* sp, Scalars* sc, double skin, int* cand,
+                                                  int* ncand, double* refC)
 {
     const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
     const long n = S.n;
@@ -32,17 +37,37 @@ __global__ void __launch_bounds__(BLOCK) k_vbuild(Soa S, const double4* posr, co
     refC[2 * n + t] = me.z;
     int count = 0;
     if (S.k(I_VALID, t)) {
-        const SceneGrid& g = grid[S.k(I_SCENE, t)];
+        const int scene = S.k(I_SCENE, t);
+        const SceneGrid& g = grid[scene];
+        const double h = 1.0 / sp[scene].inv;
+        const double far = me.w + sp[scene].maxr + skin + cell_margin(me, h);
+        const double far2 = far * far;
         const long X = (long)g.dims[0], Y = (long)g.dims[1], Z = (long)g.dims[2];
-        const long cx = S.k(I_CX, t) - g.lo[0], cy = S.k(I_CY, t) - g.lo[1], cz = S.k(I_CZ, t) - g.lo[2];
-        const long zlo = cz > 1 ? cz - 2 : 0, zhi = cz + 2 < Z ? cz + 2 : Z - 1;
+        const long ax = S.k(I_CX, t), ay = S.k(I_CY, t), az = S.k(I_CZ, t);
+        const long cx = ax - g.lo[0], cy = ay - g.lo[1], cz = az - g.lo[2];
         const bool dense = sc->dense;
         for (long dx = -2; dx <= 2; ++dx) {
             const long xx = cx + dx;
             if (xx < 0 || xx >= X) continue;
+            const double gx = slab_gap(me.x, ax + dx, h);
+            if (gx >= far) continue;
             for (long dy = -2; dy <= 2; ++dy) {
                 const long yy = cy + dy;
                 if (yy < 0 || yy >= Y) continue;
+                const double gy = slab_gap(me.y, ay + dy, h);
+                const double gxy = gx * gx + gy * gy;
+                if (gxy >= far2) continue;
+                long zlo = cz > 1 ? cz - 2 : 0, zhi = cz + 2 < Z ? cz + 2 : Z - 1;
+                while (zlo < cz) {
+                    const double gz = slab_gap(me.z, zlo + g.lo[2], h);
+                    if (gxy + gz * gz < far2) break;
+                    ++zlo;
+                }
+                while (zhi > cz) {
+                    const double gz = slab_gap(me.z, zhi + g.lo[2], h);
+                    if (gxy + gz * gz < far2) break;
+                    --zhi;
+                }
                 const unsigned long long col = g.base + ((unsigned long long)xx * Y + yy) * Z;
                 long b, e;
                 if (dense) {
