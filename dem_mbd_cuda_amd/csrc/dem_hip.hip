@@ -994,8 +994,8 @@ __device__ __forceinline__ Body rec_body(const Recs& R, long p)
 // values they just stored, saving k_pack's second pass over the state).  The
 // constant part stays as the last k_pack wrote it: rows only change place
 // in a reorder or a reload, and those clear the records' freshness.
-__device__ __forceinline__ void put_rec(const Recs& R, const Soa& S, long p, V3 pos, V3 vel, V3 vh, double qw,
-                                        double qx, double qy, double qz, V3 w, V3 wh)
+__device__ __forceinline__ void put_rec(const Recs& R, long p, V3 pos, V3 vel, V3 vh, double qw, double qx,
+                                        double qy, double qz, V3 w, V3 wh, double mass)
 {
     Rec q;
     const V3 ww = rot(qw, qx, qy, qz, w), wwh = rot(qw, qx, qy, qz, wh);
@@ -1004,8 +1004,13 @@ __device__ __forceinline__ void put_rec(const Recs& R, const Soa& S, long p, V3 
     q.hx = vh.x; q.hy = vh.y; q.hz = vh.z;
     q.wx = ww.x; q.wy = ww.y; q.wz = ww.z;
     q.ux = wwh.x; q.uy = wwh.y; q.uz = wwh.z;
-    q.mass = S.f(F_MASS, p);
+    q.mass = mass;
     R.main[p] = q;
+}
+__device__ __forceinline__ void put_rec(const Recs& R, const Soa& S, long p, V3 pos, V3 vel, V3 vh, double qw,
+                                        double qx, double qy, double qz, V3 w, V3 wh)
+{
+    put_rec(R, p, pos, vel, vh, qw, qx, qy, qz, w, wh, S.f(F_MASS, p));
 }
 
 __global__ void __launch_bounds__(BLOCK) k_pack(Soa S, bool offs, Recs R)
@@ -2050,28 +2055,44 @@ __device__ __forceinline__ void tri_pass(const LawArgs& A, const Body& I, long r
     tri_pass_geo(A, I, r, u, hr, o, listed, geo, bF, bT, bodyHigh, ptOver, bcount, depthBits);
 }
 
+// A row's orientation, angular velocity and mass at the end of its force
+// evaluation (what its acceleration was computed from).
+struct RowSpin {
+    double qw, qx, qy, qz, wx, wy, wz, mass;
+};
+
+__device__ __forceinline__ RowSpin load_spin(const Soa& S, long p)
+{
+    return RowSpin{S.f(F_QW, p), S.f(F_QX, p), S.f(F_QY, p), S.f(F_QZ, p),
+                   S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p), S.f(F_MASS, p)};
+}
+
 // advanceHalfVelocityKernel of the previous step followed by
 // advancePositionKernel of the next one (particle:213-265) for row p, in one
 // pass over its state, given the row's accelerations (a, aw) of the force
 // evaluation in between: the reference's loop calls them back to back with
-// only host work (the body solve) in between.
-__device__ __forceinline__ void finish_advance_row(const Soa& S, long p, double dtf, double dt, const Recs& R,
-                                                   double ax, double ay, double az, double awx, double awy,
-                                                   double awz)
+// only host work (the body solve) in between.  KEEP_HALF_QUAT: also store the
+// half-step quaternion rate (F_HQ*), which only k_advance reads; the law
+// kernel's fused pass leaves it out, since the run's next step rewrites it
+// in its own finish before any advance could read it.
+template <bool KEEP_HALF_QUAT>
+__device__ __forceinline__ void finish_advance_spin(const Soa& S, long p, double dtf, double dt, const Recs& R,
+                                                    const double (&a)[6], const RowSpin& r)
 {
+    const double ax = a[0], ay = a[1], az = a[2], awx = a[3], awy = a[4], awz = a[5];
     double vhx = S.f(F_VHX, p) + ax * dtf, vhy = S.f(F_VHY, p) + ay * dtf, vhz = S.f(F_VHZ, p) + az * dtf;
     double whx = S.f(F_WHX, p) + awx * dtf, why = S.f(F_WHY, p) + awy * dtf, whz = S.f(F_WHZ, p) + awz * dtf;
-    double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
+    double qw = r.qw, qx = r.qx, qy = r.qy, qz = r.qz;
     const double h = 0.5 * dtf;
     double dw, dx, dy, dz;
-    quat_dot(qw, qx, qy, qz, S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p), dw, dx, dy, dz);
+    quat_dot(qw, qx, qy, qz, r.wx, r.wy, r.wz, dw, dx, dy, dz);
     double hw = qw + dw * h, hx = qx + dx * h, hy = qy + dy * h, hz = qz + dz * h;
     quat_unit(hw, hx, hy, hz);
     double gw, gx, gy, gz;
     quat_dot(hw, hx, hy, hz, whx, why, whz, gw, gx, gy, gz);
     S.f(F_VHX, p) = vhx; S.f(F_VHY, p) = vhy; S.f(F_VHZ, p) = vhz;
     S.f(F_WHX, p) = whx; S.f(F_WHY, p) = why; S.f(F_WHZ, p) = whz;
-    S.f(F_HQW, p) = gw; S.f(F_HQX, p) = gx; S.f(F_HQY, p) = gy; S.f(F_HQZ, p) = gz;
+    if (KEEP_HALF_QUAT) { S.f(F_HQW, p) = gw; S.f(F_HQX, p) = gx; S.f(F_HQY, p) = gy; S.f(F_HQZ, p) = gz; }
     V3 pos = mk(S.f(F_X, p), S.f(F_Y, p), S.f(F_Z, p)), vel, w;
     if (S.k(I_STATUS, p) != -1) {
         pos = mk(pos.x + vhx * dt, pos.y + vhy * dt, pos.z + vhz * dt);
@@ -2086,34 +2107,52 @@ __device__ __forceinline__ void finish_advance_row(const Soa& S, long p, double 
     } else {
         if (!R.main) return;
         vel = mk(S.f(F_VX, p), S.f(F_VY, p), S.f(F_VZ, p));
-        w = mk(S.f(F_WX, p), S.f(F_WY, p), S.f(F_WZ, p));
+        w = mk(r.wx, r.wy, r.wz);
     }
-    if (R.main) put_rec(R, S, p, pos, vel, mk(vhx, vhy, vhz), qw, qx, qy, qz, w, mk(whx, why, whz));
+    if (R.main) put_rec(R, p, pos, vel, mk(vhx, vhy, vhz), qw, qx, qy, qz, w, mk(whx, why, whz), r.mass);
+}
+
+__device__ __forceinline__ void finish_advance_row(const Soa& S, long p, double dtf, double dt, const Recs& R,
+                                                   double ax, double ay, double az, double awx, double awy,
+                                                   double awz)
+{
+    const double a[6] = {ax, ay, az, awx, awy, awz};
+    finish_advance_spin<true>(S, p, dtf, dt, R, a, load_spin(S, p));
 }
 
 // Force outputs of owner row p (contact:1211-1220) and its acceleration
-// (accelerationKernel, particle:71-101), also returned in acc.
+// (accelerationKernel, particle:71-101), also returned in acc, with the
+// orientation and angular velocity it was computed from in spin.  STORE:
+// write the force outputs and the acceleration.  The law kernel's fused pass
+// (a step that another step of the same run follows) keeps them in
+// registers: the next step's evaluation rewrites every row's force outputs
+// and acceleration before anything reads them.
+template <bool STORE = true>
 __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3 cF, V3 cT, V3 fF, V3 wF, V3 wT,
-                                          V3 bF, V3 bT, double (&acc)[6])
+                                          V3 bF, V3 bT, double (&acc)[6], RowSpin& spin)
 {
     const Soa& S = A.S;
     const long n = S.n;
-    double* oc = A.O.contact;
-    oc[0 * n + p] = cF.x; oc[1 * n + p] = cF.y; oc[2 * n + p] = cF.z;
-    oc[3 * n + p] = cT.x; oc[4 * n + p] = cT.y; oc[5 * n + p] = cT.z;
-    double* ow = A.O.wall;
-    ow[0 * n + p] = wF.x; ow[1 * n + p] = wF.y; ow[2 * n + p] = wF.z;
-    ow[3 * n + p] = wT.x; ow[4 * n + p] = wT.y; ow[5 * n + p] = wT.z;
-    double* of = A.O.field;
-    of[0 * n + p] = fF.x; of[1 * n + p] = fF.y; of[2 * n + p] = fF.z;
+    if (STORE) {
+        double* oc = A.O.contact;
+        oc[0 * n + p] = cF.x; oc[1 * n + p] = cF.y; oc[2 * n + p] = cF.z;
+        oc[3 * n + p] = cT.x; oc[4 * n + p] = cT.y; oc[5 * n + p] = cT.z;
+        double* ow = A.O.wall;
+        ow[0 * n + p] = wF.x; ow[1 * n + p] = wF.y; ow[2 * n + p] = wF.z;
+        ow[3 * n + p] = wT.x; ow[4 * n + p] = wT.y; ow[5 * n + p] = wT.z;
+        double* of = A.O.field;
+        of[0 * n + p] = fF.x; of[1 * n + p] = fF.y; of[2 * n + p] = fF.z;
+    }
     const V3 tF = add(add(cF, fF), add(wF, bF));
     const V3 tT = add(cT, add(wT, bT));
     acc[0] = tF.x / m;
     acc[1] = tF.y / m;
     acc[2] = tF.z / m;
-    S.f(F_AX, p) = acc[0];
-    S.f(F_AY, p) = acc[1];
-    S.f(F_AZ, p) = acc[2];
+    if (STORE) {
+        S.f(F_AX, p) = acc[0];
+        S.f(F_AY, p) = acc[1];
+        S.f(F_AZ, p) = acc[2];
+    }
     const double qw = S.f(F_QW, p), qx = S.f(F_QX, p), qy = S.f(F_QY, p), qz = S.f(F_QZ, p);
     const double tx = (1.0 - 2.0 * (qy * qy + qz * qz)) * tT.x + 2.0 * (qx * qy + qw * qz) * tT.y +
                       2.0 * (qx * qz - qw * qy) * tT.z;
@@ -2126,9 +2165,18 @@ __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3
     acc[3] = (tx - (iz - iy) * wz * wy) / ix;
     acc[4] = (ty - (ix - iz) * wx * wz) / iy;
     acc[5] = (tz - (iy - ix) * wy * wx) / iz;
-    S.f(F_AWX, p) = acc[3];
-    S.f(F_AWY, p) = acc[4];
-    S.f(F_AWZ, p) = acc[5];
+    if (STORE) {
+        S.f(F_AWX, p) = acc[3];
+        S.f(F_AWY, p) = acc[4];
+        S.f(F_AWZ, p) = acc[5];
+    }
+    spin = RowSpin{qw, qx, qy, qz, wx, wy, wz, m};
+}
+__device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3 cF, V3 cT, V3 fF, V3 wF, V3 wT,
+                                          V3 bF, V3 bT, double (&acc)[6])
+{
+    RowSpin spin;
+    owner_out<true>(A, p, m, cF, cT, fF, wF, wT, bF, bT, acc, spin);
 }
 
 // computeContactsKernel (contact:1065-1221) for one single-sphere owner,
@@ -2139,8 +2187,10 @@ __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3
 // acceleration here, from registers, instead of in k_finish_advance (no row
 // reads another row's state or record in this kernel), and the first block
 // resets the next evaluation's step counters (this evaluation's are not read:
-// it collects no diagnostics).  Kept at 3 waves per SIMD (at most 168
-// VGPRs) without universal gravitation.
+// it collects no diagnostics).  Such a step stores neither its force outputs
+// nor its accelerations nor the half-step quaternion rate: the run's next
+// step rewrites all three before anything reads them.  Kept at 3 waves per
+// SIMD (at most 168 VGPRs) without universal gravitation.
 template <bool UNIVERSAL, bool FINISH>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIVERSAL ? 2 : 3))) k_law(LawArgs A)
 {
@@ -2200,9 +2250,9 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIV
             bT = mk(ob_[3 * n + p], ob_[4 * n + p], ob_[5 * n + p]);
         }
         double acc[6];
-        owner_out(A, p, mass, cF, cT, fF, wF, wT, bF, bT, acc);
-        if (FINISH)
-            finish_advance_row(S, p, A.finishDt, A.nextDt, A.rec, acc[0], acc[1], acc[2], acc[3], acc[4], acc[5]);
+        RowSpin spin;
+        owner_out<!FINISH>(A, p, mass, cF, cT, fF, wF, wT, bF, bT, acc, spin);
+        if (FINISH) finish_advance_spin<false>(S, p, A.finishDt, A.nextDt, A.rec, acc, spin);
     }
     if (A.counters) atomic_max_block(&A.sc->body_high, bodyHigh);
     atomic_add_wave(&A.sc->pw_overflow, pwOver);
