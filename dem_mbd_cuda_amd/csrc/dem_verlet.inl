@@ -295,25 +295,51 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
 // History rows follow the particles: after a rebuild reordered the particle
 // arrays, row t's history moves from its old row (I_HIST) to row t, so the
 // per-contact history traffic of the following steps is as local as the
-// particle data.  Only the slots below the highest occupied one are copied
-// (the others are never read before being reset).
+// particle data.  Only the slots below the block's highest occupied one are
+// copied (a free slot is never read before acquisition resets it, so the
+// free slots of a row below that bound may be copied too).  The pair slots
+// move as whole tiles: for each slot, the block's rows are one contiguous
+// run of 9-f64 values in the destination, copied one f64 per lane, and the
+// rows' keys move in 16-byte quads.
 __global__ void __launch_bounds__(BLOCK) k_hist_move(Soa S, History src, History dst)
 {
-    const long t = (long)blockIdx.x * BLOCK + threadIdx.x;
+    __shared__ long hrow[BLOCK];
+    __shared__ int topw[BLOCK / 64];
+    const long t0 = (long)blockIdx.x * BLOCK;
+    const long t = t0 + threadIdx.x;
     const long n = S.n;
-    if (t >= n) return;
-    const long h = S.k(I_HIST, t);
-    const unsigned long long occ = src.ppOcc[h];
-    dst.ppOcc[t] = occ;
-    dst.ppSeen[t] = src.ppSeen[h];
-    dst.ppSeen[n + t] = src.ppSeen[n + h];
-    const int top = occ ? 64 - __clzll((long long)occ) : 0;
-    for (int b = 0; b < top; ++b) dst.ppKey[t * PP_SLOTS + b] = src.ppKey[h * PP_SLOTS + b];
-    for (int b = 0; b < top; ++b) {
-        const double* sv = pp_slot(src, h, b);
-        double* dv = pp_slot(dst, t, b);
-        for (int k = 0; k < 9; ++k) dv[k] = sv[k];
+    const int rowsHere = (int)min((long)BLOCK, n - t0);
+    int top = 0;
+    if (t < n) {
+        const long h = S.k(I_HIST, t);
+        hrow[threadIdx.x] = h;
+        const unsigned long long occ = src.ppOcc[h];
+        dst.ppOcc[t] = occ;
+        dst.ppSeen[t] = src.ppSeen[h];
+        dst.ppSeen[n + t] = src.ppSeen[n + h];
+        top = occ ? 64 - __clzll((long long)occ) : 0;
     }
+    top = wave_max(top);
+    if ((threadIdx.x & 63) == 0) topw[threadIdx.x >> 6] = top;
+    __syncthreads();
+    int bt = topw[0];
+    for (int w = 1; w < BLOCK / 64; ++w) bt = max(bt, topw[w]);
+    const int q4 = (bt + 3) >> 2;  // key quads per row
+    for (int e = threadIdx.x; e < rowsHere * q4; e += BLOCK) {
+        const int r = e / q4, q = e - r * q4;
+        reinterpret_cast<uint4*>(dst.ppKey + (t0 + r) * PP_SLOTS)[q] =
+            reinterpret_cast<const uint4*>(src.ppKey + hrow[r] * PP_SLOTS)[q];
+    }
+    for (int b = 0; b < bt; ++b) {
+        const double* sv = pp_slot(src, 0, b);
+        double* dv = pp_slot(dst, t0, b);
+        for (int e = threadIdx.x; e < rowsHere * 9; e += BLOCK) {
+            const int r = e / 9, k = e - r * 9;
+            dv[e] = sv[hrow[r] * 9 + k];
+        }
+    }
+    if (t >= n) return;
+    const long h = hrow[threadIdx.x];
     const unsigned wocc = src.pwOcc[h];
     dst.pwOcc[t] = wocc;
     dst.pwSeen[t] = src.pwSeen[h];
