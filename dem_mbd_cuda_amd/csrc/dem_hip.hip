@@ -426,6 +426,24 @@ __global__ void __launch_bounds__(BLOCK) k_expand(Soa S, Soa R, const int* owner
     R.k(I_STATUS, c) = S.k(I_STATUS, o);
 }
 
+// A row's cell and identity as the candidate-list scan reads them, one
+// 16-byte load per partner: the cell's three coordinates, each offset by
+// 2^20 into 21 bits, packed in x (low word) and y (high word), with bit 63
+// set for a component outside the hash range (adjacent to no cell); the
+// input row in z and the component id in w.
+__device__ __forceinline__ int4 pack_meta(bool ok, int cx, int cy, int cz, int orig, uint32_t cid)
+{
+    const unsigned long long c =
+        ok ? ((unsigned long long)(uint32_t)(cx + (1 << 20)) << 42) |
+                 ((unsigned long long)(uint32_t)(cy + (1 << 20)) << 21) | (unsigned long long)(uint32_t)(cz + (1 << 20))
+           : 1ull << 63;
+    return make_int4((int)(uint32_t)c, (int)(uint32_t)(c >> 32), orig, (int)cid);
+}
+__device__ __forceinline__ unsigned long long meta_cell(int4 m)
+{
+    return ((unsigned long long)(uint32_t)m.y << 32) | (uint32_t)m.x;
+}
+
 // component centre + cell + range check + bbox (grid:51-71), cell size of
 // the particle's scene.  The cell bounding box of each scene is reduced per
 // block (wave shuffles, then LDS) into part[block][6] when the block lies in
@@ -474,9 +492,9 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
         S.k(I_VALID, p) = ok ? 1 : 0;
         if (posr) {
             posr[p] = make_double4(cx, cy, cz, S.f(F_CR, p));
-            cidS[p] = (uint32_t)S.k(I_CID, p);
-            meta[p] = ok ? make_int4(lo[0], lo[1], lo[2], S.k(I_ORIG, p))
-                         : make_int4(INT_MIN / 2, INT_MIN / 2, INT_MIN / 2, S.k(I_ORIG, p));
+            const uint32_t cid = (uint32_t)S.k(I_CID, p);
+            cidS[p] = cid;
+            meta[p] = pack_meta(ok, lo[0], lo[1], lo[2], S.k(I_ORIG, p), cid);
         }
     }
     if (refC) atomic_max_wave(&sc->need_rebuild, moved);
@@ -1054,12 +1072,9 @@ __global__ void __launch_bounds__(BLOCK) k_posr(Soa S, const int* sorted, bool o
     v.w = S.f(F_CR, p);
     posr[t] = v;
     cidS[t] = (uint32_t)S.k(I_CID, p);
-    // cell and input row for the candidate-list test (one 16-byte load per
-    // partner); a component outside the hash range gets a cell no neighbour
-    // is adjacent to
     if (meta)
-        meta[t] = S.k(I_VALID, p) ? make_int4(S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p), S.k(I_ORIG, p))
-                                  : make_int4(INT_MIN / 2, INT_MIN / 2, INT_MIN / 2, S.k(I_ORIG, p));
+        meta[t] = pack_meta(S.k(I_VALID, p), S.k(I_CX, p), S.k(I_CY, p), S.k(I_CZ, p), S.k(I_ORIG, p),
+                            (uint32_t)S.k(I_CID, p));
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own

@@ -94,13 +94,19 @@ __global__ void __launch_bounds__(BLOCK) k_vbuild(Soa S, const double4* posr, co
     ncand[t] = min(count, CAND_MAX);
 }
 
-// The reference's candidate order of a contact of a component in cell
-// `mine` with one in cell `other` (at most one cell apart on each axis):
-// cell offset (dx, dy, dz) lexicographic, then input row.
-__device__ __forceinline__ uint32_t order_key(int4 mine, int4 other)
+// The reference's candidate order of a contact of a component in packed
+// cell `mine` (pack_meta) with the component whose record is `other`: cell
+// offset (dx, dy, dz) lexicographic, then input row; ~0u when `other` lies
+// outside the 27 cells around `mine` (no contact).
+__device__ __forceinline__ uint32_t order_key(unsigned long long mine, int4 other)
 {
-    const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
-    return ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) | (uint32_t)other.w;
+    constexpr unsigned long long M = (1ull << 21) - 1;
+    const unsigned long long c = meta_cell(other);
+    const int ddx = (int)((c >> 42) & M) - (int)((mine >> 42) & M);
+    const int ddy = (int)((c >> 21) & M) - (int)((mine >> 21) & M);
+    const int ddz = (int)(c & M) - (int)(mine & M);
+    if ((c >> 63) || ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) return ~0u;
+    return ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) | (uint32_t)other.z;
 }
 
 // History slots of a row's listed contacts (assign_batched for candidate-list
@@ -167,8 +173,10 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
     int found = 0, stored = 0, high = 0, over = 0;
     uint32_t last = 0;  // order key of the last contact in the list
     if (t < n) {
-        // history state of the row (expiry: slots not seen last step drop)
-        const long o = S.k(I_HIST, t);
+        // history state of the row (expiry: slots not seen last step drop);
+        // on list steps every row's history is its own row (rebuilds move
+        // the histories with the rows; differentiable runs move no row)
+        const long o = t;
         unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
         uint32_t k16[16];
         {
@@ -179,9 +187,9 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
                 k16[4 * i] = v.x; k16[4 * i + 1] = v.y; k16[4 * i + 2] = v.z; k16[4 * i + 3] = v.w;
             }
         }
-        if (S.k(I_VALID, t)) {
+        const unsigned long long mine = meta_cell(meta[t]);
+        if (!(mine >> 63)) {  // inside the hash range
             const double4 me = posr[t];
-            const int4 mine = meta[t];
             const int m = ncand[t];
             constexpr int VSCAN_BATCH = 4;
             for (int c0 = 0; c0 < m; c0 += VSCAN_BATCH) {
@@ -200,24 +208,19 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
                     const double rs = me.w + c.w;
                     hit[j] = c0 + j < m && !(d2 >= rs * rs || d2 <= 1.0e-28);
                 }
-                // cell records and ids of the overlapping candidates (the
-                // others read the row's own, which it holds already)
+                // cell records (with the component ids) of the overlapping
+                // candidates (the others read the row's own, which it read
+                // already)
                 int4 mq[VSCAN_BATCH];
-                uint32_t cq[VSCAN_BATCH];
 #pragma unroll
-                for (int j = 0; j < VSCAN_BATCH; ++j) {
-                    const long k = hit[j] ? kq[j] : t;
-                    mq[j] = meta[k];
-                    cq[j] = cidS[k];
-                }
+                for (int j = 0; j < VSCAN_BATCH; ++j) mq[j] = meta[hit[j] ? kq[j] : t];
 #pragma unroll
                 for (int j = 0; j < VSCAN_BATCH; ++j) {
                     if (!hit[j]) continue;
                     const long k = kq[j];
-                    const int4 other = mq[j];
-                    const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
-                    if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
-                    const uint32_t kk = order_key(mine, other);
+                    const uint32_t kk = order_key(mine, mq[j]);
+                    if (kk == ~0u) continue;  // beyond the 27 cells
+                    const uint32_t cid = (uint32_t)mq[j].w;
                     if (found >= MAX_LIST) {
                         ++found;
                         continue;
@@ -229,7 +232,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
                     // cell records
                     if (found == 0 || kk > last) {
                         nbr[(long)found * n + t] = (int)k;
-                        ncid[(long)found * n + t] = cq[j];
+                        ncid[(long)found * n + t] = cid;
                         last = kk;
                     } else {
                         int pos = found;
@@ -241,7 +244,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
                             --pos;
                         }
                         nbr[(long)pos * n + t] = (int)k;
-                        ncid[(long)pos * n + t] = cq[j];
+                        ncid[(long)pos * n + t] = cid;
                     }
                     ++found;
                 }
@@ -252,7 +255,6 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
             // order by selection (the next larger order key each time) while
             // the slots are acquired
             const double4 me = posr[t];
-            const int4 mine = meta[t];
             const int m = ncand[t];
             const int total = found;
             stored = assign_inline(S, t, t, ident_order(), cidS, H, step, nbr, occTape, high, over,
@@ -268,10 +270,8 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
                         const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
                         const double rs = me.w + o.w;
                         if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
-                        const int4 other = meta[k];
-                        const int ddx = other.x - mine.x, ddy = other.y - mine.y, ddz = other.z - mine.z;
-                        if (ddx < -1 || ddx > 1 || ddy < -1 || ddy > 1 || ddz < -1 || ddz > 1) continue;
-                        const uint32_t kk = order_key(mine, other);
+                        const uint32_t kk = order_key(mine, meta[k]);
+                        if (kk == ~0u) continue;  // beyond the 27 cells
                         if ((r == 0 || kk > last) && kk < best) {
                             best = kk;
                             bk = k;
