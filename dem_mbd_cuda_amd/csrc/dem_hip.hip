@@ -1166,6 +1166,8 @@ struct LawArgs {
     double finishDt, nextDt;  // k_law<.., true>: this step's finish and the next step's advance
     int* sceneAcc;            // k_law<.., true>: per-scene grid boxes reset for the next evaluation
     int nscenes;
+    bool identRows;  // sorted is the identity (candidate-list steps): k_law reads no index
+    bool oneScene;   // a single scene: its walls are all of W
 };
 
 // History lookup with the first 16 keys of the owner held in registers (kept
@@ -2215,16 +2217,22 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIV
     const long n = S.n;
     int bodyHigh = 0, pwOver = 0;
     if (t < n) {
-        const long p = A.sorted[t];
+        const long p = A.identRows ? t : (long)A.sorted[t];
         const double mass = S.f(F_MASS, p);
         V3 cF = zero3(), cT = zero3(), wF = zero3(), wT = zero3(), bF = zero3(), bT = zero3();
         const V3 fF = field_force<UNIVERSAL>(A, p, mass);
         // walls: the depth test of wall_pass on the packed centre and radius;
-        // the full contact record is read only by rows that touch a wall
+        // the full contact record is read only by rows that touch a wall (a
+        // single scene's walls are uniform across the grid, so the test waits
+        // on no per-row load but the centre)
         const int scene = S.k(I_SCENE, p);
-        const long hr = S.k(I_HIST, p);
+        const long hr = A.identRows ? p : (long)S.k(I_HIST, p);  // rows hold their own histories on list steps
         const double4 cr = A.posr[t];
-        const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
+        int w0 = 0, nw = A.W.n;
+        if (!A.oneScene) {
+            w0 = A.sp[scene].wall0;
+            nw = A.sp[scene].nwall;
+        }
         bool touch = false;
         for (int w = 0; w < nw; ++w) {
             const int g = w0 + w;
@@ -4234,6 +4242,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     A.g[0] = s->g[0]; A.g[1] = s->g[1]; A.g[2] = s->g[2];
     A.gravity = s->gravity; A.universal = s->universal; A.byorig = s->byorig;
     A.step = step; A.offs = s->offs; A.nb = nb; A.tape = s->tape; A.counters = counters;
+    A.identRows = ord == s->ident; A.oneScene = s->nscenes == 1;
     if (s->universal) hipLaunchKernelGGL(k_byorig, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->byorig);
     if (nb > 0) {
         const int fb = 1024;
