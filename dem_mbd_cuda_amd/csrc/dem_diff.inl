@@ -552,7 +552,7 @@ int diff_reorder(dem_solver* s)
     hipStream_t st = s->stream;
     Soa S = s->soa();
     hipLaunchKernelGGL(k_step_init, dim3(1), dim3(BLOCK), 0, st, s->sc, s->sceneAcc, s->nscenes);
-    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dSp, s->offs, s->sc, s->boxPart,
+    hipLaunchKernelGGL(k_cells, dim3(blocks_for(n)), dim3(BLOCK), 0, st, S, s->dSp, 0.0, s->offs, s->sc, s->boxPart,
                        s->partScene, s->sceneAcc, (const double*)nullptr, 0.0, (double4*)nullptr,
                        (uint32_t*)nullptr, (int4*)nullptr, (double*)nullptr);
     hipLaunchKernelGGL(k_grid_setup, dim3(s->nscenes), dim3(BLOCK), 0, st, s->sc, s->dSp, s->boxPart,

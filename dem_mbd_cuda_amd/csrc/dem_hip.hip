@@ -450,8 +450,9 @@ __device__ __forceinline__ unsigned long long meta_cell(int4 m)
 // one scene; a block that straddles scenes adds its lanes to the scenes'
 // accumulators with atomics.  k_grid_setup folds both.  With posr (rows in
 // cell order: sorted position = row) it also writes k_posr's outputs, which
-// a step without rebuild then uses as they are.
-__global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, bool offs, Scalars* sc,
+// a step without rebuild then uses as they are.  inv1: the inverse cell size
+// of a single scene (0 with several), so the cell waits on no other load.
+__global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, double inv1, bool offs, Scalars* sc,
                                                  int* part, int* partScene, int* sceneAcc, const double* refC,
                                                  double halfSkin2, double4* posr, uint32_t* cidS, int4* meta,
                                                  double* dispMax)
@@ -463,7 +464,7 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, b
     double disp2 = 0.0;
     if (p < S.n) {
         scene = S.k(I_SCENE, p);
-        const double inv = sp[scene].inv;
+        const double inv = inv1 > 0.0 ? inv1 : sp[scene].inv;
         double cx = S.f(F_X, p), cy = S.f(F_Y, p), cz = S.f(F_Z, p);
         if (offs) {
             const V3 o = rot(S.f(F_QW, p), S.f(F_QX, p), S.f(F_QY, p), S.f(F_QZ, p),
@@ -3561,7 +3562,7 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
             // the skin follows the motion unless DEM_SKIN fixes it (or DEM_SKIN_ADAPT=0)
             const char* a = getenv("DEM_SKIN_ADAPT");
             s->skinAdapt = !(a && atoi(a) == 0);
-            s->skinMin = 0.025 * rmin;
+            s->skinMin = 0.00625 * rmin;
             s->skinMax = 0.4 * rmin;
             for (int k = 0; k < nsc; ++k) s->skinMax = std::min(s->skinMax, 2.0 / s->hsp[k].inv - 2.0 * maxrScene[k]);
             s->skinMax = std::max(s->skinMax, s->skin);
@@ -4039,7 +4040,8 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     if (s->clump) hipLaunchKernelGGL(k_expand, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, R, s->cOwner);
     // a list step on current records gets k_posr's outputs from k_cells
     const bool fusedPosr = lists && s->recFresh;
-    hipLaunchKernelGGL(k_cells, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dSp, s->offs, s->sc,
+    hipLaunchKernelGGL(k_cells, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, R, s->dSp,
+                       s->nscenes == 1 ? s->hsp[0].inv : 0.0, s->offs, s->sc,
                        s->boxPart, s->partScene, s->sceneAcc, lists ? s->refC : nullptr,
                        0.25 * s->skin * s->skin, fusedPosr ? s->posr : nullptr, s->cidS, s->meta,
                        lists ? s->dispMax : nullptr);
