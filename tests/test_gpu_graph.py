@@ -73,8 +73,15 @@ def test_autotuned_run_equals_eager_steps(hip):
     b.close()
 
 
-def test_batch_graph_equals_eager(hip):
-    scenes = scene.sweep_scenes(count=4, seed=7, nmin=200, nmax=900)
+@pytest.mark.parametrize("skin", ["0.2", "0"])
+def test_batch_graph_equals_eager(hip, skin, monkeypatch):
+    """A batch of scenes with their own walls (the law kernel's fused pass
+    reads each row's scene walls) through multi-step runs, against
+    step-by-step calls."""
+    from test_gpu_batch import agitated_sweep
+
+    monkeypatch.setenv("DEM_SKIN", skin)
+    scenes = agitated_sweep(count=4, seed=7, nmin=200, nmax=900)
     batch = scene.SceneBatch(scenes)
     a, b = hip.solver(), hip.solver()
     for s in (a, b):
@@ -82,9 +89,15 @@ def test_batch_graph_equals_eager(hip):
         s.compute_forces(collect=True)
         s.init_half_step(0.5 * batch.combined.dt)
     dt = batch.combined.dt
-    a.run_steps(50, dt)
-    b.run_steps(50, dt, graph=False)
-    for x, y in zip(snapshot(a), snapshot(b)):
-        assert np.array_equal(x, y)
+    for chunk in (30, 20):
+        assert a.run_steps(chunk, dt) == chunk
+        for _ in range(chunk):
+            b.advance(dt)
+            b.compute_forces(collect=False)
+            b.finish(dt)
+        for x, y in zip(snapshot(a), snapshot(b)):
+            assert np.array_equal(x, y)
+    assert a.stats()["particle_contacts"] > 0
+    assert np.abs(a.forces()["wall"]).max() > 0.0
     a.close()
     b.close()
