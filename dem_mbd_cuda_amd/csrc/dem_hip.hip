@@ -2229,17 +2229,19 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIV
         const int scene = S.k(I_SCENE, p);
         const long hr = A.identRows ? p : (long)S.k(I_HIST, p);  // rows hold their own histories on list steps
         const double4 cr = A.posr[t];
-        int w0 = 0, nw = A.W.n;
-        if (!A.oneScene) {
-            w0 = A.sp[scene].wall0;
-            nw = A.sp[scene].nwall;
-        }
         bool touch = false;
-        for (int w = 0; w < nw; ++w) {
-            const int g = w0 + w;
+        auto test_wall = [&](int g) {
             const V3 outward = mk(A.W.nrm[3 * g], A.W.nrm[3 * g + 1], A.W.nrm[3 * g + 2]);
             const V3 origin = mk(A.W.o[3 * g], A.W.o[3 * g + 1], A.W.o[3 * g + 2]);
             touch |= cr.w - dot(sub(mk(cr.x, cr.y, cr.z), origin), outward) > 0.0;
+        };
+        if (A.oneScene) {
+            // wall index uniform across the wave: the planes come through
+            // the scalar cache
+            for (int g = 0; g < A.W.n; ++g) test_wall(g);
+        } else {
+            const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
+            for (int w = 0; w < nw; ++w) test_wall(w0 + w);
         }
         if (touch) {
             Body I = rec_body(A.rec, p);
