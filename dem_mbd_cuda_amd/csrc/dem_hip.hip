@@ -1768,8 +1768,9 @@ __device__ __forceinline__ V3 field_force(const LawArgs& A, long p, double mass)
 // contact:975-1063), after the row's wall-history expiry (expireHistory,
 // contact:217-229: slots not seen last step are dropped); adds to wF / wT.
 // TAPE: the differentiable run's records are compiled in (k_law's variant
-// that finishes the step never runs with a tape).
-template <bool TAPE = true>
+// that finishes the step never runs with a tape).  ONE: a single scene, whose
+// walls and wall law are the same for every lane (scalar registers).
+template <bool TAPE = true, bool ONE = false>
 __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long r, long hr, int scene, V3& wF,
                                           V3& wT, int& bodyHigh, int& pwOver)
 {
@@ -1780,8 +1781,8 @@ __device__ __forceinline__ void wall_pass(const LawArgs& A, const Body& I, long 
     unsigned pwCur = 0;
     if (TAPE && A.tape.pwOcc) A.tape.pwOcc[hr] = pwOcc;
     // walls of the row's scene (history key: the scene's wall index + 1)
-    const int w0 = A.sp[scene].wall0, nw = A.sp[scene].nwall;
-    Material mw = A.matPW[scene];
+    const int w0 = ONE ? 0 : A.sp[scene].wall0, nw = ONE ? A.W.n : A.sp[scene].nwall;
+    Material mw = A.matPW[ONE ? 0 : scene];
     if (A.first) mw.dt = 0.0;
     for (int w = 0; w < nw; ++w) {
         const int g = w0 + w;
@@ -2209,7 +2210,7 @@ __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3
 // nor its accelerations nor the half-step quaternion rate: the run's next
 // step rewrites all three before anything reads them.  Kept at 3 waves per
 // SIMD (at most 168 VGPRs) without universal gravitation.
-template <bool UNIVERSAL, bool FINISH>
+template <bool UNIVERSAL, bool FINISH, bool ONE = false>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIVERSAL ? 2 : 3))) k_law(LawArgs A)
 {
     if (FINISH && blockIdx.x == 0) reset_step_scalars(A.sc, A.sceneAcc, A.nscenes);
@@ -2235,7 +2236,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIV
             const V3 origin = mk(A.W.o[3 * g], A.W.o[3 * g + 1], A.W.o[3 * g + 2]);
             touch |= cr.w - dot(sub(mk(cr.x, cr.y, cr.z), origin), outward) > 0.0;
         };
-        if (A.oneScene) {
+        if (ONE || A.oneScene) {
             // wall index uniform across the wave: the planes come through
             // the scalar cache
             for (int g = 0; g < A.W.n; ++g) test_wall(g);
@@ -2246,7 +2247,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIV
         if (touch) {
             Body I = rec_body(A.rec, p);
             if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
-            wall_pass<!FINISH>(A, I, p, hr, scene, wF, wT, bodyHigh, pwOver);
+            wall_pass<!FINISH, ONE>(A, I, p, hr, scene, wF, wT, bodyHigh, pwOver);
         } else {
             // expiry only: no wall slot is seen this step
             const History& H = A.H;
@@ -4280,7 +4281,10 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             A.nextDt = s->nextDt;
             A.sceneAcc = s->sceneAcc;
             A.nscenes = s->nscenes;
-            hipLaunchKernelGGL((k_law<false, true>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+            if (s->nscenes == 1)
+                hipLaunchKernelGGL((k_law<false, true, true>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+            else
+                hipLaunchKernelGGL((k_law<false, true>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
             s->advancedAhead = true;
             s->scalarsReset = true;
             s->recFresh = true;
