@@ -463,8 +463,13 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Soa S, const SceneParams* sp, d
     int moved = 0;
     double disp2 = 0.0;
     if (p < S.n) {
-        scene = S.k(I_SCENE, p);
-        const double inv = inv1 > 0.0 ? inv1 : sp[scene].inv;
+        double inv = inv1;
+        if (inv1 > 0.0) {
+            scene = 0;
+        } else {
+            scene = S.k(I_SCENE, p);
+            inv = sp[scene].inv;
+        }
         double cx = S.f(F_X, p), cy = S.f(F_Y, p), cz = S.f(F_Z, p);
         if (offs) {
             const V3 o = rot(S.f(F_QW, p), S.f(F_QX, p), S.f(F_QY, p), S.f(F_QZ, p),
@@ -2227,7 +2232,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIV
         // the full contact record is read only by rows that touch a wall (a
         // single scene's walls are uniform across the grid, so the test waits
         // on no per-row load but the centre)
-        const int scene = S.k(I_SCENE, p);
+        const int scene = ONE ? 0 : S.k(I_SCENE, p);
         const long hr = A.identRows ? p : (long)S.k(I_HIST, p);  // rows hold their own histories on list steps
         const double4 cr = A.posr[t];
         bool touch = false;
