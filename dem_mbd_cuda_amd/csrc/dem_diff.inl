@@ -571,15 +571,25 @@ int diff_reorder(dem_solver* s)
     return launch_check("differentiable run row order");
 }
 
-int diff_step(dem_solver* s, double dt)
+// One step.  more: the run's next step follows with no checkpoint in
+// between, so the law kernel may finish this step and advance the next one
+// itself (the forward run without tapes of the pair law).
+int diff_step(dem_solver* s, double dt, bool more = false)
 {
     const long n = s->n;
     // the advance also writes the rows' contact records (runs take
-    // zero-offset single spheres), so the evaluation packs none
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, s->recs());
+    // zero-offset single spheres), so the evaluation packs none; the law
+    // kernel of the previous step may have advanced this one already
+    if (!s->advancedAhead)
+        hipLaunchKernelGGL(k_advance, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt, s->recs());
+    s->advancedAhead = false;
     s->recFresh = true;
-    TRY(dem_compute_forces(s, nullptr, nullptr, 0));
-    hipLaunchKernelGGL(k_finish, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
+    s->nextDt = more ? dt : 0.0;
+    const int rc = dem_compute_forces(s, nullptr, nullptr, 0);
+    s->nextDt = 0.0;
+    TRY(rc);
+    if (!s->advancedAhead)
+        hipLaunchKernelGGL(k_finish, dim3(blocks_for(n)), dim3(BLOCK), 0, s->stream, s->soa(), dt);
     return launch_check("differentiable step");
 }
 
@@ -658,7 +668,9 @@ int dem_diff_forward(dem_solver* s, int steps, int checkpoint_interval, int plat
                 rc = diff_high(s, R, ppW, pwW);
                 if (rc == DEM_OK) rc = save_checkpoint(s, R.cps[(step - 1) / R.K], ppW, pwW);
             }
-            if (rc == DEM_OK) rc = diff_step(s, dt);
+            // the step before a checkpoint (and the last) finishes on its own,
+            // so the checkpoint holds a finished state
+            if (rc == DEM_OK) rc = diff_step(s, dt, step < steps && step % R.K != 0);
             if (rc == DEM_OK) diff_count(s, R, step);
         }
     }

@@ -2214,11 +2214,15 @@ __device__ __forceinline__ void owner_out(const LawArgs& A, long p, double m, V3
 // it collects no diagnostics).  Such a step stores neither its force outputs
 // nor its accelerations nor the half-step quaternion rate: the run's next
 // step rewrites all three before anything reads them.  Kept at 3 waves per
-// SIMD (at most 168 VGPRs) without universal gravitation.
-template <bool UNIVERSAL, bool FINISH, bool ONE = false>
+// SIMD (at most 168 VGPRs) without universal gravitation.  ONE: a single
+// scene (walls and wall law wave-uniform).  COUNTED: the differentiable
+// forward run's fused pass, which keeps this evaluation's counters for the
+// host (the next evaluation resets them) and records the wall tape (contact
+// count and plate load).
+template <bool UNIVERSAL, bool FINISH, bool ONE = false, bool COUNTED = false>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIVERSAL ? 2 : 3))) k_law(LawArgs A)
 {
-    if (FINISH && blockIdx.x == 0) reset_step_scalars(A.sc, A.sceneAcc, A.nscenes);
+    if (FINISH && !COUNTED && blockIdx.x == 0) reset_step_scalars(A.sc, A.sceneAcc, A.nscenes);
     const long t = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     const Soa& S = A.S;
     const long n = S.n;
@@ -2252,12 +2256,12 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(UNIV
         if (touch) {
             Body I = rec_body(A.rec, p);
             if (A.offs) I.off = mk(S.f(F_OFX, p), S.f(F_OFY, p), S.f(F_OFZ, p));
-            wall_pass<!FINISH, ONE>(A, I, p, hr, scene, wF, wT, bodyHigh, pwOver);
+            wall_pass<!FINISH || COUNTED, ONE>(A, I, p, hr, scene, wF, wT, bodyHigh, pwOver);
         } else {
             // expiry only: no wall slot is seen this step
             const History& H = A.H;
             const unsigned occ = H.pwOcc[hr] & H.pwSeen[(long)((A.step + 1) & 1) * n + hr];
-            if (!FINISH && A.tape.pwOcc) A.tape.pwOcc[hr] = occ;
+            if ((!FINISH || COUNTED) && A.tape.pwOcc) A.tape.pwOcc[hr] = occ;
             H.pwOcc[hr] = occ;
             H.pwSeen[(long)(A.step & 1) * n + hr] = 0u;
         }
@@ -4276,9 +4280,12 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
         // a step of a multi-step run that another step follows finishes and
         // advances its rows in the law kernel (zero-offset single spheres,
         // no bodies, no diagnostics or tape, no universal gravitation)
-        const bool finishHere = s->nextDt > 0.0 && !collect && !s->countAlways && nb == 0 && !s->offs &&
-                                !s->universal && !s->tape.pair && !s->tape.pwOcc && !s->tape.wallCount &&
-                                !s->tape.plateAcc;
+        // (the differentiable forward run, which counts every evaluation and
+        // records the wall tape, takes the counted variant: it keeps the
+        // counters for the host and writes the tape)
+        const bool finishHere = s->nextDt > 0.0 && !collect && nb == 0 && !s->offs && !s->universal &&
+                                !s->tape.pair && !s->tape.pwOcc;
+        const bool counted = s->countAlways || s->tape.wallCount || s->tape.plateAcc;
         if (s->universal) {
             hipLaunchKernelGGL((k_law<true, false>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
         } else if (finishHere) {
@@ -4286,12 +4293,14 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             A.nextDt = s->nextDt;
             A.sceneAcc = s->sceneAcc;
             A.nscenes = s->nscenes;
-            if (s->nscenes == 1)
+            if (counted)
+                hipLaunchKernelGGL((k_law<false, true, false, true>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
+            else if (s->nscenes == 1)
                 hipLaunchKernelGGL((k_law<false, true, true>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
             else
                 hipLaunchKernelGGL((k_law<false, true>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
             s->advancedAhead = true;
-            s->scalarsReset = true;
+            s->scalarsReset = !counted;
             s->recFresh = true;
         } else {
             hipLaunchKernelGGL((k_law<false, false>), dim3(blocks_for(n)), dim3(BLOCK), 0, st, A);
