@@ -1178,37 +1178,34 @@ struct LawArgs {
 
 // History lookup with the first 16 keys of the owner held in registers (kept
 // in step with memory when a slot is reassigned): a hit among them needs no
-// dependent load.  Same result as acquire().
-__device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long long& cur,
-                                          uint32_t (&k16)[16], uint32_t* keys, double* vals, long vstride,
-                                          uint32_t key, int& high, int& overflow)
+// dependent load.  Same result as acquire().  find_pp is its lookup among the
+// occupied slots (-1: a new contact), new_pp its allocation of a new one.
+__device__ __forceinline__ int find_pp(unsigned long long occ, const uint32_t (&k16)[16], const uint32_t* keys,
+                                       uint32_t key)
 {
     unsigned match = 0;
 #pragma unroll
     for (int i = 0; i < 16; ++i) match |= (k16[i] == key ? 1u : 0u) << i;
     match &= (unsigned)(occ & 0xFFFFull);
-    int b = -1;
-    if (match) {
-        b = __ffs(match) - 1;
-    } else {
-        unsigned long long m = occ & ~0xFFFFull;
-        while (m) {
-            const int c = (int)__ffsll(m) - 1;
-            if (keys[c] == key) { b = c; break; }
-            m &= m - 1;
-        }
+    if (match) return __ffs(match) - 1;
+    unsigned long long m = occ & ~0xFFFFull;
+    while (m) {
+        const int c = (int)__ffsll(m) - 1;
+        if (keys[c] == key) return c;
+        m &= m - 1;
     }
-    if (b >= 0) {
-        cur |= 1ull << b;
-        high = max(high, b + 1);
-        return b;
-    }
+    return -1;
+}
+__device__ __forceinline__ int new_pp(unsigned long long& occ, unsigned long long& cur, uint32_t (&k16)[16],
+                                      uint32_t* keys, double* vals, long vstride, uint32_t key, int& high,
+                                      int& overflow)
+{
     const unsigned long long freem = ~occ;
     if (!freem) {
         ++overflow;
         return -1;
     }
-    b = (int)__ffsll(freem) - 1;
+    const int b = (int)__ffsll(freem) - 1;
     occ |= 1ull << b;
     cur |= 1ull << b;
     keys[b] = key;
@@ -1219,6 +1216,18 @@ __device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long
     for (int q = 0; q < 9; ++q) v[q] = 0.0;
     high = max(high, b + 1);
     return b;
+}
+__device__ __forceinline__ int acquire_pp(unsigned long long& occ, unsigned long long& cur,
+                                          uint32_t (&k16)[16], uint32_t* keys, double* vals, long vstride,
+                                          uint32_t key, int& high, int& overflow)
+{
+    const int b = find_pp(occ, k16, keys, key);
+    if (b >= 0) {
+        cur |= 1ull << b;
+        high = max(high, b + 1);
+        return b;
+    }
+    return new_pp(occ, cur, k16, keys, vals, vstride, key, high, overflow);
 }
 
 // History expiry at the start of the owner's step (expireHistory,
@@ -3051,7 +3060,6 @@ struct dem_solver {
     int* cand = nullptr;       // [CAND_MAX][n]
     int* ncand = nullptr;      // [n]
     int4* meta = nullptr;      // [n] cell and input row in cell order
-    uint32_t* ncid = nullptr;  // [MAX_LIST][n] component id of each contact the list scan found
     int* hFlag = nullptr;      // pinned: displacement / overflow flags
     int* hMirror = nullptr;    // pinned, coherent: [seq, need_rebuild, grid_overflow] written by k_vscan
     int* dMirror = nullptr;    // device address of hMirror
@@ -3201,7 +3209,7 @@ struct dem_solver {
         diff_release(diff);
         graph_release(graphs);
         void* ptrs[] = {dState[0], dState[1], iState[0], iState[1], sc, keyA, keyB, valA, valB, ident, mKey, mVal,
-                        rec, recAux, refC, cand, ncand, meta, ncid, dispMax, dDisp, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
+                        rec, recAux, refC, cand, ncand, meta, dispMax, dDisp, rsCounts, rsDigits, cellStart, boxPart, partScene, sceneAcc, gridDone, dSp, dGrid,
                         dMatPP, dMatPW, posr, cidS, nbr, ncount, listOff, listBlk, pairs, pairF, rowF, byorig, H.ppOcc, H.ppSeen, H.ppKey,
                         H.ppVal, H.pwOcc, H.pwSeen, H.pwKey, H.pwVal, H.ptOcc, H.ptSeen, H.ptKey,
                         H.ptVal, H2.ppOcc, H2.ppSeen, H2.ppKey, H2.ppVal, H2.pwOcc, H2.pwSeen, H2.pwKey,
@@ -3584,7 +3592,6 @@ int initialize_impl(dem_solver* s, const dem_particles_view* P, const dem_compon
             TRY(dalloc(s->cand, (size_t)CAND_MAX * n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->ncand, n, "hipMalloc(candidate lists)"));
             TRY(dalloc(s->meta, n, "hipMalloc(candidate lists)"));
-            TRY(dalloc(s->ncid, (size_t)MAX_LIST * n, "hipMalloc(candidate lists)"));
             if (s->skinAdapt) {
                 TRY(dalloc(s->dispMax, blocks_for(n), "hipMalloc(candidate lists)"));
                 TRY(dalloc(s->dDisp, 1, "hipMalloc(candidate lists)"));
@@ -4073,7 +4080,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             if (fusedPosr) {
                 HIP_TRY(hipEventRecord(s->flagEv, st), "event");
                 if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
-                hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ncid, s->posr,
+                hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->posr,
                                    s->meta, s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr,
                                    s->ncount, s->tape.ppOcc, collect || s->countAlways, s->listBlk,
                                    (const int*)&s->sc->need_rebuild, (int*)nullptr, 0);
@@ -4093,7 +4100,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
             const int* skip = &s->sc->need_rebuild;
             const int seq = ++s->mirrorSeq;
             if (timed) HIP_TRY(hipEventRecord(s->ev[1], st), "event");
-            hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->ncid, s->posr, s->meta,
+            hipLaunchKernelGGL(k_vscan, dim3(blocks_for(nr)), dim3(BLOCK), 0, st, S, s->posr, s->meta,
                                s->cidS, s->cand, s->ncand, s->sc, s->H, s->stepIndex, s->nbr, s->ncount,
                                s->tape.ppOcc, collect || s->countAlways, s->listBlk, skip, s->dMirror, seq);
             TRY(launch_check("contact scan"));
@@ -4220,7 +4227,7 @@ static int compute_forces_impl(dem_solver* s, const double* pose13, double* load
     if (scanned)
         ;
     else if (useLists)
-        hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->ncid, s->posr, s->meta, s->cidS,
+        hipLaunchKernelGGL(k_vscan, dim3(nblk), dim3(BLOCK), 0, st, S, s->posr, s->meta, s->cidS,
                            s->cand, s->ncand, s->sc, s->H, step, s->nbr, s->ncount, s->tape.ppOcc,
                            counters, s->listBlk, (const int*)nullptr, (int*)nullptr, 0);
     else if (s->clump)

@@ -109,51 +109,44 @@ __device__ __forceinline__ uint32_t order_key(unsigned long long mine, int4 othe
     return ((uint32_t)(((ddx + 1) * 3 + (ddy + 1)) * 3 + (ddz + 1)) << 25) | (uint32_t)other.z;
 }
 
-// History slots of a row's listed contacts (assign_batched for candidate-list
-// steps, where rows are their own sorted positions): nbr[c][t] and
-// cid[c][t] hold the c-th contact's row and component id in the reference's
-// order; on return nbr[c][t] = (row << SLOT_BITS | slot) for the contacts
-// that got a slot.  The row's occupancy, seen mask and first 16 keys are
-// read by the caller before its scan (occ, seen, k16), and the entries are
-// read four at a time, so no lookup waits on another.
-__device__ __forceinline__ int assign_listed(long t, long n, long o, int found, unsigned long long occ,
-                                             uint32_t (&k16)[16], const History& H, int step, int* nbr,
-                                             const uint32_t* cid, int& high, int& over)
+// List entries of a row during the list scan: (partner row << SLOT_BITS |
+// slot) for a contact that holds a history slot already, ~(partner row) for a
+// new contact, which takes its slot after the scan.
+__device__ __forceinline__ long entry_row(int e) { return e >= 0 ? (long)(e >> SLOT_BITS) : (long)~e; }
+
+// The new contacts of a row's list take their slots in list order
+// (acquireHistorySlot, contact:190-215: lookups of existing contacts do not
+// change the occupancy, so taking the new ones after the others is the same
+// sequence of allocations); a contact left without one (all 64 taken) is
+// dropped, as the reference drops it.  Returns the entries kept.
+__device__ __forceinline__ int assign_new(long t, long n, int found, const int4* meta, unsigned long long& occ,
+                                          unsigned long long& cur, uint32_t (&k16)[16], uint32_t* keys,
+                                          double* vals, long vstride, int* nbr, int& high, int& over)
 {
-    unsigned long long cur = 0;
-    uint32_t* keys = H.ppKey + o * PP_SLOTS;
-    double* vals = pp_slot(H, o, 0);
-    int stored = 0;
-    for (int c0 = 0; c0 < found; c0 += 4) {
-        int kq[4];
-        uint32_t cq[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long cc = min(c0 + j, found - 1);
-            kq[j] = nbr[cc * n + t];
-            cq[j] = cid[cc * n + t];
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (c0 + j >= found) break;
-            const int slot = acquire_pp(occ, cur, k16, keys, vals, H.rows * 9, cq[j] + 1u, high, over);
+    int kept = 0;
+    for (int c = 0; c < found; ++c) {
+        int e = nbr[(long)c * n + t];
+        if (e < 0) {
+            const long k = ~e;
+            const int slot = new_pp(occ, cur, k16, keys, vals, vstride, (uint32_t)meta[k].w + 1u, high, over);
             if (slot < 0) continue;  // slot overflow: counted, no force
-            nbr[(long)stored * n + t] = (kq[j] << SLOT_BITS) | slot;
-            ++stored;
+            e = ((int)k << SLOT_BITS) | slot;
         }
+        nbr[(long)kept * n + t] = e;
+        ++kept;
     }
-    H.ppOcc[o] = occ;
-    H.ppSeen[(long)(step & 1) * n + o] = cur;
-    return stored;
+    return kept;
 }
 
 // Step between rebuilds: exact contact test over the listed partners, the
 // reference's candidate order, then history slots.  Rows are their own
 // sorted positions on these steps.  Per batch of VSCAN_BATCH candidates the
 // indices, then the centres, then (for the overlapping ones) the cell
-// records and component ids are fetched together; the row's history state
-// is fetched before the scan.
-__global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const double4* posr, const int4* meta,
+// records with the component ids are fetched together; the row's history
+// state is fetched before the scan.  A contact that holds a slot finds it as
+// the scan meets it (the lookup does not depend on the order); new contacts
+// take theirs after the scan, in list order (assign_new).
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_vscan(Soa S, const double4* posr, const int4* meta,
                                                  const uint32_t* cidS, const int* cand, const int* ncand,
                                                  Scalars* sc, History H, int step, int* nbr,
                                                  int* ncount, unsigned long long* occTape, bool counters,
@@ -170,7 +163,7 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
     const long b = xcd_block(blockIdx.x, gridDim.x);
     const long t = b * BLOCK + threadIdx.x;
     const long n = S.n;
-    int found = 0, stored = 0, high = 0, over = 0;
+    int found = 0, stored = 0, high = 0, over = 0, fresh = 0;
     uint32_t last = 0;  // order key of the last contact in the list
     if (t < n) {
         // history state of the row (expiry: slots not seen last step drop);
@@ -178,9 +171,11 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
         // the histories with the rows; differentiable runs move no row)
         const long o = t;
         unsigned long long occ = H.ppOcc[o] & H.ppSeen[(long)((step + 1) & 1) * n + o];
+        unsigned long long cur = 0;  // slots seen this step
+        uint32_t* keys = H.ppKey + o * PP_SLOTS;
         uint32_t k16[16];
         {
-            const uint4* kv = reinterpret_cast<const uint4*>(H.ppKey + o * PP_SLOTS);
+            const uint4* kv = reinterpret_cast<const uint4*>(keys);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const uint4 v = kv[i];
@@ -220,10 +215,18 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
                     const long k = kq[j];
                     const uint32_t kk = order_key(mine, mq[j]);
                     if (kk == ~0u) continue;  // beyond the 27 cells
-                    const uint32_t cid = (uint32_t)mq[j].w;
                     if (found >= MAX_LIST) {
                         ++found;
                         continue;
+                    }
+                    const int slot = find_pp(occ, k16, keys, (uint32_t)mq[j].w + 1u);
+                    int e = ~(int)k;  // a new contact
+                    if (slot >= 0) {
+                        e = ((int)k << SLOT_BITS) | slot;
+                        cur |= 1ull << slot;
+                        high = max(high, slot + 1);
+                    } else {
+                        ++fresh;
                     }
                     // the contacts in order: candidate lists keep the order of
                     // the rebuild, so a contact almost always comes after the
@@ -231,20 +234,17 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
                     // the keys of the entries it passes recomputed from their
                     // cell records
                     if (found == 0 || kk > last) {
-                        nbr[(long)found * n + t] = (int)k;
-                        ncid[(long)found * n + t] = cid;
+                        nbr[(long)found * n + t] = e;
                         last = kk;
                     } else {
                         int pos = found;
                         while (pos > 0) {
-                            const int kp = nbr[(long)(pos - 1) * n + t];
-                            if (order_key(mine, meta[kp]) <= kk) break;
-                            nbr[(long)pos * n + t] = kp;
-                            ncid[(long)pos * n + t] = ncid[(long)(pos - 1) * n + t];
+                            const int ep = nbr[(long)(pos - 1) * n + t];
+                            if (order_key(mine, meta[entry_row(ep)]) <= kk) break;
+                            nbr[(long)pos * n + t] = ep;
                             --pos;
                         }
-                        nbr[(long)pos * n + t] = (int)k;
-                        ncid[(long)pos * n + t] = cid;
+                        nbr[(long)pos * n + t] = e;
                     }
                     ++found;
                 }
@@ -284,7 +284,11 @@ __global__ void __launch_bounds__(BLOCK) k_vscan(Soa S, uint32_t* ncid, const do
             });
         } else {
             if (occTape) occTape[o] = occ;
-            stored = assign_listed(t, n, o, found, occ, k16, H, step, nbr, ncid, high, over);
+            stored = fresh ? assign_new(t, n, found, meta, occ, cur, k16, keys, pp_slot(H, o, 0), H.rows * 9, nbr,
+                                        high, over)
+                           : found;
+            H.ppOcc[o] = occ;
+            H.ppSeen[(long)(step & 1) * n + o] = cur;
         }
         ncount[t] = stored;
     }
