@@ -1488,30 +1488,53 @@ __global__ void __launch_bounds__(BLOCK) k_scan(Soa S, const int* sorted, const 
             const long xx = cx + dx;
             if (xx >= 0 && xx < X) {
                 const long zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < Z ? cz + 1 : Z - 1;
+                // the pass's three column ranges, read together (empty off the grid)
+                long rb[3], re[3];
                 for (int dy = -1; dy <= 1; ++dy) {
                     const long yy = cy + dy;
-                    if (yy < 0 || yy >= Y) continue;
-                    const unsigned long long c0 = base + ((unsigned long long)xx * Y + yy) * Z;
-                    long b, e;
-                    if (dense) {
-                        b = start[c0 + zlo];
-                        e = start[c0 + zhi + 1];
-                    } else {
-                        b = lower_bound_u64(key, n, c0 + zlo);
-                        e = lower_bound_u64(key, n, c0 + zhi + 1);
+                    long b = 0, e = 0;
+                    if (yy >= 0 && yy < Y) {
+                        const unsigned long long c0 = base + ((unsigned long long)xx * Y + yy) * Z;
+                        if (dense) {
+                            b = start[c0 + zlo];
+                            e = start[c0 + zhi + 1];
+                        } else {
+                            b = lower_bound_u64(key, n, c0 + zlo);
+                            e = lower_bound_u64(key, n, c0 + zhi + 1);
+                        }
                     }
-                    const int r = seg * 3 + dy + 1;
-                    const bool fromLds = useLds && staged >= 0;
-                    const long shift = fromLds ? (long)rngOff[r] - rngB[r] : 0;
-                    for (long k = b; k < e; ++k) {
-                        if (CLUMP ? (fromLds ? stageOwn[k + shift] : ownS[k]) == mine : k == t) continue;
-                        const double4 c = fromLds ? stage[k + shift] : posr[k];
+                    rb[dy + 1] = b;
+                    re[dy + 1] = e;
+                }
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const long b = rb[dy + 1], e = re[dy + 1];
+                    if (b >= e) continue;
+                    // the overlap test of one candidate, in the reference's order
+                    auto hit = [&](const double4& c) {
                         const double dx_ = c.x - me.x, dy_ = c.y - me.y, dz_ = c.z - me.z;
                         const double d2 = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;
                         const double rs = me.w + c.w;
-                        if (d2 >= rs * rs || d2 <= 1.0e-28) continue;
-                        if (found < MAX_LIST) nbr[(long)found * n + t] = (int)k;
-                        ++found;
+                        return !(d2 >= rs * rs || d2 <= 1.0e-28);
+                    };
+                    if (useLds && staged >= 0) {
+                        // staged range: 32-bit stage indices, sorted row = i - shift
+                        const int r = seg * 3 + dy + 1;
+                        const int shift = rngOff[r] - (int)rngB[r];
+                        // (the row itself fails the overlap test: its distance is 0)
+                        #pragma unroll 2
+                        for (int i = (int)b + shift, ie = (int)e + shift; i < ie; ++i) {
+                            if (CLUMP && stageOwn[i] == mine) continue;
+                            if (!hit(stage[i])) continue;
+                            if (found < MAX_LIST) nbr[(long)found * n + t] = i - shift;
+                            ++found;
+                        }
+                    } else {
+                        for (long k = b; k < e; ++k) {
+                            if (CLUMP && ownS[k] == mine) continue;
+                            if (!hit(posr[k])) continue;
+                            if (found < MAX_LIST) nbr[(long)found * n + t] = (int)k;
+                            ++found;
+                        }
                     }
                 }
             }
